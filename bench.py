@@ -1,0 +1,173 @@
+"""Benchmark: information Gbit/s of batched LDPC decoding on one MI355X.
+
+Workload: the N=64800, K=32400 (rate 1/2) QC-IRA code with Z=360 and the
+DVB-T2 normal-1/2 degree profile (90x180 base graph, E=226,440 edges),
+4096 frames per batch of BPSK/AWGN channel LLRs (FP32) at the Eb/N0 where
+plain min-sum BP reaches FER about 1e-3, flooding min-sum, at most 50
+iterations with syndrome early stop.  A step decodes one whole batch whose
+LLRs are already resident in HBM; outputs stay in HBM.
+
+Prints one JSON line (see the keys below), including
+  roofline      achieved HBM bandwidth of the dominant kernel, from HIP events
+                recorded by the library around every launch in the timed
+                region, against the 8 TB/s HBM3E peak
+  cpu_baseline  the structural C++ restatement of the reference's rayon
+                decode_batch (oracle/ref_port.cpp) on a bounded sample of the
+                same frames, on this host's cores
+
+    python bench.py [--steps K] [--warmup W] [--frames F] [--ebn0 X]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+# Plain min-sum FER on this code: about 1e-3 at this Eb/N0 (gf2_amd.fer sweep).
+DEFAULT_EBN0 = 2.25
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=4096)
+    ap.add_argument("--ebn0", type=float, default=DEFAULT_EBN0)
+    ap.add_argument("--max-it", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=7)
+    return ap.parse_args()
+
+
+def cpu_baseline(spec, base, llr, max_it, seconds):
+    """Time the structural restatement of the rayon path on a bounded sample."""
+    from oracle import oracle as orc
+    threads = max(1, min(16, os.cpu_count() or 1))
+    oc = orc.Code.from_qc(base, spec.z)
+    done = 0
+    t0 = time.perf_counter()
+    chunk = threads
+    while time.perf_counter() - t0 < seconds and done + chunk <= llr.shape[0]:
+        oc.port_decode_batch(llr[done:done + chunk], max_it, threads)
+        done += chunk
+    dt = time.perf_counter() - t0
+    return {"value": done * spec.k / dt / 1e9, "unit": "Gbit/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{done} frames of the same batch (first {done}), {dt:.1f} s, "
+                      f"oracle/ref_port.cpp with {threads} worker threads"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(frames shard across GPUs with no exchange, see DESIGN.md)"}))
+        return 2
+    import torch
+    from gf2_amd import _lib, channel, codes
+    from gf2_amd.ldpc import Context, LdpcCode
+
+    spec = codes.n64800_r12()
+    base = spec.base_matrix()
+    info, _, llr = channel.random_frames(base, spec.z, a.frames, a.ebn0, seed=a.seed)
+    ctx = Context(0)
+    code = LdpcCode(spec.qc(), ctx)
+    lib = _lib.load()
+    F, kw = a.frames, (spec.k + 63) // 64
+    d_llr = torch.from_numpy(llr).cuda()
+    d_msg = torch.empty((F, kw), dtype=torch.int64, device="cuda")
+    d_it = torch.empty(F, dtype=torch.int32, device="cuda")
+    d_conv = torch.empty(F, dtype=torch.uint8, device="cuda")
+    d_synd = torch.empty(F, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+
+    def step():
+        _lib.check(ctx.handle, lib.gf2l_decode_batch_device(
+            ctx.handle, code.handle, d_llr.data_ptr(), F, a.max_it, d_msg.data_ptr(),
+            d_it.data_ptr(), d_conv.data_ptr(), d_synd.data_ptr()))
+        ctx.synchronize()
+
+    for _ in range(a.warmup):
+        step()
+    _lib.check(ctx.handle, lib.gf2l_profile_enable(ctx.handle, 1))
+    torch.cuda.synchronize()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+
+    ms = (np.ctypeslib.ctypes.c_double * 3)()
+    launches = (np.ctypeslib.ctypes.c_uint64 * 3)()
+    fi = np.ctypeslib.ctypes.c_uint64()
+    lib.gf2l_profile_read(ctx.handle, ms, launches, np.ctypeslib.ctypes.byref(fi))
+    lib.gf2l_profile_enable(ctx.handle, 0)
+
+    msg = d_msg.cpu().numpy().view(np.uint64)
+    it = d_it.cpu().numpy()
+    conv = d_conv.cpu().numpy().astype(bool)
+    bits = channel.unpack_words(msg, spec.k)
+    fe = int(((~conv) | (bits != info).any(axis=1)).sum())
+
+    E, N = code.nnz(), spec.n
+    # algorithmic bytes per frame-iteration of each kernel
+    per_fi = {"check": 12 * E,                   # read belief + old c2v, write c2v
+              "variable": 4 * E + 8 * N + N // 8,  # read c2v + LLR, write belief + hard bits
+              "syndrome": N // 8}                  # read hard bits
+    names = ["check", "variable", "syndrome"]
+    dom = max(range(3), key=lambda k: ms[k])
+    frame_iters = fi.value
+    achieved = per_fi[names[dom]] * frame_iters / (ms[dom] * 1e-3) / 1e9
+    value = a.steps * F * spec.k / dt / 1e9
+    out = {
+        "metric": "info Gbit/s on 1 MI355X at FER 1e-3, learned vs plain BP, N=64800 r=1/2",
+        "value": round(value, 4),
+        "unit": "Gbit/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(dt / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded info bits, staircase encoder, BPSK/AWGN LLRs (numpy PCG64)",
+        "config": {
+            "workload": "plain min-sum BP decode, QC-IRA N=64800 K=32400 Z=360, 4096 frames/batch",
+            "decoder": "plain flooding min-sum (learned NMS path not yet in this line)",
+            "code": spec.name, "E": E, "frames": F, "ebn0_db": a.ebn0, "max_iterations": a.max_it,
+            "mean_iterations": round(float(it.mean()), 3), "fer": fe / F,
+            "frame_errors": fe,
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": f"cn_stream_kernel" if dom == 0 else
+            ("vn_stream_kernel" if dom == 1 else "syn_stream_kernel"),
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+            "avg_launch_us": round(ms[dom] / launches[dom] * 1e3, 2),
+            "launches": int(launches[dom]),
+            "bytes_per_frame_iteration": per_fi[names[dom]],
+            "kernel_ms": {names[k]: round(ms[k], 3) for k in range(3)},
+        },
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(spec, base, llr, a.max_it, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,103 @@
+"""ctypes binding of libgf2ldpc.so (the HIP decoder's C ABI, include/gf2ldpc.h).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C gf2_amd/csrc``).  There is no CPU fallback: when the library is
+missing, or when no HIP device is present, every decode call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgf2ldpc.so")
+
+GF2L_OK = 0
+GF2L_ERR_INVALID = -1
+GF2L_ERR_HIP = -2
+GF2L_ERR_NOMEM = -3
+GF2L_ERR_UNSUPPORTED = -4
+
+PATH_AUTO = 0
+PATH_LDS = 1
+PATH_STREAM = 2
+
+_c_void_pp = ctypes.POINTER(ctypes.c_void_p)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+# name -> (restype, argtypes); every symbol include/gf2ldpc.h declares.
+SIGNATURES = {
+    "gf2l_create": (ctypes.c_int, [ctypes.c_int, _c_void_pp]),
+    "gf2l_destroy": (None, [ctypes.c_void_p]),
+    "gf2l_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "gf2l_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "gf2l_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "gf2l_set_path": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "gf2l_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "gf2l_profile_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p]),
+    "gf2l_code_qc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                    ctypes.c_uint32, _i32p, _c_void_pp]),
+    "gf2l_code_info": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p, _u32p, _u64p]),
+    "gf2l_code_destroy": (None, [ctypes.c_void_p]),
+    "gf2l_qc_ira_shifts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.c_uint32, _u32p, _i32p]),
+    "gf2l_decode_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_size_t, ctypes.c_uint32, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "gf2l_decode_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_size_t,
+                                                ctypes.c_uint32, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p]),
+    "gf2l_decode_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class Gf2lError(RuntimeError):
+    """A libgf2ldpc call returned a negative status."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"gf2ldpc error {code}: {message}")
+        self.code = code
+        self.message = message
+
+
+def load() -> ctypes.CDLL:
+    """Loads libgf2ldpc.so once; raises if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                    "(the HIP decoder has no CPU fallback)")
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+        return _lib
+
+
+def check(ctx, rc: int) -> None:
+    if rc != GF2L_OK:
+        msg = load().gf2l_last_error(ctx).decode() if ctx else ""
+        raise Gf2lError(rc, msg)
+
+
+def ptr(arr, ctype):
+    """ctypes pointer to a contiguous numpy array."""
+    return arr.ctypes.data_as(ctypes.POINTER(ctype))

@@ -1,0 +1,419 @@
+// C ABI of libgf2ldpc: contexts, quasi-cyclic codes and the batch decoder.
+// Declarations and the reference interfaces each call replaces are in
+// include/gf2ldpc.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gf2ldpc.h"
+#include "internal.h"
+
+namespace {
+
+int fail(gf2l_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define GF2L_HIP(ctx, expr)                                                                   \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? GF2L_ERR_NOMEM : GF2L_ERR_HIP,     \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+void free_workspace(gf2l::Workspace &ws) {
+    hipFree(ws.c2v);
+    hipFree(ws.belief);
+    hipFree(ws.hb);
+    hipFree(ws.lists);
+    hipFree(ws.counts);
+    if (ws.h_count) hipHostFree(ws.h_count);
+    ws = gf2l::Workspace{};
+}
+
+// Grows the streaming workspace to hold `frames` frames of `code`.
+int ensure_workspace(gf2l_ctx *ctx, const gf2l::CodeView &cv, size_t frames) {
+    gf2l::Workspace &ws = ctx->ws;
+    const size_t E = (size_t)cv.nbe * cv.z, hbw = (size_t)cv.nb * cv.zw;
+    if (ws.c2v && ws.cap_frames >= frames && ws.cap_e >= E * frames &&
+        ws.cap_n >= (size_t)cv.n * frames && ws.cap_hb >= hbw * frames)
+        return GF2L_OK;
+    const size_t f = std::max(frames, ws.cap_frames);
+    const size_t ce = std::max(E * f, ws.cap_e), cn = std::max((size_t)cv.n * f, ws.cap_n),
+                 ch = std::max(hbw * f, ws.cap_hb);
+    free_workspace(ws);
+    GF2L_HIP(ctx, hipMalloc(&ws.c2v, ce * sizeof(float)));
+    GF2L_HIP(ctx, hipMalloc(&ws.belief, cn * sizeof(float)));
+    GF2L_HIP(ctx, hipMalloc(&ws.hb, ch * sizeof(uint64_t)));
+    GF2L_HIP(ctx, hipMalloc(&ws.lists, 2 * f * sizeof(uint32_t)));
+    GF2L_HIP(ctx, hipMalloc(&ws.counts, 2 * sizeof(uint32_t)));
+    GF2L_HIP(ctx, hipHostMalloc((void **)&ws.h_count, sizeof(uint32_t), hipHostMallocDefault));
+    ws.cap_frames = f;
+    ws.cap_e = ce;
+    ws.cap_n = cn;
+    ws.cap_hb = ch;
+    return GF2L_OK;
+}
+
+bool use_lds_path(const gf2l_ctx *ctx, const gf2l::CodeView &cv) {
+    const size_t bytes = gf2l::lds_path_bytes(cv);
+    if (ctx->path == GF2L_PATH_STREAM) return false;
+    if (ctx->path == GF2L_PATH_LDS) return bytes <= ctx->lds_limit;
+    // automatic: keep at least two frames resident per CU
+    return bytes <= std::min<size_t>(ctx->lds_limit, 80 * 1024);
+}
+
+// Runs the decoder on device buffers.  trace_* (host) and trace_iters are
+// only used by gf2l_decode_trace.
+int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size_t frames,
+                  uint32_t max_it, uint64_t *d_msg, uint32_t *d_iters, uint8_t *d_conv,
+                  uint8_t *d_synd, float *h_trace_c2v, float *h_trace_bel, uint32_t trace_iters) {
+    const gf2l::CodeView &cv = code->view;
+    if (frames == 0) return GF2L_OK;
+    if (frames > 0xffffffffull) return fail(ctx, GF2L_ERR_INVALID, "too many frames in one batch");
+    const uint32_t F = (uint32_t)frames;
+    const size_t kw = ((size_t)cv.k + 63) / 64;
+    const size_t E = (size_t)cv.nbe * cv.z;
+    hipStream_t s = ctx->stream;
+    if (max_it == 0) {  // fresh decoder: zero beliefs, no iterations (core.rs:906-937)
+        GF2L_HIP(ctx, hipMemsetAsync(d_msg, 0, F * kw * sizeof(uint64_t), s));
+        GF2L_HIP(ctx, hipMemsetAsync(d_iters, 0, F * sizeof(uint32_t), s));
+        GF2L_HIP(ctx, hipMemsetAsync(d_conv, 0, F, s));
+        GF2L_HIP(ctx, hipMemsetAsync(d_synd, 1, F, s));
+        return GF2L_OK;
+    }
+    if (use_lds_path(ctx, cv)) {
+        float *d_tc = nullptr, *d_tb = nullptr;
+        if (h_trace_c2v && trace_iters) {
+            GF2L_HIP(ctx, hipMalloc(&d_tc, (size_t)trace_iters * F * E * sizeof(float)));
+            GF2L_HIP(ctx, hipMalloc(&d_tb, (size_t)trace_iters * F * cv.n * sizeof(float)));
+        }
+        GF2L_HIP(ctx, gf2l::launch_lds_decode(cv, d_llr, F, max_it, nullptr, 0, d_msg, d_iters,
+                                              d_conv, d_synd, d_tc, d_tb, trace_iters, s));
+        if (d_tc) {
+            GF2L_HIP(ctx, hipMemcpyAsync(h_trace_c2v, d_tc, (size_t)trace_iters * F * E * sizeof(float),
+                                         hipMemcpyDeviceToHost, s));
+            GF2L_HIP(ctx, hipMemcpyAsync(h_trace_bel, d_tb,
+                                         (size_t)trace_iters * F * cv.n * sizeof(float),
+                                         hipMemcpyDeviceToHost, s));
+            GF2L_HIP(ctx, hipStreamSynchronize(s));
+            hipFree(d_tc);
+            hipFree(d_tb);
+        }
+        return GF2L_OK;
+    }
+    int rc = ensure_workspace(ctx, cv, F);
+    if (rc != GF2L_OK) return rc;
+    gf2l::Workspace &ws = ctx->ws;
+    gf2l::Profile &pf = ctx->prof;
+    GF2L_HIP(ctx, gf2l::launch_stream_init(cv, ws, d_llr, F, s));
+    uint32_t running = F;  // frames entering the current iteration
+    auto collect = [&]() -> int {  // stream already synchronized
+        for (int k = 0; k < 3; ++k) {
+            float ms = 0.0f;
+            GF2L_HIP(ctx, hipEventElapsedTime(&ms, pf.ev[k], pf.ev[k + 1]));
+            pf.ms[k] += ms;
+            pf.launches[k] += 1;
+        }
+        pf.frame_iters += running;
+        return GF2L_OK;
+    };
+    for (uint32_t it = 1; it <= max_it; ++it) {
+        GF2L_HIP(ctx, gf2l::launch_stream_iteration(cv, ws, d_llr, F, it, max_it, nullptr, d_msg,
+                                                    d_iters, d_conv, d_synd, ctx->num_cus, s,
+                                                    pf.on ? pf.ev : nullptr));
+        if (pf.on && it == max_it) {
+            GF2L_HIP(ctx, hipStreamSynchronize(s));
+            if ((rc = collect()) != GF2L_OK) return rc;
+        }
+        if (h_trace_c2v && it <= trace_iters) {
+            GF2L_HIP(ctx, hipMemcpyAsync(h_trace_c2v + (size_t)(it - 1) * F * E, ws.c2v,
+                                         (size_t)F * E * sizeof(float), hipMemcpyDeviceToHost, s));
+            GF2L_HIP(ctx, hipMemcpyAsync(h_trace_bel + (size_t)(it - 1) * F * cv.n, ws.belief,
+                                         (size_t)F * cv.n * sizeof(float), hipMemcpyDeviceToHost, s));
+        }
+        if (it < max_it) {  // stop launching once every frame has retired
+            GF2L_HIP(ctx, hipMemcpyAsync(ws.h_count, ws.counts + (it & 1u), sizeof(uint32_t),
+                                         hipMemcpyDeviceToHost, s));
+            GF2L_HIP(ctx, hipStreamSynchronize(s));
+            if (pf.on && (rc = collect()) != GF2L_OK) return rc;
+            running = *ws.h_count;
+            if (*ws.h_count == 0) {
+                if (h_trace_c2v)
+                    for (uint32_t t = it + 1; t <= trace_iters; ++t) {
+                        std::memcpy(h_trace_c2v + (size_t)(t - 1) * F * E,
+                                    h_trace_c2v + (size_t)(it - 1) * F * E, (size_t)F * E * sizeof(float));
+                        std::memcpy(h_trace_bel + (size_t)(t - 1) * F * cv.n,
+                                    h_trace_bel + (size_t)(it - 1) * F * cv.n,
+                                    (size_t)F * cv.n * sizeof(float));
+                    }
+                break;
+            }
+        }
+    }
+    return GF2L_OK;
+}
+
+int ensure_io(gf2l_ctx *ctx, size_t bytes) {
+    if (ctx->d_io_bytes >= bytes) return GF2L_OK;
+    hipFree(ctx->d_io);
+    ctx->d_io = nullptr;
+    ctx->d_io_bytes = 0;
+    GF2L_HIP(ctx, hipMalloc(&ctx->d_io, bytes));
+    ctx->d_io_bytes = bytes;
+    return GF2L_OK;
+}
+
+int decode_host(gf2l_ctx *ctx, const gf2l_code *code, const float *llr, size_t frames,
+                uint32_t max_it, uint64_t *msg, uint32_t *iters, uint8_t *conv, uint8_t *synd,
+                float *trace_c2v, float *trace_bel, uint32_t trace_iters) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    if (!code) return fail(ctx, GF2L_ERR_INVALID, "null code");
+    if (frames == 0) return GF2L_OK;
+    if (!llr || !msg || !iters || !conv || !synd)
+        return fail(ctx, GF2L_ERR_INVALID, "null buffer");
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    const gf2l::CodeView &cv = code->view;
+    const size_t kw = ((size_t)cv.k + 63) / 64;
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t b_llr = align(frames * cv.n * sizeof(float)), b_msg = align(frames * kw * 8),
+                 b_it = align(frames * 4), b_c = align(frames);
+    int rc = ensure_io(ctx, b_llr + b_msg + b_it + 2 * b_c);
+    if (rc != GF2L_OK) return rc;
+    char *base = static_cast<char *>(ctx->d_io);
+    float *d_llr = reinterpret_cast<float *>(base);
+    uint64_t *d_msg = reinterpret_cast<uint64_t *>(base + b_llr);
+    uint32_t *d_it = reinterpret_cast<uint32_t *>(base + b_llr + b_msg);
+    uint8_t *d_conv = reinterpret_cast<uint8_t *>(base + b_llr + b_msg + b_it);
+    uint8_t *d_synd = d_conv + b_c;
+    hipStream_t s = ctx->stream;
+    GF2L_HIP(ctx, hipMemcpyAsync(d_llr, llr, frames * cv.n * sizeof(float), hipMemcpyHostToDevice, s));
+    rc = decode_device(ctx, code, d_llr, frames, max_it, d_msg, d_it, d_conv, d_synd, trace_c2v,
+                       trace_bel, trace_iters);
+    if (rc != GF2L_OK) return rc;
+    GF2L_HIP(ctx, hipMemcpyAsync(msg, d_msg, frames * kw * 8, hipMemcpyDeviceToHost, s));
+    GF2L_HIP(ctx, hipMemcpyAsync(iters, d_it, frames * 4, hipMemcpyDeviceToHost, s));
+    GF2L_HIP(ctx, hipMemcpyAsync(conv, d_conv, frames, hipMemcpyDeviceToHost, s));
+    GF2L_HIP(ctx, hipMemcpyAsync(synd, d_synd, frames, hipMemcpyDeviceToHost, s));
+    GF2L_HIP(ctx, hipStreamSynchronize(s));
+    return GF2L_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gf2l_create(int device, gf2l_ctx **out) {
+    if (!out) return GF2L_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+        return GF2L_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return GF2L_ERR_HIP;
+    gf2l_ctx *ctx = new gf2l_ctx;
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return GF2L_ERR_HIP;
+    }
+    int cus = 0, lds = 0;
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+    ctx->num_cus = cus > 0 ? cus : 256;
+    ctx->lds_limit = lds > 0 ? (size_t)lds : 65536;
+    *out = ctx;
+    return GF2L_OK;
+}
+
+void gf2l_destroy(gf2l_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    free_workspace(ctx->ws);
+    for (auto &e : ctx->prof.ev)
+        if (e) hipEventDestroy(e);
+    hipFree(ctx->d_io);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *gf2l_last_error(const gf2l_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int gf2l_synchronize(gf2l_ctx *ctx) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GF2L_OK;
+}
+
+void *gf2l_stream(gf2l_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int gf2l_profile_enable(gf2l_ctx *ctx, int on) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    gf2l::Profile &pf = ctx->prof;
+    if (on && !pf.ev[0])
+        for (auto &e : pf.ev) GF2L_HIP(ctx, hipEventCreate(&e));
+    for (int k = 0; k < 3; ++k) {
+        pf.ms[k] = 0;
+        pf.launches[k] = 0;
+    }
+    pf.frame_iters = 0;
+    pf.on = on != 0;
+    return GF2L_OK;
+}
+
+int gf2l_profile_read(gf2l_ctx *ctx, double *ms, uint64_t *launches, uint64_t *frame_iterations) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    for (int k = 0; k < 3; ++k) {
+        if (ms) ms[k] = ctx->prof.ms[k];
+        if (launches) launches[k] = ctx->prof.launches[k];
+    }
+    if (frame_iterations) *frame_iterations = ctx->prof.frame_iters;
+    return GF2L_OK;
+}
+
+int gf2l_set_path(gf2l_ctx *ctx, int path) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    if (path < GF2L_PATH_AUTO || path > GF2L_PATH_STREAM)
+        return fail(ctx, GF2L_ERR_INVALID, "unknown decoder path");
+    ctx->path = path;
+    return GF2L_OK;
+}
+
+int gf2l_code_qc(gf2l_ctx *ctx, uint32_t mb, uint32_t nb, uint32_t z, const int32_t *shifts,
+                 gf2l_code **out) {
+    if (!ctx || !out) return GF2L_ERR_INVALID;
+    *out = nullptr;
+    // QuasiCyclicLdpc::new panics (core.rs:519-540)
+    if (mb == 0) return fail(ctx, GF2L_ERR_INVALID, "Base matrix must have at least one row");
+    if (z == 0) return fail(ctx, GF2L_ERR_INVALID, "Expansion factor must be positive");
+    if (nb == 0 || !shifts) return fail(ctx, GF2L_ERR_INVALID, "empty base matrix");
+    if ((uint64_t)mb * z > 0x7fffffffull || (uint64_t)nb * z > 0x7fffffffull)
+        return fail(ctx, GF2L_ERR_UNSUPPORTED, "code too large");
+    for (uint32_t br = 0; br < mb; ++br)
+        for (uint32_t bc = 0; bc < nb; ++bc) {
+            const int32_t s = shifts[(size_t)br * nb + bc];
+            if (s != -1 && (s < 0 || (uint32_t)s >= z))  // core.rs:589-596
+                return fail(ctx, GF2L_ERR_INVALID,
+                            "Shift value " + std::to_string(s) + " at position (" + std::to_string(br) +
+                                "," + std::to_string(bc) + ") must be -1 or in range [0," +
+                                std::to_string(z) + ")");
+        }
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    gf2l_code *c = new gf2l_code;
+    c->device = ctx->device;
+    c->shifts.assign(shifts, shifts + (size_t)mb * nb);
+    c->h_row_ptr.assign(mb + 1, 0);
+    uint32_t max_dc = 0, max_dv = 0;
+    for (uint32_t br = 0; br < mb; ++br) {
+        for (uint32_t bc = 0; bc < nb; ++bc) {
+            const int32_t s = shifts[(size_t)br * nb + bc];
+            if (s >= 0) c->h_row_edge.push_back(make_uint2(bc, (uint32_t)s));
+        }
+        c->h_row_ptr[br + 1] = (uint32_t)c->h_row_edge.size();
+        max_dc = std::max(max_dc, c->h_row_ptr[br + 1] - c->h_row_ptr[br]);
+    }
+    c->h_col_ptr.assign(nb + 1, 0);
+    for (uint32_t bc = 0; bc < nb; ++bc) {
+        for (uint32_t br = 0; br < mb; ++br)
+            for (uint32_t e = c->h_row_ptr[br]; e < c->h_row_ptr[br + 1]; ++e)
+                if (c->h_row_edge[e].x == bc) c->h_col_edge.push_back(make_uint2(e, c->h_row_edge[e].y));
+        c->h_col_ptr[bc + 1] = (uint32_t)c->h_col_edge.size();
+        max_dv = std::max(max_dv, c->h_col_ptr[bc + 1] - c->h_col_ptr[bc]);
+    }
+    if (max_dc > gf2l::kMaxCheckDegree) {
+        delete c;
+        return fail(ctx, GF2L_ERR_UNSUPPORTED, "check degree above 32");
+    }
+    const uint32_t nbe = (uint32_t)c->h_row_edge.size();
+    const size_t b_rp = (mb + 1) * 4, b_cp = (nb + 1) * 4, b_e = (size_t)std::max(nbe, 1u) * 8;
+    const size_t o_re = (b_rp + 15) & ~(size_t)15, o_cp = o_re + b_e, o_ce = (o_cp + b_cp + 15) & ~(size_t)15;
+    const size_t total = o_ce + b_e;
+    std::vector<char> host(total, 0);
+    std::memcpy(host.data(), c->h_row_ptr.data(), b_rp);
+    if (nbe) std::memcpy(host.data() + o_re, c->h_row_edge.data(), nbe * 8);
+    std::memcpy(host.data() + o_cp, c->h_col_ptr.data(), b_cp);
+    if (nbe) std::memcpy(host.data() + o_ce, c->h_col_edge.data(), nbe * 8);
+    hipError_t e = hipMalloc(&c->d_tables, total);
+    if (e == hipSuccess) e = hipMemcpy(c->d_tables, host.data(), total, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(c->d_tables);
+        delete c;
+        return fail(ctx, GF2L_ERR_HIP, std::string("code upload: ") + hipGetErrorString(e));
+    }
+    char *d = static_cast<char *>(c->d_tables);
+    gf2l::CodeView &v = c->view;
+    v.mb = mb;
+    v.nb = nb;
+    v.z = z;
+    v.n = nb * z;
+    v.m = mb * z;
+    v.k = v.n > v.m ? v.n - v.m : 0;
+    v.nbe = nbe;
+    v.max_dc = max_dc;
+    v.max_dv = max_dv;
+    v.zw = (z + 63) / 64;
+    v.row_ptr = reinterpret_cast<const uint32_t *>(d);
+    v.row_edge = reinterpret_cast<const uint2 *>(d + o_re);
+    v.col_ptr = reinterpret_cast<const uint32_t *>(d + o_cp);
+    v.col_edge = reinterpret_cast<const uint2 *>(d + o_ce);
+    *out = c;
+    return GF2L_OK;
+}
+
+int gf2l_code_info(const gf2l_code *code, uint32_t *n, uint32_t *m, uint32_t *k, uint64_t *nnz) {
+    if (!code) return GF2L_ERR_INVALID;
+    if (n) *n = code->view.n;
+    if (m) *m = code->view.m;
+    if (k) *k = code->view.k;
+    if (nnz) *nnz = (uint64_t)code->view.nbe * code->view.z;
+    return GF2L_OK;
+}
+
+void gf2l_code_destroy(gf2l_code *code) {
+    if (!code) return;
+    hipSetDevice(code->device);
+    hipFree(code->d_tables);
+    delete code;
+}
+
+int gf2l_decode_batch(gf2l_ctx *ctx, const gf2l_code *code, const float *llr, size_t frames,
+                      uint32_t max_it, uint64_t *msg_words, uint32_t *iterations,
+                      uint8_t *converged, uint8_t *syndrome_ok) {
+    return decode_host(ctx, code, llr, frames, max_it, msg_words, iterations, converged, syndrome_ok,
+                       nullptr, nullptr, 0);
+}
+
+int gf2l_decode_batch_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr,
+                             size_t frames, uint32_t max_it, uint64_t *d_msg_words,
+                             uint32_t *d_iterations, uint8_t *d_converged,
+                             uint8_t *d_syndrome_ok) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    if (!code) return fail(ctx, GF2L_ERR_INVALID, "null code");
+    if (frames == 0) return GF2L_OK;
+    if (!d_llr || !d_msg_words || !d_iterations || !d_converged || !d_syndrome_ok)
+        return fail(ctx, GF2L_ERR_INVALID, "null buffer");
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    return decode_device(ctx, code, d_llr, frames, max_it, d_msg_words, d_iterations, d_converged,
+                         d_syndrome_ok, nullptr, nullptr, 0);
+}
+
+int gf2l_decode_trace(gf2l_ctx *ctx, const gf2l_code *code, const float *llr, size_t frames,
+                      uint32_t max_it, uint32_t trace_iters, float *c2v_out, float *belief_out,
+                      uint64_t *msg_words, uint32_t *iterations, uint8_t *converged,
+                      uint8_t *syndrome_ok) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    if (trace_iters > max_it) return fail(ctx, GF2L_ERR_INVALID, "trace_iters > max_it");
+    if (trace_iters && (!c2v_out || !belief_out)) return fail(ctx, GF2L_ERR_INVALID, "null trace");
+    return decode_host(ctx, code, llr, frames, max_it, msg_words, iterations, converged, syndrome_ok,
+                       trace_iters ? c2v_out : nullptr, trace_iters ? belief_out : nullptr,
+                       trace_iters);
+}
+
+}  // extern "C"

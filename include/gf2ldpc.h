@@ -1,0 +1,148 @@
+/*
+ * gf2ldpc.h -- C ABI of the MI355X (gfx950) batched LDPC belief-propagation
+ * decoder.
+ *
+ * This is the drop-in boundary for the reference's rayon batch path,
+ *   LdpcDecoder::decode_batch(code: &LdpcCode, llr_blocks: &[Vec<Llr>],
+ *                             max_iterations: usize) -> Vec<DecoderResult>
+ *   (crates/gf2-coding/src/ldpc/core.rs:729-756),
+ * and of the code constructor it depends on,
+ *   LdpcCode::from_quasi_cyclic(&QuasiCyclicLdpc)   (core.rs:168-173).
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  The library never keeps a caller
+ *     pointer beyond the call that received it.
+ *   - Every entry point returns GF2L_OK (0) or a negative GF2L_ERR_* code;
+ *     gf2l_last_error() then holds a message.  Where the reference panics
+ *     (invalid shift, LLR length mismatch) these calls return
+ *     GF2L_ERR_INVALID with the reference's panic text.
+ *   - A context owns one HIP stream and its device buffers.  It is not
+ *     thread-safe: use one context per calling thread (the reference gives
+ *     each rayon task its own decoder, core.rs:737-743).
+ *   - Results are deterministic and independent of batch size and order.
+ *
+ * Bit layout of decoded words: bit i of a frame's message is bit (i & 63) of
+ * word (i >> 6), little-endian u64 words, tail bits zero -- the BitVec layout
+ * of crates/gf2-core/src/bitvec.rs:40-63, so BitVec::from_words(words, k)
+ * (bitvec.rs:204) rebuilds DecoderResult::decoded_bits directly.
+ */
+#ifndef GF2LDPC_H
+#define GF2LDPC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GF2L_OK 0
+#define GF2L_ERR_INVALID (-1)     /* bad argument; reference would panic       */
+#define GF2L_ERR_HIP (-2)         /* HIP runtime error                           */
+#define GF2L_ERR_NOMEM (-3)       /* device or host allocation failed            */
+#define GF2L_ERR_UNSUPPORTED (-4) /* shape outside what the kernels handle       */
+
+typedef struct gf2l_ctx gf2l_ctx;
+typedef struct gf2l_code gf2l_code;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Creates a context on HIP device `device`.  No reference counterpart: the
+ * reference has no device; this replaces rayon's global pool
+ * (crates/gf2-core/src/compute/cpu.rs:39-45). */
+int gf2l_create(int device, gf2l_ctx **out);
+void gf2l_destroy(gf2l_ctx *ctx);
+/* Message of the last failing call on this context ("" if none). */
+const char *gf2l_last_error(const gf2l_ctx *ctx);
+/* Waits for all work queued on the context's stream. */
+int gf2l_synchronize(gf2l_ctx *ctx);
+/* The context's hipStream_t, for callers that order their own work on it. */
+void *gf2l_stream(gf2l_ctx *ctx);
+
+/* Per-kernel timing of the streaming path: HIP events are recorded around
+ * every check / variable / syndrome launch on the context's stream.
+ * Enabling (or re-enabling) resets the counters.  No reference counterpart:
+ * the reference has no in-code timing (profiling was done with perf,
+ * crates/gf2-coding/docs/LDPC_PERFORMANCE.md:80-96). */
+int gf2l_profile_enable(gf2l_ctx *ctx, int on);
+/* ms[3] and launches[3] for the check, variable and syndrome kernels;
+ * frame_iterations = frames processed summed over iterations. */
+int gf2l_profile_read(gf2l_ctx *ctx, double *ms, uint64_t *launches, uint64_t *frame_iterations);
+
+/* Decoder path selection (tests force each path; default automatic).
+ *   GF2L_PATH_AUTO    frame state in LDS when it fits, else HBM streaming
+ *   GF2L_PATH_LDS     one workgroup per frame, all iterations in one launch
+ *   GF2L_PATH_STREAM  per-iteration check/variable/syndrome kernels over HBM */
+#define GF2L_PATH_AUTO 0
+#define GF2L_PATH_LDS 1
+#define GF2L_PATH_STREAM 2
+int gf2l_set_path(gf2l_ctx *ctx, int path);
+
+/* ---- codes ------------------------------------------------------------ */
+
+/* Quasi-cyclic code from an mb x nb base matrix of circulant shifts
+ * (row-major, -1 = zero block, else 0 <= s < z) and lift z.
+ * Replaces QuasiCyclicLdpc::new + LdpcCode::from_quasi_cyclic
+ * (core.rs:519-540, 168-173): block (br, bc) with shift s connects check
+ * br*z + i to variable bc*z + (i + s) mod z (CirculantMatrix::to_edges,
+ * core.rs:436-447).  Errors: an out-of-range shift returns GF2L_ERR_INVALID
+ * with the text of core.rs:589-596. */
+int gf2l_code_qc(gf2l_ctx *ctx, uint32_t mb, uint32_t nb, uint32_t z, const int32_t *shifts,
+                 gf2l_code **out);
+/* n = nb*z (LdpcCode::n), m = mb*z (LdpcCode::m), k = n - m (LdpcCode::k,
+ * core.rs:107-119), nnz = number of Tanner-graph edges. */
+int gf2l_code_info(const gf2l_code *code, uint32_t *n, uint32_t *m, uint32_t *k, uint64_t *nnz);
+void gf2l_code_destroy(gf2l_code *code);
+
+/* Seeded quasi-cyclic IRA base matrix (no reference counterpart: the
+ * project's synthetic code family).  Info block-columns 0..nb-mb-1 get the
+ * given column weights; block rows are chosen to balance check degrees and
+ * shifts are drawn uniformly in [0, z), redrawn to avoid length-4 cycles
+ * where possible.  The parity part is the block dual-diagonal staircase:
+ * block-column nb-mb+j has identity blocks at block-rows j and j+1 (the last
+ * one weight 1).  Writes mb*nb shifts, row-major. */
+int gf2l_qc_ira_shifts(uint64_t seed, uint32_t mb, uint32_t nb, uint32_t z,
+                       const uint32_t *info_col_weight, int32_t *shifts_out);
+
+/* ---- decoding --------------------------------------------------------- */
+
+/* LdpcDecoder::decode_batch (core.rs:729-756) on host buffers.
+ *   llr          frames * n channel LLRs, frame-major (frame f at llr + f*n)
+ *   max_it       iteration cap (max_iterations)
+ *   msg_words    frames * ceil(k/64) words: first k hard decisions per frame
+ *   iterations   frames entries  (DecoderResult::iterations)
+ *   converged    frames entries  (DecoderResult::converged)
+ *   syndrome_ok  frames entries  (DecoderResult::syndrome_check_passed)
+ * frames == 0 is valid and does nothing (core.rs:1297-1309).  Flooding
+ * min-sum with syndrome early stop, results bit-identical to the reference
+ * decode_iterative for finite LLRs. */
+int gf2l_decode_batch(gf2l_ctx *ctx, const gf2l_code *code, const float *llr, size_t frames,
+                      uint32_t max_it, uint64_t *msg_words, uint32_t *iterations,
+                      uint8_t *converged, uint8_t *syndrome_ok);
+
+/* Same semantics on device-resident buffers (hipMalloc'd by the caller),
+ * queued on the context's stream; returns after queueing (call
+ * gf2l_synchronize before reading the outputs).  No reference counterpart
+ * beyond decode_batch itself: it lets a GPU-resident pipeline (frame
+ * generator, FER sweep) skip the PCIe round trip. */
+int gf2l_decode_batch_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr,
+                             size_t frames, uint32_t max_it, uint64_t *d_msg_words,
+                             uint32_t *d_iterations, uint8_t *d_converged,
+                             uint8_t *d_syndrome_ok);
+
+/* Per-iteration state dump for parity testing (host buffers).  Decodes like
+ * gf2l_decode_batch and, after each iteration t < trace_iters, copies
+ *   c2v_out[t][f][e*z + i]   check->variable message of block edge e
+ *                            (block-row-major, block-columns ascending)
+ *                            into check (row-of-e)*z + i
+ *   belief_out[t][f][v]      posterior LLR of variable v
+ * A frame that stopped early keeps its final values in later rows. */
+int gf2l_decode_trace(gf2l_ctx *ctx, const gf2l_code *code, const float *llr, size_t frames,
+                      uint32_t max_it, uint32_t trace_iters, float *c2v_out, float *belief_out,
+                      uint64_t *msg_words, uint32_t *iterations, uint8_t *converged,
+                      uint8_t *syndrome_ok);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GF2LDPC_H */

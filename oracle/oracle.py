@@ -1,0 +1,208 @@
+"""ctypes wrapper of the CPU oracle (liboracle_gf2.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the gf2_amd package.  Build with
+``make -C oracle``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_gf2.so")
+
+
+class OrResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_uint32), ("converged", ctypes.c_uint8),
+                ("syndrome_ok", ctypes.c_uint8)]
+
+
+class OrCode(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_uint32), ("n", ctypes.c_uint32), ("nnz", ctypes.c_uint64),
+                ("row_ptr", ctypes.POINTER(ctypes.c_uint64)),
+                ("row_idx", ctypes.POINTER(ctypes.c_uint32)),
+                ("col_ptr", ctypes.POINTER(ctypes.c_uint64)),
+                ("col_idx", ctypes.POINTER(ctypes.c_uint32)),
+                ("csc_to_csr", ctypes.POINTER(ctypes.c_uint64))]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        lib.or_qc_to_edges.restype = ctypes.c_int
+        lib.or_qc_to_edges.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp, vp,
+                                       ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        lib.or_code_from_edges.restype = ctypes.POINTER(OrCode)
+        lib.or_code_from_edges.argtypes = [ctypes.c_uint32, ctypes.c_uint32, vp, ctypes.c_size_t]
+        lib.or_code_free.restype = None
+        lib.or_code_free.argtypes = [ctypes.POINTER(OrCode)]
+        lib.or_minsum_avx2.restype = ctypes.c_float
+        lib.or_minsum_avx2.argtypes = [vp, ctypes.c_size_t]
+        lib.or_minsum_scalar.restype = ctypes.c_float
+        lib.or_minsum_scalar.argtypes = [vp, ctypes.c_size_t]
+        lib.or_syndrome.restype = ctypes.c_uint64
+        lib.or_syndrome.argtypes = [ctypes.POINTER(OrCode), vp, vp]
+        lib.or_decode_iterative_ex.restype = None
+        lib.or_decode_iterative_ex.argtypes = [ctypes.POINTER(OrCode), vp, ctypes.c_uint32, vp, vp,
+                                               ctypes.POINTER(OrResult), vp, vp, ctypes.c_int]
+        lib.or_decode_batch.restype = None
+        lib.or_decode_batch.argtypes = [ctypes.POINTER(OrCode), vp, ctypes.c_size_t,
+                                        ctypes.c_uint32, vp, vp, vp, ctypes.c_int]
+        lib.or_nms_loss_grad.restype = ctypes.c_double
+        lib.or_nms_loss_grad.argtypes = [ctypes.POINTER(OrCode), vp, vp, ctypes.c_uint32, vp, vp]
+        lib.or_philox4x32.restype = None
+        lib.or_philox4x32.argtypes = [vp, vp, vp]
+        lib.port_code_new.restype = vp
+        lib.port_code_new.argtypes = [ctypes.c_uint32, ctypes.c_uint32, vp, ctypes.c_size_t]
+        lib.port_code_free.restype = None
+        lib.port_code_free.argtypes = [vp]
+        lib.port_decode_batch.restype = None
+        lib.port_decode_batch.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_uint32, vp, vp,
+                                          ctypes.c_int]
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+def qc_to_edges(base: np.ndarray, z: int) -> np.ndarray:
+    base = np.ascontiguousarray(base, dtype=np.int32)
+    mb, nb = base.shape
+    cap = int((base >= 0).sum()) * z + 1
+    out = np.zeros((cap, 2), dtype=np.uint32)
+    cnt = ctypes.c_size_t()
+    rc = load().or_qc_to_edges(mb, nb, z, _p(base), _p(out), cap, ctypes.byref(cnt))
+    if rc != 0:
+        raise ValueError(f"or_qc_to_edges failed ({rc})")
+    return out[: cnt.value].copy()
+
+
+class Code:
+    """Oracle code: reference CSR/CSC built by from_coo."""
+
+    def __init__(self, m: int, n: int, edges: np.ndarray):
+        e = np.ascontiguousarray(np.asarray(edges, dtype=np.uint32).reshape(-1, 2))
+        self._edges = e
+        self._p = load().or_code_from_edges(m, n, _p(e), e.shape[0])
+        if not self._p:
+            raise ValueError("edge index out of bounds")
+        c = self._p.contents
+        self.m, self.n, self.nnz = c.m, c.n, int(c.nnz)
+        self.k = max(self.n - self.m, 0)
+        self.row_ptr = np.ctypeslib.as_array(c.row_ptr, (self.m + 1,)).copy()
+        self.row_idx = np.ctypeslib.as_array(c.row_idx, (max(self.nnz, 1),))[: self.nnz].copy()
+        self.col_ptr = np.ctypeslib.as_array(c.col_ptr, (self.n + 1,)).copy()
+        self.col_idx = np.ctypeslib.as_array(c.col_idx, (max(self.nnz, 1),))[: self.nnz].copy()
+        self.csc_to_csr = np.ctypeslib.as_array(c.csc_to_csr, (max(self.nnz, 1),))[: self.nnz].copy()
+        self._port = None
+
+    @classmethod
+    def from_qc(cls, base: np.ndarray, z: int) -> "Code":
+        mb, nb = np.asarray(base).shape
+        return cls(mb * z, nb * z, qc_to_edges(base, z))
+
+    def __del__(self):
+        try:
+            if self._p:
+                load().or_code_free(self._p)
+            if self._port:
+                load().port_code_free(self._port)
+        except Exception:
+            pass
+
+    # -- decoding -------------------------------------------------------
+    def decode_batch(self, llr: np.ndarray, max_it: int, edge_w=None, threads: int = 1):
+        llr = np.ascontiguousarray(llr, dtype=np.float32).reshape(-1, self.n)
+        F = llr.shape[0]
+        kw = (self.k + 63) // 64
+        msg = np.zeros((F, kw), dtype=np.uint64)
+        res = (OrResult * max(F, 1))()
+        w = None if edge_w is None else np.ascontiguousarray(edge_w, dtype=np.float32)
+        load().or_decode_batch(self._p, _p(llr), F, max_it, _p(w), _p(msg), res, threads)
+        it = np.array([res[i].iterations for i in range(F)], dtype=np.uint32)
+        conv = np.array([res[i].converged for i in range(F)], dtype=bool)
+        synd = np.array([res[i].syndrome_ok for i in range(F)], dtype=bool)
+        return msg, it, conv, synd
+
+    def decode_trace(self, llr: np.ndarray, max_it: int, edge_w=None, no_early_stop=False):
+        """One frame: (c2v [max_it, nnz] CSR order, belief [max_it, n], msg, result)."""
+        llr = np.ascontiguousarray(llr, dtype=np.float32).reshape(self.n)
+        tc = np.full((max(max_it, 1), self.nnz), np.nan, dtype=np.float32)
+        tb = np.full((max(max_it, 1), self.n), np.nan, dtype=np.float32)
+        msg = np.zeros((self.k + 63) // 64, dtype=np.uint64)
+        res = OrResult()
+        w = None if edge_w is None else np.ascontiguousarray(edge_w, dtype=np.float32)
+        load().or_decode_iterative_ex(self._p, _p(llr), max_it, _p(w), _p(msg), ctypes.byref(res),
+                                      _p(tc), _p(tb), int(no_early_stop))
+        return tc, tb, msg, (res.iterations, bool(res.converged), bool(res.syndrome_ok))
+
+    def port_decode_batch(self, llr: np.ndarray, max_it: int, threads: int):
+        """The structural restatement of the rayon path (ref_port.cpp)."""
+        if self._port is None:
+            self._port = load().port_code_new(self.m, self.n, _p(self._edges), self._edges.shape[0])
+        llr = np.ascontiguousarray(llr, dtype=np.float32).reshape(-1, self.n)
+        F = llr.shape[0]
+        kw = (self.k + 63) // 64
+        msg = np.zeros((F, kw), dtype=np.uint64)
+        res = (OrResult * max(F, 1))()
+        load().port_decode_batch(self._port, _p(llr), F, max_it, _p(msg), res, threads)
+        it = np.array([res[i].iterations for i in range(F)], dtype=np.uint32)
+        conv = np.array([res[i].converged for i in range(F)], dtype=bool)
+        synd = np.array([res[i].syndrome_ok for i in range(F)], dtype=bool)
+        return msg, it, conv, synd
+
+    def syndrome_weight(self, bits: np.ndarray) -> int:
+        words = pack_bits(bits)
+        return int(load().or_syndrome(self._p, _p(words), None))
+
+    def nms_loss_grad(self, llr: np.ndarray, bits: np.ndarray, T: int, edge_w: np.ndarray):
+        llr = np.ascontiguousarray(llr, dtype=np.float32).reshape(self.n)
+        b = np.ascontiguousarray(bits, dtype=np.uint8).reshape(self.n)
+        w = np.ascontiguousarray(edge_w, dtype=np.float64).reshape(T * self.nnz)
+        g = np.zeros(T * self.nnz, dtype=np.float64)
+        loss = load().or_nms_loss_grad(self._p, _p(llr), _p(b), T, _p(w), _p(g))
+        return loss, g.reshape(T, self.nnz)
+
+
+def minsum_avx2(x) -> np.float32:
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    return np.float32(load().or_minsum_avx2(_p(a), a.size))
+
+
+def minsum_scalar(x) -> np.float32:
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    return np.float32(load().or_minsum_scalar(_p(a), a.size))
+
+
+def philox4x32(ctr, key) -> np.ndarray:
+    c = np.ascontiguousarray(np.asarray(ctr, dtype=np.uint32).reshape(4))
+    k = np.ascontiguousarray(np.asarray(key, dtype=np.uint32).reshape(2))
+    out = np.zeros(4, dtype=np.uint32)
+    load().or_philox4x32(_p(c), _p(k), _p(out))
+    return out
+
+
+def pack_bits(bits: np.ndarray) -> np.ndarray:
+    """bool[n] -> LE u64 words (BitVec layout)."""
+    b = np.asarray(bits, dtype=np.uint8).reshape(-1)
+    n = b.size
+    pad = np.zeros(((n + 63) // 64) * 64, dtype=np.uint8)
+    pad[:n] = b
+    return np.packbits(pad, bitorder="little").view(np.uint64).copy()
