@@ -22,6 +22,10 @@ GF2L_ERR_UNSUPPORTED = -4
 PATH_AUTO = 0
 PATH_LDS = 1
 PATH_STREAM = 2
+PATH_STATE = 3
+
+TUNE_WG_PER_CU = 1
+TUNE_XCD_MAP = 2
 
 _c_void_pp = ctypes.POINTER(ctypes.c_void_p)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -38,6 +42,7 @@ SIGNATURES = {
     "gf2l_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "gf2l_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
     "gf2l_set_path": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "gf2l_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "gf2l_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "gf2l_profile_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p]),

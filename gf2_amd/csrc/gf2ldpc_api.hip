@@ -28,6 +28,7 @@ int fail(gf2l_ctx *ctx, int code, const std::string &msg) {
 
 void free_workspace(gf2l::Workspace &ws) {
     hipFree(ws.c2v);
+    hipFree(ws.state);
     hipFree(ws.belief);
     hipFree(ws.hb);
     hipFree(ws.lists);
@@ -36,36 +37,48 @@ void free_workspace(gf2l::Workspace &ws) {
     ws = gf2l::Workspace{};
 }
 
-// Grows the streaming workspace to hold `frames` frames of `code`.
-int ensure_workspace(gf2l_ctx *ctx, const gf2l::CodeView &cv, size_t frames) {
+// Grows the HBM-path workspace to hold `frames` frames of `code` on `path`.
+int ensure_workspace(gf2l_ctx *ctx, const gf2l::CodeView &cv, int path, size_t frames) {
     gf2l::Workspace &ws = ctx->ws;
-    const size_t E = (size_t)cv.nbe * cv.z, hbw = (size_t)cv.nb * cv.zw;
-    if (ws.c2v && ws.cap_frames >= frames && ws.cap_e >= E * frames &&
-        ws.cap_n >= (size_t)cv.n * frames && ws.cap_hb >= hbw * frames)
+    const size_t need_c2v = path == gf2l::kStream ? (size_t)cv.nbe * cv.z * frames : 0;
+    const size_t need_state = path == gf2l::kState ? 3ull * cv.mb * cv.zp * frames : 0;
+    const size_t need_bel = (path == gf2l::kState ? (size_t)cv.nb * cv.zp : (size_t)cv.n) * frames;
+    const size_t need_hb = (size_t)cv.nb * cv.zw * frames;
+    if (ws.lists && ws.cap_frames >= frames && ws.cap_c2v >= need_c2v &&
+        ws.cap_state >= need_state && ws.cap_bel >= need_bel && ws.cap_hb >= need_hb)
         return GF2L_OK;
     const size_t f = std::max(frames, ws.cap_frames);
-    const size_t ce = std::max(E * f, ws.cap_e), cn = std::max((size_t)cv.n * f, ws.cap_n),
-                 ch = std::max(hbw * f, ws.cap_hb);
+    const size_t c2v = std::max(need_c2v, ws.cap_c2v), st = std::max(need_state, ws.cap_state),
+                 bel = std::max(need_bel, ws.cap_bel), hb = std::max(need_hb, ws.cap_hb);
     free_workspace(ws);
-    GF2L_HIP(ctx, hipMalloc(&ws.c2v, ce * sizeof(float)));
-    GF2L_HIP(ctx, hipMalloc(&ws.belief, cn * sizeof(float)));
-    GF2L_HIP(ctx, hipMalloc(&ws.hb, ch * sizeof(uint64_t)));
+    if (c2v) GF2L_HIP(ctx, hipMalloc(&ws.c2v, c2v * sizeof(float)));
+    if (st) GF2L_HIP(ctx, hipMalloc(&ws.state, st * sizeof(uint32_t)));
+    GF2L_HIP(ctx, hipMalloc(&ws.belief, bel * sizeof(float)));
+    GF2L_HIP(ctx, hipMalloc(&ws.hb, hb * sizeof(uint64_t)));
     GF2L_HIP(ctx, hipMalloc(&ws.lists, 2 * f * sizeof(uint32_t)));
     GF2L_HIP(ctx, hipMalloc(&ws.counts, 2 * sizeof(uint32_t)));
     GF2L_HIP(ctx, hipHostMalloc((void **)&ws.h_count, sizeof(uint32_t), hipHostMallocDefault));
     ws.cap_frames = f;
-    ws.cap_e = ce;
-    ws.cap_n = cn;
-    ws.cap_hb = ch;
+    ws.cap_c2v = c2v;
+    ws.cap_state = st;
+    ws.cap_bel = bel;
+    ws.cap_hb = hb;
     return GF2L_OK;
 }
 
-bool use_lds_path(const gf2l_ctx *ctx, const gf2l::CodeView &cv) {
+// Resolves the decoder path for a code (GF2L_PATH_*), or -1 if the forced
+// path cannot run it.
+int resolve_path(const gf2l_ctx *ctx, const gf2l::CodeView &cv) {
     const size_t bytes = gf2l::lds_path_bytes(cv);
-    if (ctx->path == GF2L_PATH_STREAM) return false;
-    if (ctx->path == GF2L_PATH_LDS) return bytes <= ctx->lds_limit;
-    // automatic: keep at least two frames resident per CU
-    return bytes <= std::min<size_t>(ctx->lds_limit, 80 * 1024);
+    switch (ctx->path) {
+        case gf2l::kLds: return bytes <= ctx->lds_limit ? gf2l::kLds : -1;
+        case gf2l::kStream: return gf2l::kStream;
+        case gf2l::kState: return cv.max_dc <= gf2l::kMaxStateDegree ? gf2l::kState : -1;
+        default:
+            // keep at least two frames resident per CU on the LDS path
+            if (bytes <= std::min<size_t>(ctx->lds_limit, 80 * 1024)) return gf2l::kLds;
+            return cv.max_dc <= gf2l::kMaxStateDegree ? gf2l::kState : gf2l::kStream;
+    }
 }
 
 // Runs the decoder on device buffers.  trace_* (host) and trace_iters are
@@ -87,7 +100,9 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
         GF2L_HIP(ctx, hipMemsetAsync(d_synd, 1, F, s));
         return GF2L_OK;
     }
-    if (use_lds_path(ctx, cv)) {
+    const int path = resolve_path(ctx, cv);
+    if (path < 0) return fail(ctx, GF2L_ERR_UNSUPPORTED, "the forced decoder path cannot run this code");
+    if (path == gf2l::kLds) {
         float *d_tc = nullptr, *d_tb = nullptr;
         if (h_trace_c2v && trace_iters) {
             GF2L_HIP(ctx, hipMalloc(&d_tc, (size_t)trace_iters * F * E * sizeof(float)));
@@ -107,11 +122,20 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
         }
         return GF2L_OK;
     }
-    int rc = ensure_workspace(ctx, cv, F);
+    int rc = ensure_workspace(ctx, cv, path, F);
     if (rc != GF2L_OK) return rc;
     gf2l::Workspace &ws = ctx->ws;
     gf2l::Profile &pf = ctx->prof;
-    GF2L_HIP(ctx, gf2l::launch_stream_init(cv, ws, d_llr, F, s));
+    float *d_tc = nullptr, *d_tb = nullptr;  // explicit-layout trace staging (state path)
+    if (h_trace_c2v && trace_iters && path == gf2l::kState) {
+        GF2L_HIP(ctx, hipMalloc(&d_tc, (size_t)F * E * sizeof(float)));
+        GF2L_HIP(ctx, hipMalloc(&d_tb, (size_t)F * cv.n * sizeof(float)));
+    }
+    struct Free {
+        float *a, *b;
+        ~Free() { hipFree(a); hipFree(b); }
+    } free_trace{d_tc, d_tb};
+    GF2L_HIP(ctx, gf2l::launch_hbm_init(cv, ws, path, d_llr, F, s));
     uint32_t running = F;  // frames entering the current iteration
     auto collect = [&]() -> int {  // stream already synchronized
         for (int k = 0; k < 3; ++k) {
@@ -124,17 +148,23 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
         return GF2L_OK;
     };
     for (uint32_t it = 1; it <= max_it; ++it) {
-        GF2L_HIP(ctx, gf2l::launch_stream_iteration(cv, ws, d_llr, F, it, max_it, nullptr, d_msg,
-                                                    d_iters, d_conv, d_synd, ctx->num_cus, s,
-                                                    pf.on ? pf.ev : nullptr));
+        gf2l::IterArgs a{d_llr, F, it, max_it, nullptr, nullptr, d_msg, d_iters, d_conv, d_synd};
+        GF2L_HIP(ctx, gf2l::launch_hbm_iteration(cv, ws, path, a, ctx->num_cus, ctx->tune, s,
+                                                 pf.on ? pf.ev : nullptr));
         if (pf.on && it == max_it) {
             GF2L_HIP(ctx, hipStreamSynchronize(s));
             if ((rc = collect()) != GF2L_OK) return rc;
         }
         if (h_trace_c2v && it <= trace_iters) {
-            GF2L_HIP(ctx, hipMemcpyAsync(h_trace_c2v + (size_t)(it - 1) * F * E, ws.c2v,
+            const float *src_c = ws.c2v, *src_b = ws.belief;
+            if (path == gf2l::kState) {
+                GF2L_HIP(ctx, gf2l::launch_state_expand(cv, ws, F, nullptr, d_tc, d_tb, s));
+                src_c = d_tc;
+                src_b = d_tb;
+            }
+            GF2L_HIP(ctx, hipMemcpyAsync(h_trace_c2v + (size_t)(it - 1) * F * E, src_c,
                                          (size_t)F * E * sizeof(float), hipMemcpyDeviceToHost, s));
-            GF2L_HIP(ctx, hipMemcpyAsync(h_trace_bel + (size_t)(it - 1) * F * cv.n, ws.belief,
+            GF2L_HIP(ctx, hipMemcpyAsync(h_trace_bel + (size_t)(it - 1) * F * cv.n, src_b,
                                          (size_t)F * cv.n * sizeof(float), hipMemcpyDeviceToHost, s));
         }
         if (it < max_it) {  // stop launching once every frame has retired
@@ -156,6 +186,7 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
             }
         }
     }
+    if (h_trace_c2v) GF2L_HIP(ctx, hipStreamSynchronize(s));
     return GF2L_OK;
 }
 
@@ -280,10 +311,25 @@ int gf2l_profile_read(gf2l_ctx *ctx, double *ms, uint64_t *launches, uint64_t *f
 
 int gf2l_set_path(gf2l_ctx *ctx, int path) {
     if (!ctx) return GF2L_ERR_INVALID;
-    if (path < GF2L_PATH_AUTO || path > GF2L_PATH_STREAM)
+    if (path < GF2L_PATH_AUTO || path > GF2L_PATH_STATE)
         return fail(ctx, GF2L_ERR_INVALID, "unknown decoder path");
     ctx->path = path;
     return GF2L_OK;
+}
+
+int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    switch (key) {
+        case GF2L_TUNE_WG_PER_CU:
+            if (value < 1 || value > 32) return fail(ctx, GF2L_ERR_INVALID, "wg_per_cu out of [1,32]");
+            ctx->tune.wg_per_cu = value;
+            return GF2L_OK;
+        case GF2L_TUNE_XCD_MAP:
+            ctx->tune.xcd_map = value != 0;
+            return GF2L_OK;
+        default:
+            return fail(ctx, GF2L_ERR_INVALID, "unknown tuning key");
+    }
 }
 
 int gf2l_code_qc(gf2l_ctx *ctx, uint32_t mb, uint32_t nb, uint32_t z, const int32_t *shifts,
@@ -323,7 +369,10 @@ int gf2l_code_qc(gf2l_ctx *ctx, uint32_t mb, uint32_t nb, uint32_t z, const int3
     for (uint32_t bc = 0; bc < nb; ++bc) {
         for (uint32_t br = 0; br < mb; ++br)
             for (uint32_t e = c->h_row_ptr[br]; e < c->h_row_ptr[br + 1]; ++e)
-                if (c->h_row_edge[e].x == bc) c->h_col_edge.push_back(make_uint2(e, c->h_row_edge[e].y));
+                if (c->h_row_edge[e].x == bc) {
+                    c->h_col_edge.push_back(make_uint2(e, c->h_row_edge[e].y));
+                    c->h_col_info.push_back(make_uint4(br, e - c->h_row_ptr[br], c->h_row_edge[e].y, e));
+                }
         c->h_col_ptr[bc + 1] = (uint32_t)c->h_col_edge.size();
         max_dv = std::max(max_dv, c->h_col_ptr[bc + 1] - c->h_col_ptr[bc]);
     }
@@ -332,14 +381,17 @@ int gf2l_code_qc(gf2l_ctx *ctx, uint32_t mb, uint32_t nb, uint32_t z, const int3
         return fail(ctx, GF2L_ERR_UNSUPPORTED, "check degree above 32");
     }
     const uint32_t nbe = (uint32_t)c->h_row_edge.size();
-    const size_t b_rp = (mb + 1) * 4, b_cp = (nb + 1) * 4, b_e = (size_t)std::max(nbe, 1u) * 8;
-    const size_t o_re = (b_rp + 15) & ~(size_t)15, o_cp = o_re + b_e, o_ce = (o_cp + b_cp + 15) & ~(size_t)15;
-    const size_t total = o_ce + b_e;
+    const size_t b_rp = (mb + 1) * 4, b_cp = (nb + 1) * 4, b_e = (size_t)std::max(nbe, 1u) * 8,
+                 b_ci = (size_t)std::max(nbe, 1u) * 16;
+    auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    const size_t o_re = al(b_rp), o_cp = al(o_re + b_e), o_ce = al(o_cp + b_cp), o_ci = al(o_ce + b_e);
+    const size_t total = o_ci + b_ci;
     std::vector<char> host(total, 0);
     std::memcpy(host.data(), c->h_row_ptr.data(), b_rp);
     if (nbe) std::memcpy(host.data() + o_re, c->h_row_edge.data(), nbe * 8);
     std::memcpy(host.data() + o_cp, c->h_col_ptr.data(), b_cp);
     if (nbe) std::memcpy(host.data() + o_ce, c->h_col_edge.data(), nbe * 8);
+    if (nbe) std::memcpy(host.data() + o_ci, c->h_col_info.data(), nbe * 16);
     hipError_t e = hipMalloc(&c->d_tables, total);
     if (e == hipSuccess) e = hipMemcpy(c->d_tables, host.data(), total, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -363,6 +415,8 @@ int gf2l_code_qc(gf2l_ctx *ctx, uint32_t mb, uint32_t nb, uint32_t z, const int3
     v.row_edge = reinterpret_cast<const uint2 *>(d + o_re);
     v.col_ptr = reinterpret_cast<const uint32_t *>(d + o_cp);
     v.col_edge = reinterpret_cast<const uint2 *>(d + o_ce);
+    v.col_info = reinterpret_cast<const uint4 *>(d + o_ci);
+    v.zp = ((z + 31) / 32) * 32;
     *out = c;
     return GF2L_OK;
 }

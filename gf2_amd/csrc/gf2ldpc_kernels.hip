@@ -16,7 +16,18 @@
 //     the same subtraction the reference performs.  With belief := L and
 //     c2v := +0 before the first iteration this gives v2c = L exactly.
 //   * hard decision belief < 0 (llr.rs:73-75); syndrome over GF(2) by XOR.
+//
+// Three decoder paths:
+//   LDS     one workgroup per frame, whole frame state in LDS, all
+//           iterations in one launch (small codes).
+//   STATE   per-iteration kernels over HBM; each check keeps a compressed
+//           state (min1, min2, argmin | output sign bits: 12 bytes) from
+//           which every check->variable message is rebuilt bit for bit.
+//   STREAM  per-iteration kernels over HBM with one stored message per
+//           edge (any check degree up to 32).
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "internal.h"
 
@@ -47,19 +58,41 @@ struct CheckAcc {
         sb |= (uint64_t)(__float_as_uint(x) >> 31) << p;
         lt |= (uint64_t)(x < 0.0f) << p;
     }
-    // Message to position p of a degree-d check.
-    __device__ float out(uint32_t d, uint32_t p) const {
-        if (d == 1) return 0.0f;
-        const float mag = (p == idx) ? min2 : min1;
-        const uint32_t l8 = (d - 1) & ~7u;  // inputs covered by full 8-lane chunks
+    // Sign of the message to position p: product of the other inputs' signs,
+    // sign bit for inputs inside full 8-lane chunks of the extrinsic list,
+    // `< 0` for the remainder.
+    __device__ uint32_t neg(uint32_t d, uint32_t p) const {
+        const uint32_t l8 = (d - 1) & ~7u;
         const uint64_t pb = 1ull << p;
         const uint64_t all = (1ull << d) - 1;
         const uint64_t lane = (p < l8) ? (((1ull << (l8 + 1)) - 1) & ~pb) : ((1ull << l8) - 1);
         const uint64_t rem = all & ~lane & ~pb;
-        const uint32_t neg = (__popcll(sb & lane) ^ __popcll(lt & rem)) & 1u;
-        return neg ? -mag : mag;
+        return (__popcll(sb & lane) ^ __popcll(lt & rem)) & 1u;
+    }
+    // Output sign bits of all d positions.  When sign bit and `< 0` agree
+    // for every input (no -0.0) the rule reduces to parity(all) ^ own sign.
+    template <int DMAX>
+    __device__ uint32_t signs(uint32_t d) const {
+        const uint64_t all = (1ull << d) - 1;
+        if (sb == lt) return (uint32_t)(((__popcll(sb) & 1u) ? all : 0ull) ^ sb);
+        uint32_t s = 0;
+#pragma unroll
+        for (int p = 0; p < DMAX; ++p)
+            if (p < (int)d) s |= neg(d, p) << p;
+        return s;
+    }
+    __device__ float out(uint32_t d, uint32_t p) const {
+        if (d == 1) return 0.0f;
+        const float mag = (p == idx) ? min2 : min1;
+        return neg(d, p) ? -mag : mag;
     }
 };
+
+// Message of position p rebuilt from a compressed check state.
+__device__ __forceinline__ float recon(float m1, float m2, uint32_t meta, uint32_t p) {
+    const float mag = (p == (meta & 31u)) ? m2 : m1;
+    return ((meta >> (5 + p)) & 1u) ? -mag : mag;
+}
 
 __device__ __forceinline__ uint32_t wrap_add(uint32_t a, uint32_t b, uint32_t z) {
     uint32_t s = a + b;
@@ -69,159 +102,407 @@ __device__ __forceinline__ uint32_t wrap_sub(uint32_t a, uint32_t b, uint32_t z)
     return a >= b ? a - b : a + z - b;
 }
 
-// ---------------------------------------------------------------------------
-// Streaming path, one launch per phase per iteration.  Frames still running
-// are listed in `list[0..*count)`; persistent workgroups stride over
-// (frame, block-row) or (frame, block-column) items.
+// Calls f(std::integral_constant<int, d>) for a wave-uniform degree d in
+// [0, MAXD], so per-edge loops are fully unrolled with no guards and every
+// load of an item is issued before the first wait.  Degree 0 (a variable or
+// check outside every edge) is a valid case.
+#define GF2L_DEG_CASE(K)                                      \
+    case K:                                                   \
+        if constexpr (K <= MAXD) f(std::integral_constant<int, K>{}); \
+        break;
+template <int MAXD, typename F>
+__device__ __forceinline__ void dispatch_degree(uint32_t d, F &&f) {
+    switch (d) {
+        GF2L_DEG_CASE(0) GF2L_DEG_CASE(1) GF2L_DEG_CASE(2) GF2L_DEG_CASE(3) GF2L_DEG_CASE(4)
+        GF2L_DEG_CASE(5) GF2L_DEG_CASE(6) GF2L_DEG_CASE(7) GF2L_DEG_CASE(8)
+        GF2L_DEG_CASE(9) GF2L_DEG_CASE(10) GF2L_DEG_CASE(11) GF2L_DEG_CASE(12)
+        GF2L_DEG_CASE(13) GF2L_DEG_CASE(14) GF2L_DEG_CASE(15) GF2L_DEG_CASE(16)
+        GF2L_DEG_CASE(17) GF2L_DEG_CASE(18) GF2L_DEG_CASE(19) GF2L_DEG_CASE(20)
+        GF2L_DEG_CASE(21) GF2L_DEG_CASE(22) GF2L_DEG_CASE(23) GF2L_DEG_CASE(24)
+        GF2L_DEG_CASE(25) GF2L_DEG_CASE(26) GF2L_DEG_CASE(27) GF2L_DEG_CASE(28)
+        GF2L_DEG_CASE(29) GF2L_DEG_CASE(30) GF2L_DEG_CASE(31) GF2L_DEG_CASE(32)
+        default: break;
+    }
+}
+#undef GF2L_DEG_CASE
 
-// Check update for all checks of one block-row of one frame.  Threads map to
-// the z lifted checks; belief reads of a circulant are a rotated contiguous
-// run, c2v reads/writes are contiguous.
-template <int DMAX>
-__global__ void __launch_bounds__(1024)
-cn_stream_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
-                 float *__restrict__ c2v, const float *__restrict__ belief,
-                 const float *__restrict__ w_it) {
-    const uint32_t items = *count_ptr * cv.mb;
-    const size_t E = (size_t)cv.nbe * cv.z;
-    for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
-        const uint32_t f = list[item / cv.mb];
-        const uint32_t br = item % cv.mb;
+// XCD-aware split of a persistent grid: workgroups are dispatched round-robin
+// over the 8 XCDs, so workgroup b runs on XCD b % 8.  Frames list[x],
+// list[x+8], ... go to XCD x and the workgroups of that XCD stride over the
+// items of those frames in frame order, so the few frames in flight on an
+// XCD keep their beliefs / check states in that XCD's L2 while their
+// block-rows (block-columns) re-read them.  Needs gridDim.x % 8 == 0.
+// With xcd == 0 the whole grid strides over all items in frame order.
+struct XcdSplit {
+    uint32_t x, q, Q, items, step;
+    __device__ XcdSplit(uint32_t count, uint32_t per_frame, int xcd) {
+        if (xcd) {
+            x = blockIdx.x & 7u;
+            q = blockIdx.x >> 3;
+            Q = gridDim.x >> 3;
+            step = 8u;
+            const uint32_t nfx = count > x ? (count - x + 7u) >> 3 : 0u;
+            items = nfx * per_frame;
+        } else {
+            x = 0;
+            q = blockIdx.x;
+            Q = gridDim.x;
+            step = 1u;
+            items = count * per_frame;
+        }
+    }
+    __device__ uint32_t list_index(uint32_t j, uint32_t per_frame) const {
+        return x + step * (j / per_frame);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// STATE path.
+
+// Check update of one block-row of one frame.  Threads map to the z lifted
+// checks; belief reads of a circulant are a rotated contiguous run, the
+// check's own state is contiguous.  In the first iteration the inputs are
+// the channel LLRs (belief = L, c2v = 0).
+template <int MAXD, bool FIRST, bool NMS>
+__global__ void __launch_bounds__(512)
+cn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
+                uint32_t *__restrict__ state, const float *__restrict__ belief,
+                const float *__restrict__ llr, const float *__restrict__ w_prev, int xcd) {
+    const XcdSplit xs(*count_ptr, cv.mb, xcd);
+    const size_t fstride = 3ull * cv.mb * cv.zp, plane = (size_t)cv.mb * cv.zp;
+    const uint32_t bstride = FIRST ? cv.z : cv.zp;
+    for (uint32_t j = xs.q; j < xs.items; j += xs.Q) {
+        const uint32_t f = list[xs.list_index(j, cv.mb)];
+        const uint32_t br = j % cv.mb;
         const uint32_t e0 = cv.row_ptr[br];
         const uint32_t d = cv.row_ptr[br + 1] - e0;
-        float *fc = c2v + (size_t)f * E;
-        const float *fb = belief + (size_t)f * cv.n;
-        for (uint32_t i = threadIdx.x; i < cv.z; i += blockDim.x) {
-            float x[DMAX];
+        uint32_t *st = state + f * fstride + (size_t)br * cv.zp;
+        const float *fb = FIRST ? llr + (size_t)f * cv.n : belief + (size_t)f * cv.nb * cv.zp;
+        dispatch_degree<MAXD>(d, [&](auto DC) {
+            constexpr int D = decltype(DC)::value, DA = D > 0 ? D : 1;
+            uint2 re[DA];
+            float wp[DA];
 #pragma unroll
-            for (int p = 0; p < DMAX; ++p) {
-                if (p < (int)d) {
-                    const uint2 re = cv.row_edge[e0 + p];
-                    x[p] = fb[re.x * cv.z + wrap_add(i, re.y, cv.z)] - fc[(size_t)(e0 + p) * cv.z + i];
-                }
+            for (int p = 0; p < D; ++p) {
+                re[p] = cv.row_edge[e0 + p];
+                if constexpr (NMS && !FIRST) wp[p] = w_prev[e0 + p];
             }
-            CheckAcc acc;
-            acc.init();
+            for (uint32_t i = threadIdx.x; i < cv.z; i += blockDim.x) {
+                float xv[DA];
 #pragma unroll
-            for (int p = 0; p < DMAX; ++p)
-                if (p < (int)d) acc.add(p, x[p]);
-#pragma unroll
-            for (int p = 0; p < DMAX; ++p) {
-                if (p < (int)d) {
-                    float o = acc.out(d, p);
-                    if (w_it) o = w_it[e0 + p] * o;
-                    fc[(size_t)(e0 + p) * cv.z + i] = o;
+                for (int p = 0; p < D; ++p) xv[p] = fb[re[p].x * bstride + wrap_add(i, re[p].y, cv.z)];
+                float m1o = 0.0f, m2o = 0.0f;
+                uint32_t mo = 0;
+                if constexpr (!FIRST) {
+                    m1o = __uint_as_float(st[i]);
+                    m2o = __uint_as_float(st[plane + i]);
+                    mo = st[2 * plane + i];
                 }
+                CheckAcc acc;
+                acc.init();
+#pragma unroll
+                for (int p = 0; p < D; ++p) {
+                    float v = xv[p];
+                    if constexpr (!FIRST) {
+                        float c = recon(m1o, m2o, mo, p);
+                        if constexpr (NMS) c = wp[p] * c;
+                        v = v - c;
+                    }
+                    acc.add(p, v);
+                }
+                float m1 = acc.min1, m2 = acc.min2;
+                uint32_t meta = acc.idx | (acc.signs<D>(D) << 5);
+                if constexpr (D == 1) {  // the lone message is +0.0
+                    m1 = 0.0f;
+                    m2 = 0.0f;
+                    meta = 0;
+                }
+                st[i] = __float_as_uint(m1);
+                st[plane + i] = __float_as_uint(m2);
+                st[2 * plane + i] = meta;
             }
-        }
+        });
     }
 }
 
-// Variable update for the z variables of one block-column of one frame:
-// belief = L + sum of its c2v in ascending block-row order; hard decisions
-// packed by wave ballot into 64-bit words per block-column.
-template <int DVMAX>
-__global__ void __launch_bounds__(1024)
+// Variable update of the z variables of one block-column of one frame:
+// belief = L + sum of the rebuilt check messages in ascending block-row
+// order; hard decisions packed by wave ballot into 64-bit words.
+template <int MAXD, bool NMS>
+__global__ void __launch_bounds__(512)
+vn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
+                const uint32_t *__restrict__ state, float *__restrict__ belief,
+                const float *__restrict__ llr, uint64_t *__restrict__ hb,
+                uint32_t *__restrict__ next_count, const float *__restrict__ w_cur, int xcd) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
+    const XcdSplit xs(*count_ptr, cv.nb, xcd);
+    const size_t fstride = 3ull * cv.mb * cv.zp, plane = (size_t)cv.mb * cv.zp;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (uint32_t jj = xs.q; jj < xs.items; jj += xs.Q) {
+        const uint32_t f = list[xs.list_index(jj, cv.nb)];
+        const uint32_t bc = jj % cv.nb;
+        const uint32_t c0 = cv.col_ptr[bc];
+        const uint32_t dv = cv.col_ptr[bc + 1] - c0;
+        const uint32_t *fs = state + f * fstride;
+        const float *fl = llr + (size_t)f * cv.n + (size_t)bc * cv.z;
+        float *fb = belief + ((size_t)f * cv.nb + bc) * cv.zp;
+        uint64_t *fh = hb + ((size_t)f * cv.nb + bc) * cv.zw;
+        dispatch_degree<MAXD>(dv, [&](auto DC) {
+            constexpr int D = decltype(DC)::value, DA = D > 0 ? D : 1;
+            uint4 ci[DA];
+            float wq[DA];
+#pragma unroll
+            for (int q = 0; q < D; ++q) {
+                ci[q] = cv.col_info[c0 + q];
+                if constexpr (NMS) wq[q] = w_cur[ci[q].w];
+            }
+            for (uint32_t j0 = 0; j0 < cv.z; j0 += blockDim.x) {
+                const uint32_t j = j0 + threadIdx.x;
+                const bool valid = j < cv.z;
+                float bel = 0.0f;
+                if (valid) {
+                    float m1[DA], m2[DA];
+                    uint32_t mt[DA];
+#pragma unroll
+                    for (int q = 0; q < D; ++q) {
+                        const uint32_t *sp = fs + (size_t)ci[q].x * cv.zp + wrap_sub(j, ci[q].z, cv.z);
+                        m1[q] = __uint_as_float(sp[0]);
+                        m2[q] = __uint_as_float(sp[plane]);
+                        mt[q] = sp[2 * plane];
+                    }
+                    bel = fl[j];
+#pragma unroll
+                    for (int q = 0; q < D; ++q) {
+                        float c = recon(m1[q], m2[q], mt[q], ci[q].y);
+                        if constexpr (NMS) c = wq[q] * c;
+                        bel = bel + c;
+                    }
+                    fb[j] = bel;
+                }
+                const uint64_t mask = __ballot(valid && bel < 0.0f);
+                const uint32_t word_bit = j0 + wave * 64;
+                if (lane == 0 && word_bit < cv.z) fh[word_bit >> 6] = mask;
+            }
+        });
+    }
+}
+
+// ---------------------------------------------------------------------------
+// STREAM path (one stored message per edge).
+
+template <int MAXD, bool NMS>
+__global__ void __launch_bounds__(512)
+cn_stream_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
+                 float *__restrict__ c2v, const float *__restrict__ belief,
+                 const float *__restrict__ w_it, int xcd) {
+    const XcdSplit xs(*count_ptr, cv.mb, xcd);
+    const size_t E = (size_t)cv.nbe * cv.z;
+    for (uint32_t item = xs.q; item < xs.items; item += xs.Q) {
+        const uint32_t f = list[xs.list_index(item, cv.mb)];
+        const uint32_t br = item % cv.mb;
+        const uint32_t e0 = cv.row_ptr[br];
+        const uint32_t d = cv.row_ptr[br + 1] - e0;
+        float *fc = c2v + (size_t)f * E + (size_t)e0 * cv.z;
+        const float *fb = belief + (size_t)f * cv.n;
+        dispatch_degree<MAXD>(d, [&](auto DC) {
+            constexpr int D = decltype(DC)::value, DA = D > 0 ? D : 1;
+            uint2 re[DA];
+            float wc[DA];
+#pragma unroll
+            for (int p = 0; p < D; ++p) {
+                re[p] = cv.row_edge[e0 + p];
+                if constexpr (NMS) wc[p] = w_it[e0 + p];
+            }
+            for (uint32_t i = threadIdx.x; i < cv.z; i += blockDim.x) {
+                float bl[DA], co[DA];
+#pragma unroll
+                for (int p = 0; p < D; ++p) {
+                    bl[p] = fb[re[p].x * cv.z + wrap_add(i, re[p].y, cv.z)];
+                    co[p] = fc[(size_t)p * cv.z + i];
+                }
+                CheckAcc acc;
+                acc.init();
+#pragma unroll
+                for (int p = 0; p < D; ++p) acc.add(p, bl[p] - co[p]);
+#pragma unroll
+                for (int p = 0; p < D; ++p) {
+                    float o = acc.out(D, p);
+                    if constexpr (NMS) o = wc[p] * o;
+                    fc[(size_t)p * cv.z + i] = o;
+                }
+            }
+        });
+    }
+}
+
+template <int MAXD>
+__global__ void __launch_bounds__(512)
 vn_stream_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
                  const float *__restrict__ c2v, float *__restrict__ belief,
                  const float *__restrict__ llr, uint64_t *__restrict__ hb,
-                 uint32_t *__restrict__ next_count) {
+                 uint32_t *__restrict__ next_count, int xcd) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
-    const uint32_t items = *count_ptr * cv.nb;
+    const XcdSplit xs(*count_ptr, cv.nb, xcd);
     const size_t E = (size_t)cv.nbe * cv.z;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
-        const uint32_t f = list[item / cv.nb];
+    for (uint32_t item = xs.q; item < xs.items; item += xs.Q) {
+        const uint32_t f = list[xs.list_index(item, cv.nb)];
         const uint32_t bc = item % cv.nb;
         const uint32_t c0 = cv.col_ptr[bc];
         const uint32_t dv = cv.col_ptr[bc + 1] - c0;
         const float *fc = c2v + (size_t)f * E;
         const size_t vbase = (size_t)f * cv.n + (size_t)bc * cv.z;
-        for (uint32_t j0 = 0; j0 < cv.z; j0 += blockDim.x) {
-            const uint32_t j = j0 + threadIdx.x;
-            const bool valid = j < cv.z;
-            float bel = 0.0f;
-            if (valid) {
-                float c[DVMAX];
+        uint64_t *fh = hb + ((size_t)f * cv.nb + bc) * cv.zw;
+        dispatch_degree<MAXD>(dv, [&](auto DC) {
+            constexpr int D = decltype(DC)::value, DA = D > 0 ? D : 1;
+            uint2 ce[DA];
 #pragma unroll
-                for (int q = 0; q < DVMAX; ++q) {
-                    if (q < (int)dv) {
-                        const uint2 ce = cv.col_edge[c0 + q];
-                        c[q] = fc[(size_t)ce.x * cv.z + wrap_sub(j, ce.y, cv.z)];
-                    }
+            for (int q = 0; q < D; ++q) ce[q] = cv.col_edge[c0 + q];
+            for (uint32_t j0 = 0; j0 < cv.z; j0 += blockDim.x) {
+                const uint32_t j = j0 + threadIdx.x;
+                const bool valid = j < cv.z;
+                float bel = 0.0f;
+                if (valid) {
+                    float c[DA];
+#pragma unroll
+                    for (int q = 0; q < D; ++q)
+                        c[q] = fc[(size_t)ce[q].x * cv.z + wrap_sub(j, ce[q].y, cv.z)];
+                    bel = llr[vbase + j];
+#pragma unroll
+                    for (int q = 0; q < D; ++q) bel = bel + c[q];
+                    belief[vbase + j] = bel;
                 }
-                bel = llr[vbase + j];
-#pragma unroll
-                for (int q = 0; q < DVMAX; ++q)
-                    if (q < (int)dv) bel = bel + c[q];
-                belief[vbase + j] = bel;
+                const uint64_t mask = __ballot(valid && bel < 0.0f);
+                const uint32_t word_bit = j0 + wave * 64;
+                if (lane == 0 && word_bit < cv.z) fh[word_bit >> 6] = mask;
             }
-            const uint64_t mask = __ballot(valid && bel < 0.0f);
-            const uint32_t word_bit = j0 + wave * 64;
-            if (lane == 0 && word_bit < cv.z)
-                hb[((size_t)f * cv.nb + bc) * cv.zw + (word_bit >> 6)] = mask;
-        }
+        });
     }
 }
 
-// Syndrome of each running frame's hard decisions; retires converged frames
-// (and every frame at the iteration cap), writing DecoderResult fields and
-// the first k hard bits; appends the rest to the next list.
-__global__ void __launch_bounds__(256)
-syn_stream_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
-                  uint32_t *__restrict__ next_list, uint32_t *__restrict__ next_count,
-                  const uint64_t *__restrict__ hb, uint32_t it, uint32_t max_it,
-                  uint64_t *__restrict__ msg, uint32_t *__restrict__ iters,
-                  uint8_t *__restrict__ conv, uint8_t *__restrict__ synd) {
-    extern __shared__ uint64_t sh_hb[];
+// ---------------------------------------------------------------------------
+// Syndrome + retirement, shared by both HBM paths: one wave per frame.
+
+// Bits pos .. pos+63 (cyclically mod z) of a z-bit vector stored in zw
+// words whose bits at z and above are zero.  Needs z >= 64.
+__device__ __forceinline__ uint64_t cyc64(const uint64_t *v, uint32_t z, uint32_t zw, uint32_t pos) {
+    const uint32_t wi = pos >> 6, sh = pos & 63u;
+    uint64_t r = v[wi] >> sh;
+    if (sh && wi + 1 < zw) r |= v[wi + 1] << (64u - sh);
+    const uint32_t avail = z - pos;
+    if (avail < 64u) r |= v[0] << avail;
+    return r;
+}
+
+__global__ void __launch_bounds__(64)
+syn_words_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
+                 uint32_t *__restrict__ next_list, uint32_t *__restrict__ next_count,
+                 const uint64_t *__restrict__ hb, uint32_t it, uint32_t max_it,
+                 uint64_t *__restrict__ msg, uint32_t *__restrict__ iters,
+                 uint8_t *__restrict__ conv, uint8_t *__restrict__ synd) {
+    extern __shared__ uint64_t sh[];
     const uint32_t count = *count_ptr;
     const uint32_t words = cv.nb * cv.zw;
     const uint32_t kw = (cv.k + 63) / 64;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t z = cv.z, zw = cv.zw;
     for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
         const uint32_t f = list[item];
-        for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) sh_hb[w] = hb[(size_t)f * words + w];
+        for (uint32_t w = lane; w < words; w += 64) sh[w] = hb[(size_t)f * words + w];
         __syncthreads();
-        int unsat = 0;
-        uint32_t br = threadIdx.x / cv.z, i = threadIdx.x % cv.z;
-        for (uint32_t c = threadIdx.x; c < cv.m; c += blockDim.x) {
-            const uint32_t e0 = cv.row_ptr[br], e1 = cv.row_ptr[br + 1];
-            uint32_t par = 0;
-            for (uint32_t e = e0; e < e1; ++e) {
-                const uint2 re = cv.row_edge[e];
-                const uint32_t col = wrap_add(i, re.y, cv.z);
-                par ^= (uint32_t)(sh_hb[re.x * cv.zw + (col >> 6)] >> (col & 63)) & 1u;
-            }
-            unsat |= (int)par;
-            i += blockDim.x;
-            while (i >= cv.z) { i -= cv.z; ++br; }
-        }
-        unsat = __syncthreads_or(unsat);
-        if (!unsat || it >= max_it) {
-            for (uint32_t w = threadIdx.x; w < kw; w += blockDim.x) {
-                uint64_t word = 0;
-                uint32_t v = w * 64, bc = v / cv.z, j = v - bc * cv.z;
-                for (uint32_t b = 0; b < 64 && v < cv.k; ++b, ++v) {
-                    word |= ((sh_hb[bc * cv.zw + (j >> 6)] >> (j & 63)) & 1ull) << b;
-                    if (++j == cv.z) { j = 0; ++bc; }
+        uint64_t bad = 0;
+        if (z >= 64) {
+            for (uint32_t t = lane; t < cv.mb * zw; t += 64) {
+                const uint32_t br = t / zw, w = t - br * zw;
+                uint64_t acc = 0;
+                for (uint32_t e = cv.row_ptr[br]; e < cv.row_ptr[br + 1]; ++e) {
+                    const uint2 re = cv.row_edge[e];
+                    acc ^= cyc64(sh + re.x * zw, z, zw, wrap_add(64u * w, re.y, z));
                 }
-                msg[(size_t)f * kw + w] = word;
+                if (w == zw - 1 && (z & 63u)) acc &= (1ull << (z & 63u)) - 1;
+                bad |= acc;
             }
-            if (threadIdx.x == 0) {
+        } else {
+            for (uint32_t c = lane; c < cv.m; c += 64) {
+                const uint32_t br = c / z, i = c - br * z;
+                uint32_t par = 0;
+                for (uint32_t e = cv.row_ptr[br]; e < cv.row_ptr[br + 1]; ++e) {
+                    const uint2 re = cv.row_edge[e];
+                    par ^= (uint32_t)(sh[re.x * zw] >> wrap_add(i, re.y, z)) & 1u;
+                }
+                bad |= par;
+            }
+        }
+        const int unsat = __syncthreads_or(bad != 0);
+        if (!unsat || it >= max_it) {
+            for (uint32_t w = lane; w < kw; w += 64) {
+                const uint32_t v0 = 64u * w;
+                uint64_t r = 0;
+                if (z >= 64) {
+                    const uint32_t bc = v0 / z, j = v0 - bc * z;
+                    const uint64_t *blk = sh + bc * zw;
+                    const uint32_t wi = j >> 6, s = j & 63u;
+                    r = blk[wi] >> s;
+                    if (s && wi + 1 < zw) r |= blk[wi + 1] << (64u - s);
+                    const uint32_t avail = z - j;
+                    if (avail < 64u && bc + 1 < cv.nb) r |= blk[zw] << avail;
+                } else {
+                    for (uint32_t b = 0; b < 64 && v0 + b < cv.k; ++b) {
+                        const uint32_t v = v0 + b, bc = v / z, j = v - bc * z;
+                        r |= ((sh[bc * zw] >> j) & 1ull) << b;
+                    }
+                }
+                if (v0 + 64u > cv.k) r &= (1ull << (cv.k - v0)) - 1;
+                msg[(size_t)f * kw + w] = r;
+            }
+            if (lane == 0) {
                 iters[f] = it;
                 conv[f] = (uint8_t)!unsat;
                 synd[f] = (uint8_t)!unsat;
             }
-        } else if (threadIdx.x == 0) {
+        } else if (lane == 0) {
             next_list[atomicAdd(next_count, 1u)] = f;
         }
         __syncthreads();
     }
 }
 
-__global__ void stream_init_kernel(uint32_t *list, uint32_t *counts, uint32_t frames) {
+__global__ void hbm_init_kernel(uint32_t *list, uint32_t *counts, uint32_t frames) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < frames) list[t] = t;
-    if (t == 0) { counts[0] = frames; counts[1] = 0; }
+    if (t == 0) {
+        counts[0] = frames;
+        counts[1] = 0;
+    }
+}
+
+// Trace support for the STATE path: explicit messages and unpadded beliefs.
+__global__ void state_expand_kernel(CodeView cv, const uint32_t *__restrict__ state,
+                                    const float *__restrict__ belief, uint32_t frames,
+                                    const float *__restrict__ w, float *__restrict__ c2v_out,
+                                    float *__restrict__ bel_out) {
+    const size_t E = (size_t)cv.nbe * cv.z, fstride = 3ull * cv.mb * cv.zp,
+                 plane = (size_t)cv.mb * cv.zp;
+    const size_t total = (size_t)frames * E;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+         t += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t f = (uint32_t)(t / E);
+        const uint32_t r = (uint32_t)(t - (size_t)f * E), e = r / cv.z, i = r - e * cv.z;
+        uint32_t br = 0;
+        while (cv.row_ptr[br + 1] <= e) ++br;
+        const uint32_t *s = state + f * fstride + (size_t)br * cv.zp + i;
+        float v = recon(__uint_as_float(s[0]), __uint_as_float(s[plane]), s[2 * plane],
+                        e - cv.row_ptr[br]);
+        if (w) v = w[e] * v;
+        c2v_out[t] = v;
+    }
+    const size_t nt = (size_t)frames * cv.n;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < nt;
+         t += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t f = (uint32_t)(t / cv.n), v = (uint32_t)(t - (size_t)f * cv.n);
+        const uint32_t bc = v / cv.z, j = v - bc * cv.z;
+        bel_out[t] = belief[((size_t)f * cv.nb + bc) * cv.zp + j];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -366,9 +647,20 @@ hipError_t set_lds(K kernel, size_t bytes) {
 int pick_dmax(uint32_t d) { return d <= 8 ? 8 : d <= 16 ? 16 : 32; }
 int pick_dvmax(uint32_t d) { return d <= 4 ? 4 : d <= 8 ? 8 : d <= 16 ? 16 : 32; }
 
+// One thread per lifted check / variable of a block, in whole waves; larger
+// lifts loop inside the block.
 uint32_t round_block(uint32_t z) {
     uint32_t b = ((z + 63) / 64) * 64;
-    return b > 1024 ? 1024 : b;
+    return b > 512 ? 512 : b;
+}
+
+// Persistent grid of a multiple of 8 workgroups, no larger than the work.
+uint32_t hbm_grid(uint32_t items, int num_cus, int wg_per_cu) {
+    uint32_t g = (uint32_t)num_cus * (uint32_t)wg_per_cu;
+    g = ((g + 7u) / 8u) * 8u;
+    const uint32_t need = ((items + 7u) / 8u) * 8u;
+    if (need < g) g = need;
+    return g < 8u ? 8u : g;
 }
 
 template <int DMAX, int DVMAX>
@@ -397,8 +689,8 @@ hipError_t launch_lds_decode(const CodeView &cv, const float *llr, uint32_t fram
                              uint32_t trace_iters, hipStream_t s) {
     if (frames == 0) return hipSuccess;
     const int dm = pick_dmax(cv.max_dc), dvm = pick_dvmax(cv.max_dv);
-#define GF2L_LDS(DM, DV)                                                                     \
-    if (dm == DM && dvm == DV)                                                               \
+#define GF2L_LDS(DM, DV)                                                                        \
+    if (dm == DM && dvm == DV)                                                                  \
         return launch_lds_t<DM, DV>(cv, llr, frames, max_it, w, w_iters, msg, iters, conv, synd, \
                                     trace_c2v, trace_bel, trace_iters, s);
     GF2L_LDS(8, 4) GF2L_LDS(8, 8) GF2L_LDS(8, 16) GF2L_LDS(8, 32)
@@ -408,74 +700,89 @@ hipError_t launch_lds_decode(const CodeView &cv, const float *llr, uint32_t fram
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_stream_init(const CodeView &cv, Workspace &ws, const float *llr, uint32_t frames,
-                              hipStream_t s) {
-    const size_t E = (size_t)cv.nbe * cv.z;
-    hipError_t err = hipMemcpyAsync(ws.belief, llr, (size_t)frames * cv.n * sizeof(float),
-                                    hipMemcpyDeviceToDevice, s);
-    if (err != hipSuccess) return err;
-    err = hipMemsetAsync(ws.c2v, 0, (size_t)frames * E * sizeof(float), s);
-    if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(stream_init_kernel, dim3((frames + 255) / 256), dim3(256), 0, s, ws.lists,
+hipError_t launch_hbm_init(const CodeView &cv, Workspace &ws, int path, const float *llr,
+                           uint32_t frames, hipStream_t s) {
+    if (path == kStream) {
+        hipError_t err = hipMemcpyAsync(ws.belief, llr, (size_t)frames * cv.n * sizeof(float),
+                                        hipMemcpyDeviceToDevice, s);
+        if (err != hipSuccess) return err;
+        err = hipMemsetAsync(ws.c2v, 0, (size_t)frames * cv.nbe * cv.z * sizeof(float), s);
+        if (err != hipSuccess) return err;
+    }
+    hipLaunchKernelGGL(hbm_init_kernel, dim3((frames + 255) / 256), dim3(256), 0, s, ws.lists,
                        ws.counts, frames);
     return hipGetLastError();
 }
 
-hipError_t launch_stream_iteration(const CodeView &cv, Workspace &ws, const float *llr,
-                                   uint32_t frames, uint32_t it, uint32_t max_it, const float *w_it,
-                                   uint64_t *msg, uint32_t *iters, uint8_t *conv, uint8_t *synd,
-                                   int num_cus, hipStream_t s, hipEvent_t *ev) {
-    const uint32_t cur = (it - 1) & 1u, nxt = it & 1u;
+hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, const IterArgs &a,
+                                int num_cus, const Tuning &tune, hipStream_t s, hipEvent_t *ev) {
+    const uint32_t cur = (a.it - 1) & 1u, nxt = a.it & 1u;
     const uint32_t *list = ws.lists + (size_t)cur * ws.cap_frames;
     uint32_t *next_list = ws.lists + (size_t)nxt * ws.cap_frames;
     const uint32_t *count = ws.counts + cur;
     uint32_t *next_count = ws.counts + nxt;
     const uint32_t block = round_block(cv.z);
-    const uint32_t grid_cap = (uint32_t)num_cus * 8u;
-    uint32_t g_cn = frames * cv.mb, g_vn = frames * cv.nb;
-    if (g_cn > grid_cap) g_cn = grid_cap;
-    if (g_vn > grid_cap) g_vn = grid_cap;
-    uint32_t g_syn = frames < grid_cap ? frames : grid_cap;
+    const uint32_t g_cn = hbm_grid(a.frames * cv.mb, num_cus, tune.wg_per_cu);
+    const uint32_t g_vn = hbm_grid(a.frames * cv.nb, num_cus, tune.wg_per_cu);
+    const int xcd = tune.xcd_map;
+    const uint32_t g_syn = a.frames < (uint32_t)num_cus * 8u ? a.frames : (uint32_t)num_cus * 8u;
+    const int first = a.it == 1;
 
     if (ev) hipEventRecord(ev[0], s);
-    switch (pick_dmax(cv.max_dc)) {
-        case 8:
-            hipLaunchKernelGGL(cn_stream_kernel<8>, dim3(g_cn), dim3(block), 0, s, cv, list, count,
-                               ws.c2v, ws.belief, w_it);
-            break;
-        case 16:
-            hipLaunchKernelGGL(cn_stream_kernel<16>, dim3(g_cn), dim3(block), 0, s, cv, list, count,
-                               ws.c2v, ws.belief, w_it);
-            break;
-        default:
-            hipLaunchKernelGGL(cn_stream_kernel<32>, dim3(g_cn), dim3(block), 0, s, cv, list, count,
-                               ws.c2v, ws.belief, w_it);
-            break;
-    }
-    if (ev) hipEventRecord(ev[1], s);
-    switch (pick_dvmax(cv.max_dv)) {
-        case 4:
-            hipLaunchKernelGGL(vn_stream_kernel<4>, dim3(g_vn), dim3(block), 0, s, cv, list, count,
-                               ws.c2v, ws.belief, llr, ws.hb, next_count);
-            break;
-        case 8:
-            hipLaunchKernelGGL(vn_stream_kernel<8>, dim3(g_vn), dim3(block), 0, s, cv, list, count,
-                               ws.c2v, ws.belief, llr, ws.hb, next_count);
-            break;
-        case 16:
-            hipLaunchKernelGGL(vn_stream_kernel<16>, dim3(g_vn), dim3(block), 0, s, cv, list, count,
-                               ws.c2v, ws.belief, llr, ws.hb, next_count);
-            break;
-        default:
-            hipLaunchKernelGGL(vn_stream_kernel<32>, dim3(g_vn), dim3(block), 0, s, cv, list, count,
-                               ws.c2v, ws.belief, llr, ws.hb, next_count);
-            break;
+    const bool nms_prev = a.w_prev != nullptr, nms_cur = a.w_cur != nullptr;
+    const bool wide_c = cv.max_dc > 16, wide_v = cv.max_dv > 16;
+    if (path == kState) {
+#define GF2L_CN(MD, FI, NM)                                                                    \
+    hipLaunchKernelGGL((cn_state_kernel<MD, FI, NM>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
+                       ws.state, ws.belief, a.llr, a.w_prev, xcd)
+        if (first) {
+            if (wide_c) GF2L_CN(27, true, false); else GF2L_CN(16, true, false);
+        } else if (nms_prev) {
+            if (wide_c) GF2L_CN(27, false, true); else GF2L_CN(16, false, true);
+        } else {
+            if (wide_c) GF2L_CN(27, false, false); else GF2L_CN(16, false, false);
+        }
+#undef GF2L_CN
+        if (ev) hipEventRecord(ev[1], s);
+#define GF2L_VN(MD, NM)                                                                        \
+    hipLaunchKernelGGL((vn_state_kernel<MD, NM>), dim3(g_vn), dim3(block), 0, s, cv, list, count, \
+                       ws.state, ws.belief, a.llr, ws.hb, next_count, a.w_cur, xcd)
+        if (nms_cur) {
+            if (wide_v) GF2L_VN(32, true); else GF2L_VN(16, true);
+        } else {
+            if (wide_v) GF2L_VN(32, false); else GF2L_VN(16, false);
+        }
+#undef GF2L_VN
+    } else {
+#define GF2L_CN(MD, NM)                                                                        \
+    hipLaunchKernelGGL((cn_stream_kernel<MD, NM>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
+                       ws.c2v, ws.belief, a.w_cur, xcd)
+        if (nms_cur) {
+            if (wide_c) GF2L_CN(32, true); else GF2L_CN(16, true);
+        } else {
+            if (wide_c) GF2L_CN(32, false); else GF2L_CN(16, false);
+        }
+#undef GF2L_CN
+        if (ev) hipEventRecord(ev[1], s);
+        if (wide_v)
+            hipLaunchKernelGGL((vn_stream_kernel<32>), dim3(g_vn), dim3(block), 0, s, cv, list, count,
+                               ws.c2v, ws.belief, a.llr, ws.hb, next_count, xcd);
+        else
+            hipLaunchKernelGGL((vn_stream_kernel<16>), dim3(g_vn), dim3(block), 0, s, cv, list, count,
+                               ws.c2v, ws.belief, a.llr, ws.hb, next_count, xcd);
     }
     if (ev) hipEventRecord(ev[2], s);
     const size_t sh = (size_t)cv.nb * cv.zw * sizeof(uint64_t);
-    hipLaunchKernelGGL(syn_stream_kernel, dim3(g_syn), dim3(256), sh, s, cv, list, count, next_list,
-                       next_count, ws.hb, it, max_it, msg, iters, conv, synd);
+    hipLaunchKernelGGL(syn_words_kernel, dim3(g_syn), dim3(64), sh, s, cv, list, count, next_list,
+                       next_count, ws.hb, a.it, a.max_it, a.msg, a.iters, a.conv, a.synd);
     if (ev) hipEventRecord(ev[3], s);
+    return hipGetLastError();
+}
+
+hipError_t launch_state_expand(const CodeView &cv, const Workspace &ws, uint32_t frames,
+                               const float *w, float *c2v_out, float *bel_out, hipStream_t s) {
+    hipLaunchKernelGGL(state_expand_kernel, dim3(1024), dim3(256), 0, s, cv, ws.state, ws.belief,
+                       frames, w, c2v_out, bel_out);
     return hipGetLastError();
 }
 

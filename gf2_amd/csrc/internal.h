@@ -9,9 +9,12 @@
 
 namespace gf2l {
 
-// Largest check degree the kernels accept; per-check sign bits live in one
-// 64-bit mask.
+// Largest check degree of the explicit-message kernels (sign masks are 64-bit
+// with one spare bit for the chunk arithmetic).
 constexpr uint32_t kMaxCheckDegree = 32;
+// Largest check degree of the compressed-state kernels: argmin (5 bits) and
+// one output sign bit per edge share a 32-bit word.
+constexpr uint32_t kMaxStateDegree = 27;
 
 // Device view of a quasi-cyclic code: base-matrix block edges in two orders.
 //   row order  e = 0..nbe-1: block-rows ascending, block-columns ascending
@@ -20,30 +23,37 @@ constexpr uint32_t kMaxCheckDegree = 32;
 //   col order  block-columns ascending, block-rows ascending within a
 //              column -- the order of the CSC col_iter for every lifted
 //              variable, which fixes the belief summation order.
-// Message arrays are indexed [e][i], i = check index inside the block-row.
+// Explicit message arrays are indexed [e][i], i = check index inside the
+// block-row; compressed check state and padded beliefs use a row stride zp.
 struct CodeView {
     uint32_t mb, nb, z, n, m, k, nbe;
     uint32_t max_dc, max_dv;
     uint32_t zw;                 // ceil(z/64): 64-bit words per block-column of hard bits
+    uint32_t zp;                 // z rounded up to 32 (128-byte rows)
     const uint32_t *row_ptr;     // mb+1, into row-order edges
     const uint2 *row_edge;       // nbe: {bc, shift}
     const uint32_t *col_ptr;     // nb+1, into col-order list
     const uint2 *col_edge;       // nbe: {row-order edge id, shift}
+    const uint4 *col_info;       // nbe: {block-row, position in row, shift, row-order edge id}
 };
 
-// Per-batch device state of the streaming path.
+// Decoder paths (values of gf2l_set_path).
+enum Path : int { kAuto = 0, kLds = 1, kStream = 2, kState = 3 };
+
+// Per-batch device state of the two HBM paths.
 struct Workspace {
     size_t cap_frames = 0;
-    size_t cap_e = 0, cap_n = 0, cap_hb = 0, cap_kw = 0;
-    float *c2v = nullptr;        // [frames][nbe*z]
-    float *belief = nullptr;     // [frames][n]
-    uint64_t *hb = nullptr;      // [frames][nb][zw]
-    uint32_t *lists = nullptr;   // [2][frames] active frame lists
+    size_t cap_c2v = 0, cap_state = 0, cap_bel = 0, cap_hb = 0;
+    float *c2v = nullptr;        // explicit path: [frames][nbe*z]
+    uint32_t *state = nullptr;   // state path: [frames][3][mb][zp] min1 | min2 | meta
+    float *belief = nullptr;     // explicit: [frames][n]; state: [frames][nb][zp]
+    uint64_t *hb = nullptr;      // [frames][nb][zw] hard decisions
+    uint32_t *lists = nullptr;   // [2][frames] running frame lists
     uint32_t *counts = nullptr;  // [2]
     uint32_t *h_count = nullptr; // pinned host mirror of one count
 };
 
-// Accumulated per-kernel timings of the streaming path (HIP events recorded
+// Accumulated per-kernel timings of the HBM paths (HIP events recorded
 // around each launch on the context's stream).
 struct Profile {
     bool on = false;
@@ -53,6 +63,23 @@ struct Profile {
     uint64_t frame_iters = 0;         // sum over launches of frames processed
 };
 
+// Launch-shape tunables of the HBM paths (set by gf2l_set_tuning or the
+// autotuner).
+struct Tuning {
+    int wg_per_cu = 2;   // persistent workgroups per compute unit
+    int xcd_map = 1;     // 1: frames pinned to XCDs (L2 reuse), 0: frame-major over the grid
+};
+
+// Arguments of one iteration of an HBM path.
+struct IterArgs {
+    const float *llr;
+    uint32_t frames, it, max_it;
+    const float *w_prev, *w_cur;     // per-block-edge weights (NULL = plain min-sum)
+    uint64_t *msg;
+    uint32_t *iters;
+    uint8_t *conv, *synd;
+};
+
 }  // namespace gf2l
 
 struct gf2l_code {
@@ -60,6 +87,7 @@ struct gf2l_code {
     std::vector<int32_t> shifts; // host copy of the base matrix
     std::vector<uint32_t> h_row_ptr, h_col_ptr;
     std::vector<uint2> h_row_edge, h_col_edge;
+    std::vector<uint4> h_col_info;
     void *d_tables = nullptr;    // one allocation backing the view's tables
     int device = 0;
 };
@@ -70,6 +98,7 @@ struct gf2l_ctx {
     hipStream_t stream = nullptr;
     int num_cus = 0;
     int path = 0;
+    gf2l::Tuning tune;
     size_t lds_limit = 0;        // bytes of dynamic LDS one workgroup may use
     std::string err;
     gf2l::Workspace ws;
@@ -86,11 +115,13 @@ hipError_t launch_lds_decode(const CodeView &cv, const float *llr, uint32_t fram
                              const float *w, uint32_t w_iters, uint64_t *msg, uint32_t *iters,
                              uint8_t *conv, uint8_t *synd, float *trace_c2v, float *trace_bel,
                              uint32_t trace_iters, hipStream_t s);
-hipError_t launch_stream_init(const CodeView &cv, Workspace &ws, const float *llr, uint32_t frames,
-                              hipStream_t s);
-hipError_t launch_stream_iteration(const CodeView &cv, Workspace &ws, const float *llr,
-                                   uint32_t frames, uint32_t it, uint32_t max_it, const float *w_it,
-                                   uint64_t *msg, uint32_t *iters, uint8_t *conv, uint8_t *synd,
-                                   int num_cus, hipStream_t s, hipEvent_t *ev);
+hipError_t launch_hbm_init(const CodeView &cv, Workspace &ws, int path, const float *llr,
+                           uint32_t frames, hipStream_t s);
+hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, const IterArgs &a,
+                                int num_cus, const Tuning &tune, hipStream_t s, hipEvent_t *ev);
+// Expands the state path's compressed checks and padded beliefs into the
+// explicit trace layout ([f][e*z+i] and [f][n]) for all `frames` frames.
+hipError_t launch_state_expand(const CodeView &cv, const Workspace &ws, uint32_t frames,
+                               const float *w, float *c2v_out, float *bel_out, hipStream_t s);
 
 }  // namespace gf2l

@@ -70,13 +70,29 @@ int gf2l_profile_enable(gf2l_ctx *ctx, int on);
 int gf2l_profile_read(gf2l_ctx *ctx, double *ms, uint64_t *launches, uint64_t *frame_iterations);
 
 /* Decoder path selection (tests force each path; default automatic).
- *   GF2L_PATH_AUTO    frame state in LDS when it fits, else HBM streaming
+ *   GF2L_PATH_AUTO    LDS when a frame's state fits, else STATE when every
+ *                     check degree is <= 27, else STREAM
  *   GF2L_PATH_LDS     one workgroup per frame, all iterations in one launch
- *   GF2L_PATH_STREAM  per-iteration check/variable/syndrome kernels over HBM */
+ *   GF2L_PATH_STREAM  per-iteration kernels over HBM, one stored message per
+ *                     edge (check degree <= 32)
+ *   GF2L_PATH_STATE   per-iteration kernels over HBM, 12-byte compressed
+ *                     state per check from which messages are rebuilt
+ * A forced path that cannot run a code makes decode calls return
+ * GF2L_ERR_UNSUPPORTED.  All paths give identical results. */
 #define GF2L_PATH_AUTO 0
 #define GF2L_PATH_LDS 1
 #define GF2L_PATH_STREAM 2
+#define GF2L_PATH_STATE 3
 int gf2l_set_path(gf2l_ctx *ctx, int path);
+
+/* Launch-shape tunables of the HBM paths (results never depend on them).
+ *   GF2L_TUNE_WG_PER_CU  persistent workgroups per compute unit (1..32, default 2)
+ *   GF2L_TUNE_XCD_MAP    1 (default): each frame's block-rows / block-columns
+ *                        run on one XCD so its beliefs and check states are
+ *                        re-read from that XCD's L2; 0: frame-major over the grid */
+#define GF2L_TUNE_WG_PER_CU 1
+#define GF2L_TUNE_XCD_MAP 2
+int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value);
 
 /* ---- codes ------------------------------------------------------------ */
 

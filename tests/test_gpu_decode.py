@@ -19,7 +19,8 @@ from tests import frames
 
 pytestmark = pytest.mark.gpu
 
-PATHS = {"lds": _lib.PATH_LDS, "stream": _lib.PATH_STREAM}
+PATHS = {"lds": _lib.PATH_LDS, "stream": _lib.PATH_STREAM, "state": _lib.PATH_STATE}
+ALL = ["lds", "stream", "state"]
 
 
 def say(*a):
@@ -59,7 +60,7 @@ def z1_base(edges, m, n):
 SPC = z1_base([(0, 0), (0, 1), (0, 2)], 1, 3)
 
 
-@pytest.mark.parametrize("path", ["lds", "stream"])
+@pytest.mark.parametrize("path", ALL)
 def test_trivial_decode_no_errors(ctx, path):  # core.rs:1150-1165
     say("spc trivial", path)
     code = gpu_code(ctx, SPC, 1, path)
@@ -72,7 +73,7 @@ def test_trivial_decode_no_errors(ctx, path):  # core.rs:1150-1165
     assert dec.last_iteration_count() == 0
 
 
-@pytest.mark.parametrize("path", ["lds", "stream"])
+@pytest.mark.parametrize("path", ALL)
 def test_reference_batch_kats(ctx, path):  # core.rs:1188-1212, 1272-1309; backend_integration.rs:44-75
     say("spc batch kats", path)
     code = gpu_code(ctx, SPC, 1, path)
@@ -90,7 +91,7 @@ def test_reference_batch_kats(ctx, path):  # core.rs:1188-1212, 1272-1309; backe
         assert r.syndrome_check_passed
 
 
-@pytest.mark.parametrize("path", ["lds", "stream"])
+@pytest.mark.parametrize("path", ALL)
 def test_max_iterations_zero(ctx, path):  # core.rs:906-937
     say("max_it 0", path)
     code = gpu_code(ctx, SPC, 1, path)
@@ -117,7 +118,7 @@ def test_invalid_shift_rejected_by_c_abi(ctx):  # core.rs:589-596
 
 
 # ------------------------------------------------ oracle parity, configs
-@pytest.mark.parametrize("path", ["lds", "stream"])
+@pytest.mark.parametrize("path", ALL)
 def test_n2304_1024_frames_match_oracle(ctx, path):
     spec = codes.n2304_r12()
     base = spec.base_matrix()
@@ -137,12 +138,12 @@ def test_n2304_snr_sweep_match_oracle(ctx, ebn0):
     _, _, llr = frames.random_frames(base, spec.z, 256, ebn0, seed=int(ebn0 * 100))
     say("n2304 sweep", ebn0)
     ref = orc.Code.from_qc(base, spec.z).decode_batch(llr, 50, threads=8)
-    for path in ["lds", "stream"]:
+    for path in ALL:
         code = gpu_code(ctx, base, spec.z, path)
         assert_same(decode_batch_arrays(code, llr, 50), ref, f"n2304@{ebn0}/{path}")
 
 
-@pytest.mark.parametrize("path", ["lds", "stream"])
+@pytest.mark.parametrize("path", ALL)
 def test_per_iteration_messages_bit_exact(ctx, path):
     spec = codes.n2304_r12()
     base = spec.base_matrix()
@@ -163,15 +164,15 @@ def test_per_iteration_messages_bit_exact(ctx, path):
             assert np.array_equal(tb[t, f].view(np.uint32), rb[t].view(np.uint32)), (f, t)
 
 
-@pytest.mark.parametrize("path", ["lds", "stream"])
+@pytest.mark.parametrize("path", ALL)
 def test_high_degree_and_degree_one_checks(ctx, path):
-    # check degrees from 1 to 30 exercise the 8-lane chunk / remainder sign
+    # check degrees from 1 to 27 exercise the 8-lane chunk / remainder sign
     # rule and the degree-1 (+0.0) message; LLRs include exact +-0.0
     rng = np.random.default_rng(13)
     z = 16
     mb, nb = 6, 34
     base = -np.ones((mb, nb), dtype=np.int32)
-    degs = [1, 9, 16, 17, 25, 30]
+    degs = [1, 9, 16, 17, 25, 27]
     for r, d in enumerate(degs):
         cols = rng.choice(nb, d, replace=False)
         base[r, cols] = rng.integers(0, z, d)
@@ -195,11 +196,51 @@ def test_high_degree_and_degree_one_checks(ctx, path):
             assert np.array_equal(tb[t, f].view(np.uint32), rb[t].view(np.uint32)), (f, t)
 
 
+@pytest.mark.parametrize("path", ["lds", "stream"])
+def test_check_degree_32(ctx, path):
+    rng = np.random.default_rng(16)
+    z = 64
+    base = -np.ones((2, 40), dtype=np.int32)
+    base[0, rng.choice(40, 32, replace=False)] = rng.integers(0, z, 32)
+    base[1, rng.choice(40, 30, replace=False)] = rng.integers(0, z, 30)
+    llr = rng.normal(1.0, 1.5, size=(32, 40 * z)).astype(np.float32)
+    say("degree 32", path)
+    code = gpu_code(ctx, base, z, path)
+    assert_same(decode_batch_arrays(code, llr, 15),
+                orc.Code.from_qc(base, z).decode_batch(llr, 15, threads=8), f"deg32/{path}")
+
+
+@pytest.mark.parametrize("path", ALL)
+def test_empty_rows_and_columns(ctx, path):
+    # an all-zero block-row (checks with no edges) and block-columns outside
+    # every check (degree-0 variables) are valid codes
+    rng = np.random.default_rng(18)
+    z = 72
+    base = rng.integers(-1, z, size=(5, 12)).astype(np.int32)
+    base[2, :] = -1
+    base[:, 4] = -1
+    base[:, 9] = -1
+    llr = rng.normal(0.8, 1.5, size=(16, 12 * z)).astype(np.float32)
+    say("empty rows/cols", path)
+    code = gpu_code(ctx, base, z, path)
+    assert_same(decode_batch_arrays(code, llr, 12),
+                orc.Code.from_qc(base, z).decode_batch(llr, 12, threads=8), f"empty/{path}")
+
+
+def test_state_path_rejects_degree_above_27(ctx):
+    base = np.zeros((1, 28), dtype=np.int32)
+    code = gpu_code(ctx, base, 4, "state")
+    with pytest.raises(_lib.Gf2lError) as ei:
+        decode_batch_arrays(code, np.ones((2, 28 * 4), dtype=np.float32), 5)
+    assert ei.value.code == _lib.GF2L_ERR_UNSUPPORTED
+    ctx.set_path(_lib.PATH_AUTO)
+
+
 def test_infinite_llrs(ctx):
     spec = codes.n2304_r12()
     base = spec.base_matrix()
     llr = np.full((2, spec.n), np.inf, dtype=np.float32)
-    for path in ["lds", "stream"]:
+    for path in ALL:
         say("inf llrs", path)
         code = gpu_code(ctx, base, spec.z, path)
         msg, it, conv, synd = decode_batch_arrays(code, llr, 10)
@@ -209,12 +250,32 @@ def test_infinite_llrs(ctx):
 def test_n64800_small_batch_matches_oracle(ctx):
     spec = codes.n64800_r12()
     base = spec.base_matrix()
-    _, _, llr = frames.random_frames(base, spec.z, 24, 1.25, seed=14)
-    say("n64800 24 frames stream")
-    code = gpu_code(ctx, base, spec.z, "auto")
-    gpu = decode_batch_arrays(code, llr, 50)
+    _, _, llr = frames.random_frames(base, spec.z, 24, 1.55, seed=14)
     ref = orc.Code.from_qc(base, spec.z).decode_batch(llr, 50, threads=8)
-    assert_same(gpu, ref, "n64800")
+    for path in ["stream", "state"]:
+        say("n64800 24 frames", path)
+        code = gpu_code(ctx, base, spec.z, path)
+        assert_same(decode_batch_arrays(code, llr, 50), ref, f"n64800/{path}")
+
+
+def test_n64800_trace_state_path_bit_exact(ctx):
+    spec = codes.n64800_r12()
+    base = spec.base_matrix()
+    _, _, llr = frames.random_frames(base, spec.z, 2, 1.5, seed=17)
+    say("n64800 trace state")
+    code = gpu_code(ctx, base, spec.z, "state")
+    T = 6
+    tc, tb, msg, it, conv, synd = decode_trace_arrays(code, llr, 50, T)
+    oc = orc.Code.from_qc(base, spec.z)
+    perm = frames.block_to_csr_perm(base, spec.z, oc.row_ptr)
+    for f in range(2):
+        rc, rb, _, (rit, _, _) = oc.decode_trace(llr[f], 50)
+        assert it[f] == rit
+        for t in range(min(T, rit)):
+            c2v_csr = np.empty(oc.nnz, dtype=np.float32)
+            c2v_csr[perm] = tc[t, f]
+            assert np.array_equal(c2v_csr.view(np.uint32), rc[t].view(np.uint32)), (f, t)
+            assert np.array_equal(tb[t, f].view(np.uint32), rb[t].view(np.uint32)), (f, t)
 
 
 def test_n64800_full_batch_properties(ctx):
