@@ -42,6 +42,15 @@ SIGNATURES = {
     "gf2l_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "gf2l_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
     "gf2l_set_path": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "gf2l_device_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _c_void_pp]),
+    "gf2l_device_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "gf2l_copy_to_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_size_t]),
+    "gf2l_copy_to_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_size_t]),
+    "gf2l_frames_generate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                            ctypes.c_uint64, ctypes.c_size_t, ctypes.c_float,
+                                            ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]),
     "gf2l_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "gf2l_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "gf2l_profile_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -64,6 +73,26 @@ SIGNATURES = {
                                          ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "gf2l_nms_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                       ctypes.c_void_p, ctypes.c_float, _c_void_pp]),
+    "gf2l_nms_destroy": (None, [ctypes.c_void_p]),
+    "gf2l_nms_info": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p]),
+    "gf2l_nms_get_weights": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "gf2l_nms_set_weights": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "gf2l_nms_get_adam": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, _u64p]),
+    "gf2l_nms_set_adam": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_uint64]),
+    "gf2l_nms_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                         ctypes.c_void_p]),
+    "gf2l_nms_train_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float,
+                                           ctypes.c_void_p]),
+    "gf2l_nms_decode_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_size_t, ctypes.c_uint32, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p]),
 }
 
 _lib = None

@@ -11,20 +11,9 @@
 #include "gf2ldpc.h"
 #include "internal.h"
 
+using gf2l::fail;
+
 namespace {
-
-int fail(gf2l_ctx *ctx, int code, const std::string &msg) {
-    if (ctx) ctx->err = msg;
-    return code;
-}
-
-#define GF2L_HIP(ctx, expr)                                                                   \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return fail((ctx), e_ == hipErrorOutOfMemory ? GF2L_ERR_NOMEM : GF2L_ERR_HIP,     \
-                        std::string(#expr) + ": " + hipGetErrorString(e_));                  \
-    } while (0)
 
 void free_workspace(gf2l::Workspace &ws) {
     hipFree(ws.c2v);
@@ -81,11 +70,19 @@ int resolve_path(const gf2l_ctx *ctx, const gf2l::CodeView &cv) {
     }
 }
 
-// Runs the decoder on device buffers.  trace_* (host) and trace_iters are
-// only used by gf2l_decode_trace.
+}  // namespace
+
+namespace gf2l {
+
+// Runs the decoder on device buffers.  Without weights: plain min-sum for up
+// to max_it iterations.  With weights d_w ([max_it][nbe]): max_it weighted
+// iterations, then frames still failing the syndrome are decoded again from
+// their LLRs by up to fb_it plain iterations (d_fb marks them).  trace_*
+// (host) and trace_iters are only used by gf2l_decode_trace.
 int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size_t frames,
-                  uint32_t max_it, uint64_t *d_msg, uint32_t *d_iters, uint8_t *d_conv,
-                  uint8_t *d_synd, float *h_trace_c2v, float *h_trace_bel, uint32_t trace_iters) {
+                  uint32_t max_it, const float *d_w, uint32_t fb_it, uint64_t *d_msg,
+                  uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb,
+                  float *h_trace_c2v, float *h_trace_bel, uint32_t trace_iters) {
     const gf2l::CodeView &cv = code->view;
     if (frames == 0) return GF2L_OK;
     if (frames > 0xffffffffull) return fail(ctx, GF2L_ERR_INVALID, "too many frames in one batch");
@@ -93,11 +90,13 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
     const size_t kw = ((size_t)cv.k + 63) / 64;
     const size_t E = (size_t)cv.nbe * cv.z;
     hipStream_t s = ctx->stream;
+    if (!d_w) fb_it = 0;
     if (max_it == 0) {  // fresh decoder: zero beliefs, no iterations (core.rs:906-937)
         GF2L_HIP(ctx, hipMemsetAsync(d_msg, 0, F * kw * sizeof(uint64_t), s));
         GF2L_HIP(ctx, hipMemsetAsync(d_iters, 0, F * sizeof(uint32_t), s));
         GF2L_HIP(ctx, hipMemsetAsync(d_conv, 0, F, s));
         GF2L_HIP(ctx, hipMemsetAsync(d_synd, 1, F, s));
+        if (d_fb) GF2L_HIP(ctx, hipMemsetAsync(d_fb, 0, F, s));
         return GF2L_OK;
     }
     const int path = resolve_path(ctx, cv);
@@ -108,8 +107,8 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
             GF2L_HIP(ctx, hipMalloc(&d_tc, (size_t)trace_iters * F * E * sizeof(float)));
             GF2L_HIP(ctx, hipMalloc(&d_tb, (size_t)trace_iters * F * cv.n * sizeof(float)));
         }
-        GF2L_HIP(ctx, gf2l::launch_lds_decode(cv, d_llr, F, max_it, nullptr, 0, d_msg, d_iters,
-                                              d_conv, d_synd, d_tc, d_tb, trace_iters, s));
+        GF2L_HIP(ctx, gf2l::launch_lds_decode(cv, d_llr, F, max_it, d_w, fb_it, d_msg, d_iters,
+                                              d_conv, d_synd, d_fb, d_tc, d_tb, trace_iters, s));
         if (d_tc) {
             GF2L_HIP(ctx, hipMemcpyAsync(h_trace_c2v, d_tc, (size_t)trace_iters * F * E * sizeof(float),
                                          hipMemcpyDeviceToHost, s));
@@ -147,34 +146,55 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
         pf.frame_iters += running;
         return GF2L_OK;
     };
-    for (uint32_t it = 1; it <= max_it; ++it) {
-        gf2l::IterArgs a{d_llr, F, it, max_it, nullptr, nullptr, d_msg, d_iters, d_conv, d_synd};
-        GF2L_HIP(ctx, gf2l::launch_hbm_iteration(cv, ws, path, a, ctx->num_cus, ctx->tune, s,
-                                                 pf.on ? pf.ev : nullptr));
-        if (pf.on && it == max_it) {
-            GF2L_HIP(ctx, hipStreamSynchronize(s));
-            if ((rc = collect()) != GF2L_OK) return rc;
+    const uint32_t n_phase = fb_it ? 2 : 1;
+    uint32_t g = 0;  // iterations launched so far: parity of the running list
+    bool done = false;
+    for (uint32_t phase = 0; phase < n_phase && !done; ++phase) {
+        const uint32_t n_it = phase == 0 ? max_it : fb_it;
+        const float *w = phase == 0 ? d_w : nullptr;
+        if (phase == 1 && path == gf2l::kStream) {  // restart the survivors from their LLRs
+            GF2L_HIP(ctx, hipMemcpyAsync(ws.belief, d_llr, (size_t)F * cv.n * sizeof(float),
+                                         hipMemcpyDeviceToDevice, s));
+            GF2L_HIP(ctx, hipMemsetAsync(ws.c2v, 0, (size_t)F * E * sizeof(float), s));
         }
-        if (h_trace_c2v && it <= trace_iters) {
-            const float *src_c = ws.c2v, *src_b = ws.belief;
-            if (path == gf2l::kState) {
-                GF2L_HIP(ctx, gf2l::launch_state_expand(cv, ws, F, nullptr, d_tc, d_tb, s));
-                src_c = d_tc;
-                src_b = d_tb;
+        for (uint32_t it = 1; it <= n_it; ++it, ++g) {
+            gf2l::IterArgs a{};
+            a.llr = d_llr;
+            a.frames = F;
+            a.it = it;
+            a.cur = g & 1u;
+            a.it_report = (phase == 0 ? 0 : max_it) + it;
+            a.last = it == n_it && phase + 1 == n_phase;
+            a.fb_phase = (int)phase;
+            a.w_prev = (w && it >= 2) ? w + (size_t)(it - 2) * cv.nbe : nullptr;
+            a.w_cur = w ? w + (size_t)(it - 1) * cv.nbe : nullptr;
+            a.msg = d_msg;
+            a.iters = d_iters;
+            a.conv = d_conv;
+            a.synd = d_synd;
+            a.fb = d_fb;
+            GF2L_HIP(ctx, gf2l::launch_hbm_iteration(cv, ws, path, a, ctx->num_cus, ctx->tune, s,
+                                                     pf.on ? pf.ev : nullptr));
+            if (phase == 0 && h_trace_c2v && it <= trace_iters) {
+                const float *src_c = ws.c2v, *src_b = ws.belief;
+                if (path == gf2l::kState) {
+                    GF2L_HIP(ctx, gf2l::launch_state_expand(cv, ws, F, a.w_cur, d_tc, d_tb, s));
+                    src_c = d_tc;
+                    src_b = d_tb;
+                }
+                GF2L_HIP(ctx, hipMemcpyAsync(h_trace_c2v + (size_t)(it - 1) * F * E, src_c,
+                                             (size_t)F * E * sizeof(float), hipMemcpyDeviceToHost, s));
+                GF2L_HIP(ctx, hipMemcpyAsync(h_trace_bel + (size_t)(it - 1) * F * cv.n, src_b,
+                                             (size_t)F * cv.n * sizeof(float), hipMemcpyDeviceToHost, s));
             }
-            GF2L_HIP(ctx, hipMemcpyAsync(h_trace_c2v + (size_t)(it - 1) * F * E, src_c,
-                                         (size_t)F * E * sizeof(float), hipMemcpyDeviceToHost, s));
-            GF2L_HIP(ctx, hipMemcpyAsync(h_trace_bel + (size_t)(it - 1) * F * cv.n, src_b,
-                                         (size_t)F * cv.n * sizeof(float), hipMemcpyDeviceToHost, s));
-        }
-        if (it < max_it) {  // stop launching once every frame has retired
-            GF2L_HIP(ctx, hipMemcpyAsync(ws.h_count, ws.counts + (it & 1u), sizeof(uint32_t),
+            // count of frames still running, read back to stop launching early
+            GF2L_HIP(ctx, hipMemcpyAsync(ws.h_count, ws.counts + ((g + 1) & 1u), sizeof(uint32_t),
                                          hipMemcpyDeviceToHost, s));
             GF2L_HIP(ctx, hipStreamSynchronize(s));
             if (pf.on && (rc = collect()) != GF2L_OK) return rc;
             running = *ws.h_count;
-            if (*ws.h_count == 0) {
-                if (h_trace_c2v)
+            if (running == 0) {
+                if (phase == 0 && h_trace_c2v)
                     for (uint32_t t = it + 1; t <= trace_iters; ++t) {
                         std::memcpy(h_trace_c2v + (size_t)(t - 1) * F * E,
                                     h_trace_c2v + (size_t)(it - 1) * F * E, (size_t)F * E * sizeof(float));
@@ -182,13 +202,19 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
                                     h_trace_bel + (size_t)(it - 1) * F * cv.n,
                                     (size_t)F * cv.n * sizeof(float));
                     }
+                done = true;
                 break;
             }
         }
     }
-    if (h_trace_c2v) GF2L_HIP(ctx, hipStreamSynchronize(s));
     return GF2L_OK;
 }
+
+}  // namespace gf2l
+
+namespace {
+
+using gf2l::decode_device;
 
 int ensure_io(gf2l_ctx *ctx, size_t bytes) {
     if (ctx->d_io_bytes >= bytes) return GF2L_OK;
@@ -224,8 +250,8 @@ int decode_host(gf2l_ctx *ctx, const gf2l_code *code, const float *llr, size_t f
     uint8_t *d_synd = d_conv + b_c;
     hipStream_t s = ctx->stream;
     GF2L_HIP(ctx, hipMemcpyAsync(d_llr, llr, frames * cv.n * sizeof(float), hipMemcpyHostToDevice, s));
-    rc = decode_device(ctx, code, d_llr, frames, max_it, d_msg, d_it, d_conv, d_synd, trace_c2v,
-                       trace_bel, trace_iters);
+    rc = decode_device(ctx, code, d_llr, frames, max_it, nullptr, 0, d_msg, d_it, d_conv, d_synd,
+                       nullptr, trace_c2v, trace_bel, trace_iters);
     if (rc != GF2L_OK) return rc;
     GF2L_HIP(ctx, hipMemcpyAsync(msg, d_msg, frames * kw * 8, hipMemcpyDeviceToHost, s));
     GF2L_HIP(ctx, hipMemcpyAsync(iters, d_it, frames * 4, hipMemcpyDeviceToHost, s));
@@ -314,6 +340,69 @@ int gf2l_set_path(gf2l_ctx *ctx, int path) {
     if (path < GF2L_PATH_AUTO || path > GF2L_PATH_STATE)
         return fail(ctx, GF2L_ERR_INVALID, "unknown decoder path");
     ctx->path = path;
+    return GF2L_OK;
+}
+
+int gf2l_device_alloc(gf2l_ctx *ctx, size_t bytes, void **out) {
+    if (!ctx || !out) return GF2L_ERR_INVALID;
+    *out = nullptr;
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    GF2L_HIP(ctx, hipMalloc(out, bytes ? bytes : 1));
+    return GF2L_OK;
+}
+
+int gf2l_device_free(gf2l_ctx *ctx, void *ptr) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    GF2L_HIP(ctx, hipFree(ptr));
+    return GF2L_OK;
+}
+
+int gf2l_copy_to_device(gf2l_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    GF2L_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GF2L_OK;
+}
+
+int gf2l_copy_to_host(gf2l_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    GF2L_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GF2L_OK;
+}
+
+// True when the code is a staircase QC-IRA code: block-column nb-mb+j holds
+// identity blocks exactly at block-rows j and j+1.
+static bool is_staircase_ira(const gf2l_code *c) {
+    const uint32_t mb = c->view.mb, nb = c->view.nb;
+    if (nb <= mb) return false;
+    const uint32_t kb = nb - mb;
+    for (uint32_t br = 0; br < mb; ++br)
+        for (uint32_t j = 0; j < mb; ++j) {
+            const int32_t s = c->shifts[(size_t)br * nb + kb + j];
+            const bool want = br == j || br == j + 1;
+            if (want ? s != 0 : s != -1) return false;
+        }
+    return true;
+}
+
+int gf2l_frames_generate(gf2l_ctx *ctx, const gf2l_code *code, uint64_t seed, uint64_t first_frame,
+                         size_t frames, float ebn0_lo_db, float ebn0_hi_db, float *d_llr,
+                         uint64_t *d_codeword_words) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    if (!code || (!d_llr && frames)) return fail(ctx, GF2L_ERR_INVALID, "null argument");
+    if (frames > 0xffffffffull) return fail(ctx, GF2L_ERR_INVALID, "too many frames");
+    if (!is_staircase_ira(code) || code->view.z < 64)
+        return fail(ctx, GF2L_ERR_UNSUPPORTED,
+                    "frame generation needs a staircase QC-IRA code with lift >= 64");
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    GF2L_HIP(ctx, gf2l::launch_framegen(code->view, seed, first_frame, (uint32_t)frames, ebn0_lo_db,
+                                        ebn0_hi_db, d_llr, d_codeword_words, ctx->num_cus,
+                                        ctx->stream));
     return GF2L_OK;
 }
 
@@ -454,8 +543,8 @@ int gf2l_decode_batch_device(gf2l_ctx *ctx, const gf2l_code *code, const float *
     if (!d_llr || !d_msg_words || !d_iterations || !d_converged || !d_syndrome_ok)
         return fail(ctx, GF2L_ERR_INVALID, "null buffer");
     GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    return decode_device(ctx, code, d_llr, frames, max_it, d_msg_words, d_iterations, d_converged,
-                         d_syndrome_ok, nullptr, nullptr, 0);
+    return decode_device(ctx, code, d_llr, frames, max_it, nullptr, 0, d_msg_words, d_iterations,
+                         d_converged, d_syndrome_ok, nullptr, nullptr, nullptr, 0);
 }
 
 int gf2l_decode_trace(gf2l_ctx *ctx, const gf2l_code *code, const float *llr, size_t frames,

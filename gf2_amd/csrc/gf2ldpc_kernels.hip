@@ -166,7 +166,7 @@ struct XcdSplit {
 template <int MAXD, bool FIRST, bool NMS>
 __global__ void __launch_bounds__(512)
 cn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
-                uint32_t *__restrict__ state, const float *__restrict__ belief,
+                const uint32_t *state_in, uint32_t *state_out, const float *__restrict__ belief,
                 const float *__restrict__ llr, const float *__restrict__ w_prev, int xcd) {
     const XcdSplit xs(*count_ptr, cv.mb, xcd);
     const size_t fstride = 3ull * cv.mb * cv.zp, plane = (size_t)cv.mb * cv.zp;
@@ -176,7 +176,8 @@ cn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *
         const uint32_t br = j % cv.mb;
         const uint32_t e0 = cv.row_ptr[br];
         const uint32_t d = cv.row_ptr[br + 1] - e0;
-        uint32_t *st = state + f * fstride + (size_t)br * cv.zp;
+        const uint32_t *si = state_in + f * fstride + (size_t)br * cv.zp;
+        uint32_t *st = state_out + f * fstride + (size_t)br * cv.zp;
         const float *fb = FIRST ? llr + (size_t)f * cv.n : belief + (size_t)f * cv.nb * cv.zp;
         dispatch_degree<MAXD>(d, [&](auto DC) {
             constexpr int D = decltype(DC)::value, DA = D > 0 ? D : 1;
@@ -194,9 +195,9 @@ cn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *
                 float m1o = 0.0f, m2o = 0.0f;
                 uint32_t mo = 0;
                 if constexpr (!FIRST) {
-                    m1o = __uint_as_float(st[i]);
-                    m2o = __uint_as_float(st[plane + i]);
-                    mo = st[2 * plane + i];
+                    m1o = __uint_as_float(si[i]);
+                    m2o = __uint_as_float(si[plane + i]);
+                    mo = si[2 * plane + i];
                 }
                 CheckAcc acc;
                 acc.init();
@@ -234,7 +235,7 @@ vn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *
                 const uint32_t *__restrict__ state, float *__restrict__ belief,
                 const float *__restrict__ llr, uint64_t *__restrict__ hb,
                 uint32_t *__restrict__ next_count, const float *__restrict__ w_cur, int xcd) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
+    if (next_count && blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
     const XcdSplit xs(*count_ptr, cv.nb, xcd);
     const size_t fstride = 3ull * cv.mb * cv.zp, plane = (size_t)cv.mb * cv.zp;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -246,7 +247,7 @@ vn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *
         const uint32_t *fs = state + f * fstride;
         const float *fl = llr + (size_t)f * cv.n + (size_t)bc * cv.z;
         float *fb = belief + ((size_t)f * cv.nb + bc) * cv.zp;
-        uint64_t *fh = hb + ((size_t)f * cv.nb + bc) * cv.zw;
+        uint64_t *fh = hb ? hb + ((size_t)f * cv.nb + bc) * cv.zw : nullptr;
         dispatch_degree<MAXD>(dv, [&](auto DC) {
             constexpr int D = decltype(DC)::value, DA = D > 0 ? D : 1;
             uint4 ci[DA];
@@ -279,9 +280,11 @@ vn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *
                     }
                     fb[j] = bel;
                 }
-                const uint64_t mask = __ballot(valid && bel < 0.0f);
-                const uint32_t word_bit = j0 + wave * 64;
-                if (lane == 0 && word_bit < cv.z) fh[word_bit >> 6] = mask;
+                if (fh) {
+                    const uint64_t mask = __ballot(valid && bel < 0.0f);
+                    const uint32_t word_bit = j0 + wave * 64;
+                    if (lane == 0 && word_bit < cv.z) fh[word_bit >> 6] = mask;
+                }
             }
         });
     }
@@ -397,9 +400,9 @@ __device__ __forceinline__ uint64_t cyc64(const uint64_t *v, uint32_t z, uint32_
 __global__ void __launch_bounds__(64)
 syn_words_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
                  uint32_t *__restrict__ next_list, uint32_t *__restrict__ next_count,
-                 const uint64_t *__restrict__ hb, uint32_t it, uint32_t max_it,
+                 const uint64_t *__restrict__ hb, uint32_t it_report, int last, int fb_phase,
                  uint64_t *__restrict__ msg, uint32_t *__restrict__ iters,
-                 uint8_t *__restrict__ conv, uint8_t *__restrict__ synd) {
+                 uint8_t *__restrict__ conv, uint8_t *__restrict__ synd, uint8_t *__restrict__ fb) {
     extern __shared__ uint64_t sh[];
     const uint32_t count = *count_ptr;
     const uint32_t words = cv.nb * cv.zw;
@@ -434,7 +437,7 @@ syn_words_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t 
             }
         }
         const int unsat = __syncthreads_or(bad != 0);
-        if (!unsat || it >= max_it) {
+        if (!unsat || last) {
             for (uint32_t w = lane; w < kw; w += 64) {
                 const uint32_t v0 = 64u * w;
                 uint64_t r = 0;
@@ -456,9 +459,10 @@ syn_words_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t 
                 msg[(size_t)f * kw + w] = r;
             }
             if (lane == 0) {
-                iters[f] = it;
+                iters[f] = it_report;
                 conv[f] = (uint8_t)!unsat;
                 synd[f] = (uint8_t)!unsat;
+                if (fb) fb[f] = (uint8_t)fb_phase;
             }
         } else if (lane == 0) {
             next_list[atomicAdd(next_count, 1u)] = f;
@@ -508,13 +512,17 @@ __global__ void state_expand_kernel(CodeView cv, const uint32_t *__restrict__ st
 // ---------------------------------------------------------------------------
 // LDS path: one workgroup owns one frame for all its iterations; messages,
 // beliefs and LLRs live in LDS.  Used when (E + 2n) floats fit.
+// Phase 0 runs max_it iterations (weighted by w[it-1][e] when w is given);
+// a frame that still fails the syndrome is, when fb_it > 0, decoded again
+// from its channel LLRs by fb_it plain iterations (phase 1).
 template <int DMAX, int DVMAX>
 __global__ void __launch_bounds__(512)
 lds_decode_kernel(CodeView cv, const float *__restrict__ llr, uint32_t frames, uint32_t max_it,
-                  const float *__restrict__ w, uint32_t w_iters, uint64_t *__restrict__ msg,
+                  const float *__restrict__ w, uint32_t fb_it, uint64_t *__restrict__ msg,
                   uint32_t *__restrict__ iters, uint8_t *__restrict__ conv,
-                  uint8_t *__restrict__ synd, float *__restrict__ trace_c2v,
-                  float *__restrict__ trace_bel, uint32_t trace_iters) {
+                  uint8_t *__restrict__ synd, uint8_t *__restrict__ fb,
+                  float *__restrict__ trace_c2v, float *__restrict__ trace_bel,
+                  uint32_t trace_iters) {
     extern __shared__ float sm[];
     const uint32_t E = cv.nbe * cv.z, n = cv.n, z = cv.z;
     float *s_c2v = sm;
@@ -522,100 +530,104 @@ lds_decode_kernel(CodeView cv, const float *__restrict__ llr, uint32_t frames, u
     float *s_llr = s_bel + n;
     const uint32_t kw = (cv.k + 63) / 64;
     for (uint32_t f = blockIdx.x; f < frames; f += gridDim.x) {
-        for (uint32_t v = threadIdx.x; v < n; v += blockDim.x) {
-            const float l = llr[(size_t)f * n + v];
-            s_llr[v] = l;
-            s_bel[v] = l;
-        }
-        for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) s_c2v[e] = 0.0f;
-        __syncthreads();
+        for (uint32_t v = threadIdx.x; v < n; v += blockDim.x) s_llr[v] = llr[(size_t)f * n + v];
         uint32_t it_done = 0;
-        int converged = 0;
-        for (uint32_t it = 1; it <= max_it; ++it) {
-            const float *w_it = (w && it <= w_iters) ? w + (size_t)(it - 1) * cv.nbe : nullptr;
-            {   // check update
-                uint32_t br = threadIdx.x / z, i = threadIdx.x % z;
-                for (uint32_t c = threadIdx.x; c < cv.m; c += blockDim.x) {
-                    const uint32_t e0 = cv.row_ptr[br], d = cv.row_ptr[br + 1] - e0;
-                    float x[DMAX];
-#pragma unroll
-                    for (int p = 0; p < DMAX; ++p) {
-                        if (p < (int)d) {
-                            const uint2 re = cv.row_edge[e0 + p];
-                            x[p] = s_bel[re.x * z + wrap_add(i, re.y, z)] - s_c2v[(e0 + p) * z + i];
-                        }
-                    }
-                    CheckAcc acc;
-                    acc.init();
-#pragma unroll
-                    for (int p = 0; p < DMAX; ++p)
-                        if (p < (int)d) acc.add(p, x[p]);
-#pragma unroll
-                    for (int p = 0; p < DMAX; ++p) {
-                        if (p < (int)d) {
-                            float o = acc.out(d, p);
-                            if (w_it) o = w_it[e0 + p] * o;
-                            s_c2v[(e0 + p) * z + i] = o;
-                        }
-                    }
-                    i += blockDim.x;
-                    while (i >= z) { i -= z; ++br; }
-                }
+        int converged = 0, fell_back = 0;
+        for (int phase = 0; phase < 2; ++phase) {
+            const uint32_t n_it = phase == 0 ? max_it : fb_it;
+            if (phase == 1) {
+                if (converged || fb_it == 0) break;
+                fell_back = 1;
             }
+            for (uint32_t v = threadIdx.x; v < n; v += blockDim.x) s_bel[v] = s_llr[v];
+            for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) s_c2v[e] = 0.0f;
             __syncthreads();
-            {   // variable update
-                uint32_t bc = threadIdx.x / z, j = threadIdx.x % z;
-                for (uint32_t v = threadIdx.x; v < n; v += blockDim.x) {
-                    const uint32_t c0 = cv.col_ptr[bc], dv = cv.col_ptr[bc + 1] - c0;
-                    float c[DVMAX];
+            for (uint32_t it = 1; it <= n_it; ++it) {
+                const float *w_it = (phase == 0 && w) ? w + (size_t)(it - 1) * cv.nbe : nullptr;
+                {   // check update
+                    uint32_t br = threadIdx.x / z, i = threadIdx.x % z;
+                    for (uint32_t c = threadIdx.x; c < cv.m; c += blockDim.x) {
+                        const uint32_t e0 = cv.row_ptr[br], d = cv.row_ptr[br + 1] - e0;
+                        float x[DMAX];
 #pragma unroll
-                    for (int q = 0; q < DVMAX; ++q) {
-                        if (q < (int)dv) {
-                            const uint2 ce = cv.col_edge[c0 + q];
-                            c[q] = s_c2v[ce.x * z + wrap_sub(j, ce.y, z)];
+                        for (int p = 0; p < DMAX; ++p) {
+                            if (p < (int)d) {
+                                const uint2 re = cv.row_edge[e0 + p];
+                                x[p] = s_bel[re.x * z + wrap_add(i, re.y, z)] - s_c2v[(e0 + p) * z + i];
+                            }
                         }
-                    }
-                    float bel = s_llr[v];
+                        CheckAcc acc;
+                        acc.init();
 #pragma unroll
-                    for (int q = 0; q < DVMAX; ++q)
-                        if (q < (int)dv) bel = bel + c[q];
-                    s_bel[v] = bel;
-                    j += blockDim.x;
-                    while (j >= z) { j -= z; ++bc; }
-                }
-            }
-            __syncthreads();
-            if (trace_c2v && it <= trace_iters) {
-                for (uint32_t e = threadIdx.x; e < E; e += blockDim.x)
-                    trace_c2v[((size_t)(it - 1) * frames + f) * E + e] = s_c2v[e];
-                for (uint32_t v = threadIdx.x; v < n; v += blockDim.x)
-                    trace_bel[((size_t)(it - 1) * frames + f) * n + v] = s_bel[v];
-            }
-            int unsat = 0;
-            {   // syndrome of the hard decisions
-                uint32_t br = threadIdx.x / z, i = threadIdx.x % z;
-                for (uint32_t c = threadIdx.x; c < cv.m; c += blockDim.x) {
-                    const uint32_t e0 = cv.row_ptr[br], e1 = cv.row_ptr[br + 1];
-                    uint32_t par = 0;
-                    for (uint32_t e = e0; e < e1; ++e) {
-                        const uint2 re = cv.row_edge[e];
-                        par ^= (uint32_t)(s_bel[re.x * z + wrap_add(i, re.y, z)] < 0.0f);
+                        for (int p = 0; p < DMAX; ++p)
+                            if (p < (int)d) acc.add(p, x[p]);
+#pragma unroll
+                        for (int p = 0; p < DMAX; ++p) {
+                            if (p < (int)d) {
+                                float o = acc.out(d, p);
+                                if (w_it) o = w_it[e0 + p] * o;
+                                s_c2v[(e0 + p) * z + i] = o;
+                            }
+                        }
+                        i += blockDim.x;
+                        while (i >= z) { i -= z; ++br; }
                     }
-                    unsat |= (int)par;
-                    i += blockDim.x;
-                    while (i >= z) { i -= z; ++br; }
                 }
-            }
-            unsat = __syncthreads_or(unsat);
-            it_done = it;
-            if (!unsat) { converged = 1; break; }
-        }
-        if (trace_c2v) {  // frames that stopped early keep their final state
-            for (uint32_t t = it_done + 1; t <= trace_iters && it_done > 0; ++t) {
-                for (uint32_t e = threadIdx.x; e < E; e += blockDim.x)
-                    trace_c2v[((size_t)(t - 1) * frames + f) * E + e] = s_c2v[e];
-                for (uint32_t v = threadIdx.x; v < n; v += blockDim.x)
-                    trace_bel[((size_t)(t - 1) * frames + f) * n + v] = s_bel[v];
+                __syncthreads();
+                {   // variable update
+                    uint32_t bc = threadIdx.x / z, j = threadIdx.x % z;
+                    for (uint32_t v = threadIdx.x; v < n; v += blockDim.x) {
+                        const uint32_t c0 = cv.col_ptr[bc], dv = cv.col_ptr[bc + 1] - c0;
+                        float c[DVMAX];
+#pragma unroll
+                        for (int q = 0; q < DVMAX; ++q) {
+                            if (q < (int)dv) {
+                                const uint2 ce = cv.col_edge[c0 + q];
+                                c[q] = s_c2v[ce.x * z + wrap_sub(j, ce.y, z)];
+                            }
+                        }
+                        float bel = s_llr[v];
+#pragma unroll
+                        for (int q = 0; q < DVMAX; ++q)
+                            if (q < (int)dv) bel = bel + c[q];
+                        s_bel[v] = bel;
+                        j += blockDim.x;
+                        while (j >= z) { j -= z; ++bc; }
+                    }
+                }
+                __syncthreads();
+                if (phase == 0 && trace_c2v && it <= trace_iters) {
+                    for (uint32_t e = threadIdx.x; e < E; e += blockDim.x)
+                        trace_c2v[((size_t)(it - 1) * frames + f) * E + e] = s_c2v[e];
+                    for (uint32_t v = threadIdx.x; v < n; v += blockDim.x)
+                        trace_bel[((size_t)(it - 1) * frames + f) * n + v] = s_bel[v];
+                }
+                int unsat = 0;
+                {   // syndrome of the hard decisions
+                    uint32_t br = threadIdx.x / z, i = threadIdx.x % z;
+                    for (uint32_t c = threadIdx.x; c < cv.m; c += blockDim.x) {
+                        const uint32_t e0 = cv.row_ptr[br], e1 = cv.row_ptr[br + 1];
+                        uint32_t par = 0;
+                        for (uint32_t e = e0; e < e1; ++e) {
+                            const uint2 re = cv.row_edge[e];
+                            par ^= (uint32_t)(s_bel[re.x * z + wrap_add(i, re.y, z)] < 0.0f);
+                        }
+                        unsat |= (int)par;
+                        i += blockDim.x;
+                        while (i >= z) { i -= z; ++br; }
+                    }
+                }
+                unsat = __syncthreads_or(unsat);
+                it_done = (phase == 0 ? 0 : max_it) + it;
+                if (phase == 0 && trace_c2v && !unsat) {  // frames that stop keep their final state
+                    for (uint32_t t = it + 1; t <= trace_iters; ++t) {
+                        for (uint32_t e = threadIdx.x; e < E; e += blockDim.x)
+                            trace_c2v[((size_t)(t - 1) * frames + f) * E + e] = s_c2v[e];
+                        for (uint32_t v = threadIdx.x; v < n; v += blockDim.x)
+                            trace_bel[((size_t)(t - 1) * frames + f) * n + v] = s_bel[v];
+                    }
+                }
+                if (!unsat) { converged = 1; break; }
             }
         }
         // max_it == 0: a fresh decoder's beliefs are all zero (core.rs:680-693)
@@ -632,6 +644,7 @@ lds_decode_kernel(CodeView cv, const float *__restrict__ llr, uint32_t frames, u
             iters[f] = it_done;
             conv[f] = (uint8_t)converged;
             synd[f] = (uint8_t)(converged || max_it == 0);
+            if (fb) fb[f] = (uint8_t)fell_back;
         }
         __syncthreads();
     }
@@ -665,15 +678,15 @@ uint32_t hbm_grid(uint32_t items, int num_cus, int wg_per_cu) {
 
 template <int DMAX, int DVMAX>
 hipError_t launch_lds_t(const CodeView &cv, const float *llr, uint32_t frames, uint32_t max_it,
-                        const float *w, uint32_t w_iters, uint64_t *msg, uint32_t *iters,
-                        uint8_t *conv, uint8_t *synd, float *tc, float *tb, uint32_t ti,
+                        const float *w, uint32_t fb_it, uint64_t *msg, uint32_t *iters,
+                        uint8_t *conv, uint8_t *synd, uint8_t *fb, float *tc, float *tb, uint32_t ti,
                         hipStream_t s) {
     const size_t bytes = lds_path_bytes(cv);
     auto k = lds_decode_kernel<DMAX, DVMAX>;
     hipError_t err = set_lds(k, bytes);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k, dim3(frames), dim3(512), bytes, s, cv, llr, frames, max_it, w, w_iters,
-                       msg, iters, conv, synd, tc, tb, ti);
+    hipLaunchKernelGGL(k, dim3(frames), dim3(512), bytes, s, cv, llr, frames, max_it, w, fb_it,
+                       msg, iters, conv, synd, fb, tc, tb, ti);
     return hipGetLastError();
 }
 
@@ -684,15 +697,15 @@ size_t lds_path_bytes(const CodeView &cv) {
 }
 
 hipError_t launch_lds_decode(const CodeView &cv, const float *llr, uint32_t frames, uint32_t max_it,
-                             const float *w, uint32_t w_iters, uint64_t *msg, uint32_t *iters,
-                             uint8_t *conv, uint8_t *synd, float *trace_c2v, float *trace_bel,
-                             uint32_t trace_iters, hipStream_t s) {
+                             const float *w, uint32_t fb_it, uint64_t *msg, uint32_t *iters,
+                             uint8_t *conv, uint8_t *synd, uint8_t *fb, float *trace_c2v,
+                             float *trace_bel, uint32_t trace_iters, hipStream_t s) {
     if (frames == 0) return hipSuccess;
     const int dm = pick_dmax(cv.max_dc), dvm = pick_dvmax(cv.max_dv);
 #define GF2L_LDS(DM, DV)                                                                        \
     if (dm == DM && dvm == DV)                                                                  \
-        return launch_lds_t<DM, DV>(cv, llr, frames, max_it, w, w_iters, msg, iters, conv, synd, \
-                                    trace_c2v, trace_bel, trace_iters, s);
+        return launch_lds_t<DM, DV>(cv, llr, frames, max_it, w, fb_it, msg, iters, conv, synd, \
+                                    fb, trace_c2v, trace_bel, trace_iters, s);
     GF2L_LDS(8, 4) GF2L_LDS(8, 8) GF2L_LDS(8, 16) GF2L_LDS(8, 32)
     GF2L_LDS(16, 4) GF2L_LDS(16, 8) GF2L_LDS(16, 16) GF2L_LDS(16, 32)
     GF2L_LDS(32, 4) GF2L_LDS(32, 8) GF2L_LDS(32, 16) GF2L_LDS(32, 32)
@@ -716,7 +729,7 @@ hipError_t launch_hbm_init(const CodeView &cv, Workspace &ws, int path, const fl
 
 hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, const IterArgs &a,
                                 int num_cus, const Tuning &tune, hipStream_t s, hipEvent_t *ev) {
-    const uint32_t cur = (a.it - 1) & 1u, nxt = a.it & 1u;
+    const uint32_t cur = a.cur & 1u, nxt = cur ^ 1u;
     const uint32_t *list = ws.lists + (size_t)cur * ws.cap_frames;
     uint32_t *next_list = ws.lists + (size_t)nxt * ws.cap_frames;
     const uint32_t *count = ws.counts + cur;
@@ -734,7 +747,7 @@ hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, con
     if (path == kState) {
 #define GF2L_CN(MD, FI, NM)                                                                    \
     hipLaunchKernelGGL((cn_state_kernel<MD, FI, NM>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
-                       ws.state, ws.belief, a.llr, a.w_prev, xcd)
+                       ws.state, ws.state, ws.belief, a.llr, a.w_prev, xcd)
         if (first) {
             if (wide_c) GF2L_CN(27, true, false); else GF2L_CN(16, true, false);
         } else if (nms_prev) {
@@ -774,8 +787,43 @@ hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, con
     if (ev) hipEventRecord(ev[2], s);
     const size_t sh = (size_t)cv.nb * cv.zw * sizeof(uint64_t);
     hipLaunchKernelGGL(syn_words_kernel, dim3(g_syn), dim3(64), sh, s, cv, list, count, next_list,
-                       next_count, ws.hb, a.it, a.max_it, a.msg, a.iters, a.conv, a.synd);
+                       next_count, ws.hb, a.it_report, a.last, a.fb_phase, a.msg, a.iters, a.conv,
+                       a.synd, a.fb);
     if (ev) hipEventRecord(ev[3], s);
+    return hipGetLastError();
+}
+
+hipError_t launch_state_forward(const CodeView &cv, const uint32_t *list, const uint32_t *count,
+                                uint32_t frames, bool first, const uint32_t *state_in,
+                                uint32_t *state_out, const float *belief_in, float *belief_out,
+                                const float *llr, const float *w_prev, const float *w_cur,
+                                int num_cus, const Tuning &tune, hipStream_t s) {
+    const uint32_t block = round_block(cv.z);
+    const uint32_t g_cn = hbm_grid(frames * cv.mb, num_cus, tune.wg_per_cu);
+    const uint32_t g_vn = hbm_grid(frames * cv.nb, num_cus, tune.wg_per_cu);
+    const int xcd = tune.xcd_map;
+    const bool wide_c = cv.max_dc > 16, wide_v = cv.max_dv > 16;
+    const bool nms_prev = w_prev != nullptr, nms_cur = w_cur != nullptr;
+#define GF2L_CN(MD, FI, NM)                                                                    \
+    hipLaunchKernelGGL((cn_state_kernel<MD, FI, NM>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
+                       state_in, state_out, belief_in, llr, w_prev, xcd)
+    if (first) {
+        if (wide_c) GF2L_CN(27, true, false); else GF2L_CN(16, true, false);
+    } else if (nms_prev) {
+        if (wide_c) GF2L_CN(27, false, true); else GF2L_CN(16, false, true);
+    } else {
+        if (wide_c) GF2L_CN(27, false, false); else GF2L_CN(16, false, false);
+    }
+#undef GF2L_CN
+#define GF2L_VN(MD, NM)                                                                        \
+    hipLaunchKernelGGL((vn_state_kernel<MD, NM>), dim3(g_vn), dim3(block), 0, s, cv, list, count, \
+                       state_out, belief_out, llr, nullptr, nullptr, w_cur, xcd)
+    if (nms_cur) {
+        if (wide_v) GF2L_VN(32, true); else GF2L_VN(16, true);
+    } else {
+        if (wide_v) GF2L_VN(32, false); else GF2L_VN(16, false);
+    }
+#undef GF2L_VN
     return hipGetLastError();
 }
 

@@ -73,11 +73,16 @@ struct Tuning {
 // Arguments of one iteration of an HBM path.
 struct IterArgs {
     const float *llr;
-    uint32_t frames, it, max_it;
+    uint32_t frames;
+    uint32_t it;                     // iteration within the current phase (1 = first)
+    uint32_t cur;                    // parity of the running-frame list to read
+    uint32_t it_report;              // iteration count reported for frames retiring now
+    int last;                        // retire frames that still fail the syndrome
+    int fb_phase;                    // value written to fb[] for retiring frames
     const float *w_prev, *w_cur;     // per-block-edge weights (NULL = plain min-sum)
     uint64_t *msg;
     uint32_t *iters;
-    uint8_t *conv, *synd;
+    uint8_t *conv, *synd, *fb;
 };
 
 }  // namespace gf2l
@@ -109,16 +114,53 @@ struct gf2l_ctx {
 
 namespace gf2l {
 
+// Records `msg` as the context's last error and returns `code`.
+inline int fail(gf2l_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+// Returns from the enclosing function with a GF2L error when a HIP call fails.
+#define GF2L_HIP(ctx, expr)                                                                        \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return ::gf2l::fail((ctx), e_ == hipErrorOutOfMemory ? -3 : -2,                        \
+                                std::string(#expr) + ": " + hipGetErrorString(e_));               \
+    } while (0)
+
+// Decoder driver (gf2ldpc_api.hip): plain min-sum for max_it iterations, or
+// with weights d_w ([max_it][nbe]) max_it weighted iterations followed by up
+// to fb_it plain iterations from scratch for frames still failing the
+// syndrome.  Device buffers; may synchronize the context's stream.
+int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size_t frames,
+                  uint32_t max_it, const float *d_w, uint32_t fb_it, uint64_t *d_msg,
+                  uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb,
+                  float *h_trace_c2v, float *h_trace_bel, uint32_t trace_iters);
+
 // Kernel launchers (gf2ldpc_kernels.hip).  All enqueue on `s`.
 size_t lds_path_bytes(const CodeView &cv);
+// Phase 0: max_it iterations, weighted by w[it-1][e] if w != NULL; then, for
+// frames still failing the syndrome, fb_it plain iterations from scratch.
 hipError_t launch_lds_decode(const CodeView &cv, const float *llr, uint32_t frames, uint32_t max_it,
-                             const float *w, uint32_t w_iters, uint64_t *msg, uint32_t *iters,
-                             uint8_t *conv, uint8_t *synd, float *trace_c2v, float *trace_bel,
-                             uint32_t trace_iters, hipStream_t s);
+                             const float *w, uint32_t fb_it, uint64_t *msg, uint32_t *iters,
+                             uint8_t *conv, uint8_t *synd, uint8_t *fb, float *trace_c2v,
+                             float *trace_bel, uint32_t trace_iters, hipStream_t s);
 hipError_t launch_hbm_init(const CodeView &cv, Workspace &ws, int path, const float *llr,
                            uint32_t frames, hipStream_t s);
 hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, const IterArgs &a,
                                 int num_cus, const Tuning &tune, hipStream_t s, hipEvent_t *ev);
+// One forward iteration of the state kernels on explicit buffers (no
+// syndrome, no hard decisions): used by the training forward pass.
+hipError_t launch_state_forward(const CodeView &cv, const uint32_t *list, const uint32_t *count,
+                                uint32_t frames, bool first, const uint32_t *state_in,
+                                uint32_t *state_out, const float *belief_in, float *belief_out,
+                                const float *llr, const float *w_prev, const float *w_cur,
+                                int num_cus, const Tuning &tune, hipStream_t s);
+// Frame generator (framegen.hip).
+hipError_t launch_framegen(const CodeView &cv, uint64_t seed, uint64_t first_frame, uint32_t frames,
+                           float ebn0_lo, float ebn0_hi, float *llr, uint64_t *cw_words, int num_cus,
+                           hipStream_t s);
 // Expands the state path's compressed checks and padded beliefs into the
 // explicit trace layout ([f][e*z+i] and [f][n]) for all `frames` frames.
 hipError_t launch_state_expand(const CodeView &cv, const Workspace &ws, uint32_t frames,

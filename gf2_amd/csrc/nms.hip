@@ -1,0 +1,536 @@
+// Learned (neural) normalized min-sum decoder: per-iteration, per-base-edge
+// weights, trained on the device with hand-written forward and backward
+// kernels; decoding runs the weighted iterations and re-decodes frames that
+// still fail the syndrome with plain min-sum.
+//
+// The check rule is the reference's alpha * min-sum (Llr::boxplus_normalized_
+// minsum_n, crates/gf2-coding/src/llr.rs:352-355) with one alpha per base
+// edge and iteration: c2v_t(e) = w_t(e) * sgn * min_{e' != e} |v2c_{t-1}(e')|.
+// Everything else is the reference flooding schedule (core.rs:889-938).
+//
+// Loss: mean over frames, iterations t = 1..T and variables v of
+// softplus(-y_v * belief_t[v]), y = +1 for bit 0 and -1 for bit 1: the BCE of
+// P(bit = 1) = sigmoid(-belief) under the LLR sign of llr.rs:9-15.
+// Min-sum is differentiated as a subgradient: only the extrinsic argmin of
+// each message (first position on ties) receives gradient; signs are
+// piecewise constant.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+#include <vector>
+
+#include "gf2ldpc.h"
+#include "internal.h"
+
+struct gf2l_nms {
+    const gf2l_code *code = nullptr;
+    uint32_t T = 0;           // weighted iterations
+    uint32_t nbe = 0;         // weights per iteration (base-graph edges)
+    float *d_w = nullptr;     // [T][nbe]
+    float *d_m = nullptr;     // Adam first moment
+    float *d_v = nullptr;     // Adam second moment
+    uint64_t step = 0;
+    // training workspace, grown on demand
+    size_t cap = 0;
+    uint32_t *tape_state = nullptr;  // [T][F][3][mb][zp]
+    float *tape_bel = nullptr;       // [T][F][nb][zp]
+    float *g_a = nullptr, *g_b = nullptr;  // [F][nbe][z] dL/dv2c
+    float *g_bel = nullptr;          // [F][nb][zp]
+    float *gpart = nullptr;          // [T][nbe][F]
+    float *grad = nullptr;           // [T][nbe]
+    float *loss = nullptr;           // 1
+    uint32_t *list = nullptr;        // [F] identity list
+    uint32_t *count = nullptr;       // 1
+};
+
+namespace gf2l {
+namespace {
+
+__device__ __forceinline__ uint32_t wrap_add(uint32_t a, uint32_t b, uint32_t z) {
+    uint32_t s = a + b;
+    return s >= z ? s - z : s;
+}
+__device__ __forceinline__ uint32_t wrap_sub(uint32_t a, uint32_t b, uint32_t z) {
+    return a >= b ? a - b : a + z - b;
+}
+__device__ __forceinline__ float recon(float m1, float m2, uint32_t meta, uint32_t p) {
+    const float mag = (p == (meta & 31u)) ? m2 : m1;
+    return ((meta >> (5 + p)) & 1u) ? -mag : mag;
+}
+
+#define GF2L_DEG_CASE(K)                                              \
+    case K:                                                           \
+        if constexpr (K <= MAXD) f(std::integral_constant<int, K>{}); \
+        break;
+template <int MAXD, typename F>
+__device__ __forceinline__ void dispatch_degree(uint32_t d, F &&f) {
+    switch (d) {
+        GF2L_DEG_CASE(0) GF2L_DEG_CASE(1) GF2L_DEG_CASE(2) GF2L_DEG_CASE(3)
+        GF2L_DEG_CASE(4) GF2L_DEG_CASE(5) GF2L_DEG_CASE(6) GF2L_DEG_CASE(7)
+        GF2L_DEG_CASE(8) GF2L_DEG_CASE(9) GF2L_DEG_CASE(10) GF2L_DEG_CASE(11)
+        GF2L_DEG_CASE(12) GF2L_DEG_CASE(13) GF2L_DEG_CASE(14) GF2L_DEG_CASE(15)
+        GF2L_DEG_CASE(16) GF2L_DEG_CASE(17) GF2L_DEG_CASE(18) GF2L_DEG_CASE(19)
+        GF2L_DEG_CASE(20) GF2L_DEG_CASE(21) GF2L_DEG_CASE(22) GF2L_DEG_CASE(23)
+        GF2L_DEG_CASE(24) GF2L_DEG_CASE(25) GF2L_DEG_CASE(26) GF2L_DEG_CASE(27)
+        GF2L_DEG_CASE(28) GF2L_DEG_CASE(29) GF2L_DEG_CASE(30) GF2L_DEG_CASE(31)
+        GF2L_DEG_CASE(32)
+        default: break;
+    }
+}
+#undef GF2L_DEG_CASE
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Variable pass of iteration t: dL/dbelief_t[v] = BCE'(belief_t[v]) * scale
+// + sum of dL/dv2c_t over v's edges (absent at t = T).  Also accumulates the
+// loss (logging only: float atomics).
+template <int MAXD>
+__global__ void __launch_bounds__(512)
+vn_bwd_kernel(CodeView cv, uint32_t frames, const float *__restrict__ bel_t,
+              const uint64_t *__restrict__ cw, const float *__restrict__ g_cur, int has_gcur,
+              float *__restrict__ g_bel, float scale, float *__restrict__ loss) {
+    __shared__ float red[8];
+    const uint32_t items = frames * cv.nb;
+    const uint32_t nw = (cv.n + 63) / 64;
+    const size_t E = (size_t)cv.nbe * cv.z;
+    float lsum = 0.0f;
+    for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
+        const uint32_t f = item / cv.nb, bc = item % cv.nb;
+        const uint32_t c0 = cv.col_ptr[bc], dv = cv.col_ptr[bc + 1] - c0;
+        const float *fg = g_cur + (size_t)f * E;
+        dispatch_degree<MAXD>(dv, [&](auto DC) {
+            constexpr int D = decltype(DC)::value, DA = D > 0 ? D : 1;
+            uint2 ce[DA];
+#pragma unroll
+            for (int q = 0; q < D; ++q) ce[q] = cv.col_edge[c0 + q];
+            for (uint32_t j = threadIdx.x; j < cv.z; j += blockDim.x) {
+                const size_t bi = ((size_t)f * cv.nb + bc) * cv.zp + j;
+                float gin[DA];
+                if (has_gcur) {
+#pragma unroll
+                    for (int q = 0; q < D; ++q)
+                        gin[q] = fg[(size_t)ce[q].x * cv.z + wrap_sub(j, ce[q].y, cv.z)];
+                }
+                const float s = bel_t[bi];
+                const uint32_t v = bc * cv.z + j;
+                const float y = ((cw[(size_t)f * nw + (v >> 6)] >> (v & 63u)) & 1ull) ? -1.0f : 1.0f;
+                const float ys = y * s;
+                lsum += ys > 0.0f ? log1pf(expf(-ys)) : -ys + log1pf(expf(ys));
+                float g = -y * scale / (1.0f + expf(ys));  // d softplus(-y s) / ds
+                if (has_gcur) {
+#pragma unroll
+                    for (int q = 0; q < D; ++q) g += gin[q];
+                }
+                g_bel[bi] = g;
+            }
+        });
+    }
+    lsum = wave_sum(lsum);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lsum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (uint32_t w = 0; w < blockDim.x / 64; ++w) t += red[w];
+        atomicAdd(loss, t);
+    }
+}
+
+// Check pass of iteration t (1-based).  Recomputes the check's inputs
+// v2c_{t-1} from the tape exactly as the forward pass did, then
+//   g_c2v(p) = dL/dbelief_t[v_p] - dL/dv2c_t(p)
+//   dL/dw_t(e_p) += g_c2v(p) * m_t(p)            (m = unweighted min-sum value)
+//   dL/dv2c_{t-1}(a) += g_c2v(p) * w_t(e_p) * sgn_p * sign(v2c_{t-1}(a)),
+//       a = argmin over the other inputs (t >= 2 only; v2c_0 = LLR)
+// and writes the per-frame weight gradients of the block-row's edges,
+// reduced over the z lanes in a fixed order, to gpart[e][frame].
+template <int MAXD>
+__global__ void __launch_bounds__(512)
+cn_bwd_kernel(CodeView cv, uint32_t frames, uint32_t t, const float *__restrict__ llr,
+              const uint32_t *__restrict__ st_prev, const float *__restrict__ bel_prev,
+              const float *__restrict__ w_prev, const float *__restrict__ w_t,
+              const float *__restrict__ g_bel, const float *__restrict__ g_cur, int has_gcur,
+              float *__restrict__ g_prev, float *__restrict__ gpart) {
+    __shared__ float red[MAXD > 0 ? MAXD : 1][8];
+    const uint32_t items = frames * cv.mb;
+    const size_t fstride = 3ull * cv.mb * cv.zp, plane = (size_t)cv.mb * cv.zp;
+    const size_t E = (size_t)cv.nbe * cv.z;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
+    for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
+        const uint32_t f = item / cv.mb, br = item % cv.mb;
+        const uint32_t e0 = cv.row_ptr[br], d = cv.row_ptr[br + 1] - e0;
+        dispatch_degree<MAXD>(d, [&](auto DC) {
+            constexpr int D = decltype(DC)::value, DA = D > 0 ? D : 1;
+            uint2 re[DA];
+            float wp[DA], wt[DA], gw[DA];
+#pragma unroll
+            for (int p = 0; p < D; ++p) {
+                re[p] = cv.row_edge[e0 + p];
+                wp[p] = (t >= 2) ? w_prev[e0 + p] : 0.0f;
+                wt[p] = w_t[e0 + p];
+                gw[p] = 0.0f;
+            }
+            const uint32_t *si = st_prev + f * fstride + (size_t)br * cv.zp;
+            for (uint32_t i = threadIdx.x; i < cv.z; i += blockDim.x) {
+                float x[DA], gb[DA], gc[DA];
+                float m1o = 0.0f, m2o = 0.0f;
+                uint32_t mo = 0;
+                if (t >= 2) {
+                    m1o = __uint_as_float(si[i]);
+                    m2o = __uint_as_float(si[plane + i]);
+                    mo = si[2 * plane + i];
+                }
+#pragma unroll
+                for (int p = 0; p < D; ++p) {
+                    const uint32_t col = wrap_add(i, re[p].y, cv.z);
+                    const size_t bi = ((size_t)f * cv.nb + re[p].x) * cv.zp + col;
+                    if (t >= 2) x[p] = bel_prev[bi] - wp[p] * recon(m1o, m2o, mo, p);
+                    else x[p] = llr[(size_t)f * cv.n + (size_t)re[p].x * cv.z + col];
+                    gb[p] = g_bel[bi];
+                    gc[p] = has_gcur ? g_cur[(size_t)f * E + (size_t)(e0 + p) * cv.z + i] : 0.0f;
+                }
+                // argmin, second argmin and input signs
+                float min1 = __builtin_inff(), min2 = __builtin_inff();
+                uint32_t idx1 = 0, idx2 = 0, neg = 0;
+#pragma unroll
+                for (int p = 0; p < D; ++p) {
+                    const float a = fabsf(x[p]);
+                    if (a < min1) {
+                        min2 = min1;
+                        idx2 = idx1;
+                        min1 = a;
+                        idx1 = p;
+                    } else if (a < min2) {
+                        min2 = a;
+                        idx2 = p;
+                    }
+                    neg |= (uint32_t)(x[p] < 0.0f) << p;
+                }
+                const uint32_t par = __popc(neg) & 1u;
+                float gx[DA];
+#pragma unroll
+                for (int p = 0; p < D; ++p) gx[p] = 0.0f;
+#pragma unroll
+                for (int p = 0; p < D; ++p) {
+                    const float sgn = (par ^ ((neg >> p) & 1u)) ? -1.0f : 1.0f;  // product of the others
+                    const float g_c2v = gb[p] - gc[p];
+                    if constexpr (D >= 2) {
+                        const float m = sgn * ((p == (int)idx1) ? min2 : min1);
+                        gw[p] += g_c2v * m;
+                        if (t >= 2) {
+                            const uint32_t a = (p == (int)idx1) ? idx2 : idx1;
+                            const float sa = ((neg >> a) & 1u) ? -1.0f : 1.0f;
+                            const float gv = g_c2v * wt[p] * sgn * sa;
+#pragma unroll
+                            for (int q = 0; q < D; ++q)
+                                if ((uint32_t)q == a) gx[q] += gv;
+                        }
+                    }
+                }
+                if (t >= 2) {
+#pragma unroll
+                    for (int p = 0; p < D; ++p) g_prev[(size_t)f * E + (size_t)(e0 + p) * cv.z + i] = gx[p];
+                }
+            }
+            // fixed-order reduction of the weight gradients over the lanes
+#pragma unroll
+            for (int p = 0; p < D; ++p) {
+                const float s = wave_sum(gw[p]);
+                if (lane == 0) red[p][wave] = s;
+            }
+            __syncthreads();
+            if (threadIdx.x < (uint32_t)D) {
+                float s = 0.0f;
+                for (uint32_t w = 0; w < nwave; ++w) s += red[threadIdx.x][w];
+                gpart[(size_t)(e0 + threadIdx.x) * frames + f] = s;
+            }
+            __syncthreads();
+        });
+    }
+}
+
+// grad[te] = sum over frames of gpart[te][f], fixed-order tree.
+__global__ void __launch_bounds__(256)
+grad_reduce_kernel(const float *__restrict__ gpart, uint32_t frames, float *__restrict__ grad) {
+    __shared__ float sh[256];
+    const float *row = gpart + (size_t)blockIdx.x * frames;
+    float s = 0.0f;
+    for (uint32_t f = threadIdx.x; f < frames; f += blockDim.x) s += row[f];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) grad[blockIdx.x] = sh[0];
+}
+
+// Adam (Kingma & Ba 2015), bias-corrected, beta1 0.9, beta2 0.999, eps 1e-8.
+__global__ void adam_kernel(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
+                            const float *__restrict__ g, uint32_t n, float lr, float bc1, float bc2) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = 0.9f * m[i] + 0.1f * gi;
+    const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + 1e-8f);
+}
+
+__global__ void iota_kernel(uint32_t *list, uint32_t *count, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) list[i] = i;
+    if (i == 0) *count = n;
+}
+
+uint32_t bwd_block(uint32_t z) {
+    uint32_t b = ((z + 63) / 64) * 64;
+    return b > 512 ? 512 : b;
+}
+
+void free_train(gf2l_nms *q) {
+    hipFree(q->tape_state);
+    hipFree(q->tape_bel);
+    hipFree(q->g_a);
+    hipFree(q->g_b);
+    hipFree(q->g_bel);
+    hipFree(q->gpart);
+    hipFree(q->grad);
+    hipFree(q->loss);
+    hipFree(q->list);
+    hipFree(q->count);
+    q->tape_state = nullptr;
+    q->tape_bel = nullptr;
+    q->g_a = q->g_b = q->g_bel = q->gpart = q->grad = q->loss = nullptr;
+    q->list = q->count = nullptr;
+    q->cap = 0;
+}
+
+int ensure_train(gf2l_ctx *ctx, gf2l_nms *q, size_t F) {
+    if (q->cap >= F) return GF2L_OK;
+    free_train(q);
+    const CodeView &cv = q->code->view;
+    const size_t T = q->T, E = (size_t)cv.nbe * cv.z;
+    GF2L_HIP(ctx, hipMalloc(&q->tape_state, T * F * 3 * cv.mb * cv.zp * sizeof(uint32_t)));
+    GF2L_HIP(ctx, hipMalloc(&q->tape_bel, T * F * cv.nb * cv.zp * sizeof(float)));
+    GF2L_HIP(ctx, hipMalloc(&q->g_a, F * E * sizeof(float)));
+    GF2L_HIP(ctx, hipMalloc(&q->g_b, F * E * sizeof(float)));
+    GF2L_HIP(ctx, hipMalloc(&q->g_bel, F * cv.nb * cv.zp * sizeof(float)));
+    GF2L_HIP(ctx, hipMalloc(&q->gpart, T * cv.nbe * F * sizeof(float)));
+    GF2L_HIP(ctx, hipMalloc(&q->grad, T * cv.nbe * sizeof(float)));
+    GF2L_HIP(ctx, hipMalloc(&q->loss, sizeof(float)));
+    GF2L_HIP(ctx, hipMalloc(&q->list, F * sizeof(uint32_t)));
+    GF2L_HIP(ctx, hipMalloc(&q->count, sizeof(uint32_t)));
+    q->cap = F;
+    return GF2L_OK;
+}
+
+// Forward pass with tape + backward pass; leaves the summed gradient in
+// q->grad and the summed loss in q->loss.
+int forward_backward(gf2l_ctx *ctx, gf2l_nms *q, const float *d_llr, const uint64_t *d_cw, size_t frames) {
+    const CodeView &cv = q->code->view;
+    if (cv.max_dc > kMaxStateDegree) return fail(ctx, GF2L_ERR_UNSUPPORTED, "check degree above 27");
+    if (frames == 0 || frames > 0xffffffffull) return fail(ctx, GF2L_ERR_INVALID, "bad frame count");
+    const uint32_t F = (uint32_t)frames, T = q->T;
+    int rc = ensure_train(ctx, q, F);
+    if (rc != GF2L_OK) return rc;
+    hipStream_t s = ctx->stream;
+    const size_t sstride = (size_t)F * 3 * cv.mb * cv.zp, bstride = (size_t)F * cv.nb * cv.zp;
+    hipLaunchKernelGGL(iota_kernel, dim3((F + 255) / 256), dim3(256), 0, s, q->list, q->count, F);
+    GF2L_HIP(ctx, hipGetLastError());
+    for (uint32_t t = 1; t <= T; ++t) {
+        const uint32_t *st_in = t >= 2 ? q->tape_state + (t - 2) * sstride : nullptr;
+        const float *bel_in = t >= 2 ? q->tape_bel + (t - 2) * bstride : nullptr;
+        GF2L_HIP(ctx, launch_state_forward(cv, q->list, q->count, F, t == 1, st_in,
+                                           q->tape_state + (t - 1) * sstride, bel_in,
+                                           q->tape_bel + (t - 1) * bstride, d_llr,
+                                           t >= 2 ? q->d_w + (size_t)(t - 2) * cv.nbe : nullptr,
+                                           q->d_w + (size_t)(t - 1) * cv.nbe, ctx->num_cus, ctx->tune, s));
+    }
+    GF2L_HIP(ctx, hipMemsetAsync(q->loss, 0, sizeof(float), s));
+    const uint32_t block = bwd_block(cv.z);
+    const uint32_t gv = std::min<uint32_t>(F * cv.nb, (uint32_t)ctx->num_cus * 4u);
+    const uint32_t gc = std::min<uint32_t>(F * cv.mb, (uint32_t)ctx->num_cus * 4u);
+    const float scale = 1.0f / ((float)T * (float)cv.n * (float)F);
+    float *g_cur = q->g_a, *g_prev = q->g_b;
+    for (uint32_t t = T; t >= 1; --t) {
+        const int has_gcur = t < T;
+        const float *bel_t = q->tape_bel + (t - 1) * bstride;
+        if (cv.max_dv > 16)
+            hipLaunchKernelGGL(vn_bwd_kernel<32>, dim3(gv), dim3(block), 0, s, cv, F, bel_t, d_cw, g_cur,
+                               has_gcur, q->g_bel, scale, q->loss);
+        else
+            hipLaunchKernelGGL(vn_bwd_kernel<16>, dim3(gv), dim3(block), 0, s, cv, F, bel_t, d_cw, g_cur,
+                               has_gcur, q->g_bel, scale, q->loss);
+        GF2L_HIP(ctx, hipGetLastError());
+        const uint32_t *st_prev = t >= 2 ? q->tape_state + (t - 2) * sstride : nullptr;
+        const float *bel_prev = t >= 2 ? q->tape_bel + (t - 2) * bstride : nullptr;
+        const float *w_prev = t >= 2 ? q->d_w + (size_t)(t - 2) * cv.nbe : nullptr;
+        const float *w_t = q->d_w + (size_t)(t - 1) * cv.nbe;
+        float *gpart_t = q->gpart + (size_t)(t - 1) * cv.nbe * F;
+        if (cv.max_dc > 16)
+            hipLaunchKernelGGL(cn_bwd_kernel<27>, dim3(gc), dim3(block), 0, s, cv, F, t, d_llr, st_prev,
+                               bel_prev, w_prev, w_t, q->g_bel, g_cur, has_gcur, g_prev, gpart_t);
+        else
+            hipLaunchKernelGGL(cn_bwd_kernel<16>, dim3(gc), dim3(block), 0, s, cv, F, t, d_llr, st_prev,
+                               bel_prev, w_prev, w_t, q->g_bel, g_cur, has_gcur, g_prev, gpart_t);
+        GF2L_HIP(ctx, hipGetLastError());
+        std::swap(g_cur, g_prev);
+    }
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3(T * cv.nbe), dim3(256), 0, s, q->gpart, F, q->grad);
+    GF2L_HIP(ctx, hipGetLastError());
+    return GF2L_OK;
+}
+
+}  // namespace
+}  // namespace gf2l
+
+using gf2l::fail;
+
+extern "C" {
+
+int gf2l_nms_create(gf2l_ctx *ctx, const gf2l_code *code, uint32_t iterations,
+                    const float *init_weights, float init_value, gf2l_nms **out) {
+    if (!ctx || !out) return GF2L_ERR_INVALID;
+    *out = nullptr;
+    if (!code || iterations == 0) return fail(ctx, GF2L_ERR_INVALID, "need a code and iterations >= 1");
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    gf2l_nms *q = new gf2l_nms;
+    q->code = code;
+    q->T = iterations;
+    q->nbe = code->view.nbe;
+    const size_t nw = (size_t)q->T * q->nbe;
+    std::vector<float> w(nw, init_value);
+    if (init_weights) std::memcpy(w.data(), init_weights, nw * sizeof(float));
+    hipError_t e = hipMalloc(&q->d_w, nw * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&q->d_m, nw * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&q->d_v, nw * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(q->d_w, w.data(), nw * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(q->d_m, 0, nw * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(q->d_v, 0, nw * sizeof(float));
+    if (e != hipSuccess) {
+        hipFree(q->d_w);
+        hipFree(q->d_m);
+        hipFree(q->d_v);
+        delete q;
+        return fail(ctx, GF2L_ERR_HIP, std::string("nms create: ") + hipGetErrorString(e));
+    }
+    *out = q;
+    return GF2L_OK;
+}
+
+void gf2l_nms_destroy(gf2l_nms *q) {
+    if (!q) return;
+    hipSetDevice(q->code->device);
+    hipDeviceSynchronize();
+    gf2l::free_train(q);
+    hipFree(q->d_w);
+    hipFree(q->d_m);
+    hipFree(q->d_v);
+    delete q;
+}
+
+int gf2l_nms_info(const gf2l_nms *q, uint32_t *iterations, uint32_t *weights_per_iteration) {
+    if (!q) return GF2L_ERR_INVALID;
+    if (iterations) *iterations = q->T;
+    if (weights_per_iteration) *weights_per_iteration = q->nbe;
+    return GF2L_OK;
+}
+
+int gf2l_nms_get_weights(gf2l_ctx *ctx, const gf2l_nms *q, float *w) {
+    if (!ctx || !q || !w) return GF2L_ERR_INVALID;
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    GF2L_HIP(ctx, hipMemcpy(w, q->d_w, (size_t)q->T * q->nbe * sizeof(float), hipMemcpyDeviceToHost));
+    return GF2L_OK;
+}
+
+int gf2l_nms_set_weights(gf2l_ctx *ctx, gf2l_nms *q, const float *w) {
+    if (!ctx || !q || !w) return GF2L_ERR_INVALID;
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    GF2L_HIP(ctx, hipMemcpy(q->d_w, w, (size_t)q->T * q->nbe * sizeof(float), hipMemcpyHostToDevice));
+    return GF2L_OK;
+}
+
+int gf2l_nms_get_adam(gf2l_ctx *ctx, const gf2l_nms *q, float *m, float *v, uint64_t *step) {
+    if (!ctx || !q) return GF2L_ERR_INVALID;
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t b = (size_t)q->T * q->nbe * sizeof(float);
+    if (m) GF2L_HIP(ctx, hipMemcpy(m, q->d_m, b, hipMemcpyDeviceToHost));
+    if (v) GF2L_HIP(ctx, hipMemcpy(v, q->d_v, b, hipMemcpyDeviceToHost));
+    if (step) *step = q->step;
+    return GF2L_OK;
+}
+
+int gf2l_nms_set_adam(gf2l_ctx *ctx, gf2l_nms *q, const float *m, const float *v, uint64_t step) {
+    if (!ctx || !q || !m || !v) return GF2L_ERR_INVALID;
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t b = (size_t)q->T * q->nbe * sizeof(float);
+    GF2L_HIP(ctx, hipMemcpy(q->d_m, m, b, hipMemcpyHostToDevice));
+    GF2L_HIP(ctx, hipMemcpy(q->d_v, v, b, hipMemcpyHostToDevice));
+    q->step = step;
+    return GF2L_OK;
+}
+
+int gf2l_nms_gradient(gf2l_ctx *ctx, gf2l_nms *q, const float *d_llr, const uint64_t *d_cw,
+                      size_t frames, float *grad, float *loss) {
+    if (!ctx || !q || !d_llr || !d_cw) return GF2L_ERR_INVALID;
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = gf2l::forward_backward(ctx, q, d_llr, d_cw, frames);
+    if (rc != GF2L_OK) return rc;
+    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (grad)
+        GF2L_HIP(ctx, hipMemcpy(grad, q->grad, (size_t)q->T * q->nbe * sizeof(float), hipMemcpyDeviceToHost));
+    if (loss) {
+        float l = 0.0f;
+        GF2L_HIP(ctx, hipMemcpy(&l, q->loss, sizeof(float), hipMemcpyDeviceToHost));
+        *loss = l / ((float)q->T * (float)q->code->view.n * (float)frames);
+    }
+    return GF2L_OK;
+}
+
+int gf2l_nms_train_step(gf2l_ctx *ctx, gf2l_nms *q, const float *d_llr, const uint64_t *d_cw,
+                        size_t frames, float lr, float *loss) {
+    if (!ctx || !q || !d_llr || !d_cw) return GF2L_ERR_INVALID;
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = gf2l::forward_backward(ctx, q, d_llr, d_cw, frames);
+    if (rc != GF2L_OK) return rc;
+    q->step += 1;
+    const uint32_t n = q->T * q->nbe;
+    const float bc1 = 1.0f - std::pow(0.9f, (float)q->step);
+    const float bc2 = 1.0f - std::pow(0.999f, (float)q->step);
+    hipLaunchKernelGGL(gf2l::adam_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, q->d_w,
+                       q->d_m, q->d_v, q->grad, n, lr, bc1, bc2);
+    GF2L_HIP(ctx, hipGetLastError());
+    if (loss) {
+        float l = 0.0f;
+        GF2L_HIP(ctx, hipMemcpyAsync(&l, q->loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        *loss = l / ((float)q->T * (float)q->code->view.n * (float)frames);
+    }
+    return GF2L_OK;
+}
+
+int gf2l_nms_decode_device(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr, size_t frames,
+                           uint32_t fallback_max_it, uint64_t *d_msg_words, uint32_t *d_iterations,
+                           uint8_t *d_converged, uint8_t *d_syndrome_ok, uint8_t *d_fallback) {
+    if (!ctx || !q) return GF2L_ERR_INVALID;
+    if (frames == 0) return GF2L_OK;
+    if (!d_llr || !d_msg_words || !d_iterations || !d_converged || !d_syndrome_ok)
+        return fail(ctx, GF2L_ERR_INVALID, "null buffer");
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    return gf2l::decode_device(ctx, q->code, d_llr, frames, q->T, q->d_w, fallback_max_it, d_msg_words,
+                               d_iterations, d_converged, d_syndrome_ok, d_fallback, nullptr, nullptr, 0);
+}
+
+}  // extern "C"

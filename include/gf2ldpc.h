@@ -120,6 +120,32 @@ void gf2l_code_destroy(gf2l_code *code);
 int gf2l_qc_ira_shifts(uint64_t seed, uint32_t mb, uint32_t nb, uint32_t z,
                        const uint32_t *info_col_weight, int32_t *shifts_out);
 
+/* ---- device memory (for callers without their own HIP allocator) ------ */
+
+/* hipMalloc / hipFree on the context's device; copies are synchronous on the
+ * context's stream.  No reference counterpart. */
+int gf2l_device_alloc(gf2l_ctx *ctx, size_t bytes, void **out);
+int gf2l_device_free(gf2l_ctx *ctx, void *ptr);
+int gf2l_copy_to_device(gf2l_ctx *ctx, void *dst, const void *src, size_t bytes);
+int gf2l_copy_to_host(gf2l_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* ---- frame generation ----------------------------------------------------- */
+
+/* Generates `frames` frames of a staircase QC-IRA code on the device (frame
+ * ids first_frame .. first_frame+frames-1): Philox4x32-10 information bits,
+ * systematic staircase encoding, BPSK (bit 0 -> +1) over AWGN with a
+ * per-frame Eb/N0 drawn uniformly in [ebn0_lo_db, ebn0_hi_db] (equal bounds
+ * give a fixed Eb/N0), sigma^2 = 1/(2 R 10^(EbN0/10)) and LLR = 2y/sigma^2
+ * computed in double -- the channel of crates/gf2-coding/src/channel.rs:51-57,
+ * 100-102, 153-158 with a counter-based noise stream.  Writes d_llr
+ * (frames*n floats, frame-major) and, when d_codeword_words is not NULL, the
+ * codewords (frames*ceil(n/64) LE words; the first k bits are the message).
+ * Queued on the context's stream.  GF2L_ERR_UNSUPPORTED unless the code's
+ * parity part is the dual-diagonal staircase and z >= 64. */
+int gf2l_frames_generate(gf2l_ctx *ctx, const gf2l_code *code, uint64_t seed, uint64_t first_frame,
+                         size_t frames, float ebn0_lo_db, float ebn0_hi_db, float *d_llr,
+                         uint64_t *d_codeword_words);
+
 /* ---- decoding --------------------------------------------------------- */
 
 /* LdpcDecoder::decode_batch (core.rs:729-756) on host buffers.
@@ -157,6 +183,52 @@ int gf2l_decode_trace(gf2l_ctx *ctx, const gf2l_code *code, const float *llr, si
                       uint32_t max_it, uint32_t trace_iters, float *c2v_out, float *belief_out,
                       uint64_t *msg_words, uint32_t *iterations, uint8_t *converged,
                       uint8_t *syndrome_ok);
+
+/* ---- learned normalized min-sum decoder ----------------------------------
+ * Weighted min-sum with one weight per base-graph edge e and iteration t:
+ *   c2v_t(e) = w_t(e) * minsum(...)
+ * -- the reference's alpha * min-sum rule (Llr::boxplus_normalized_minsum_n,
+ * crates/gf2-coding/src/llr.rs:352-355) with a learned alpha per edge and
+ * iteration; all other steps are the flooding schedule of decode_iterative
+ * (core.rs:889-938).  Weights are stored [iteration][edge] with edges in
+ * block-row-major order (block-columns ascending within a block-row).  The
+ * reference has no learned decoder (ROADMAP.md:93 plans one); these calls
+ * have no reference counterpart beyond that rule. */
+typedef struct gf2l_nms gf2l_nms;
+
+/* Creates a learned decoder of `iterations` weighted iterations for `code`
+ * (which must outlive it); weights from init_weights (host, iterations*nbe)
+ * or all equal to init_value. */
+int gf2l_nms_create(gf2l_ctx *ctx, const gf2l_code *code, uint32_t iterations,
+                    const float *init_weights, float init_value, gf2l_nms **out);
+void gf2l_nms_destroy(gf2l_nms *nms);
+int gf2l_nms_info(const gf2l_nms *nms, uint32_t *iterations, uint32_t *weights_per_iteration);
+int gf2l_nms_get_weights(gf2l_ctx *ctx, const gf2l_nms *nms, float *weights);
+int gf2l_nms_set_weights(gf2l_ctx *ctx, gf2l_nms *nms, const float *weights);
+/* Adam state for checkpoint / resume (host buffers of iterations*nbe). */
+int gf2l_nms_get_adam(gf2l_ctx *ctx, const gf2l_nms *nms, float *m, float *v, uint64_t *step);
+int gf2l_nms_set_adam(gf2l_ctx *ctx, gf2l_nms *nms, const float *m, const float *v, uint64_t step);
+
+/* Loss and gradient on a batch (device LLRs d_llr [frames][n], device
+ * codewords d_cw [frames][ceil(n/64)] as written by gf2l_frames_generate):
+ * loss = mean over frames, iterations 1..T and variables of
+ * softplus(-y*belief), y = +1 for bit 0 / -1 for bit 1 (BCE of
+ * P(bit=1) = sigmoid(-belief)); grad (host, iterations*nbe) = d loss / d w.
+ * Check degree <= 27. */
+int gf2l_nms_gradient(gf2l_ctx *ctx, gf2l_nms *nms, const float *d_llr, const uint64_t *d_cw,
+                      size_t frames, float *grad, float *loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, learning rate lr) on the
+ * gradient of that loss; *loss (if not NULL) receives the batch loss. */
+int gf2l_nms_train_step(gf2l_ctx *ctx, gf2l_nms *nms, const float *d_llr, const uint64_t *d_cw,
+                        size_t frames, float lr, float *loss);
+/* Decodes like gf2l_decode_batch_device with the learned rule for the
+ * decoder's iterations; frames whose syndrome still fails are decoded again
+ * from their LLRs by plain min-sum for up to fallback_max_it iterations
+ * (0: no fallback).  d_iterations counts weighted plus fallback iterations;
+ * d_fallback (may be NULL) marks re-decoded frames. */
+int gf2l_nms_decode_device(gf2l_ctx *ctx, const gf2l_nms *nms, const float *d_llr, size_t frames,
+                           uint32_t fallback_max_it, uint64_t *d_msg_words, uint32_t *d_iterations,
+                           uint8_t *d_converged, uint8_t *d_syndrome_ok, uint8_t *d_fallback);
 
 #ifdef __cplusplus
 }

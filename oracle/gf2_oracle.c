@@ -437,3 +437,57 @@ void or_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]
     }
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
+
+/* ------------------------------------------------------------------ */
+/* Frame generator restatement (one frame, bit by bit), the checker of the
+ * device generator: Philox4x32-10 information bits, staircase QC-IRA
+ * encoding, BPSK/AWGN with LLR = (float)(2 y / sigma^2) in double
+ * (crates/gf2-coding/src/channel.rs:51-57, 100-102, 153-158). */
+void or_framegen(const int32_t *shifts, uint32_t mb, uint32_t nb, uint32_t z, uint64_t seed,
+                 uint64_t g, float ebn0_lo, float ebn0_hi, float *llr, uint64_t *cw_words) {
+    const uint32_t kb = nb - mb, zw = (z + 63) / 64, n = nb * z;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint8_t *bit = (uint8_t *)calloc(n, 1);
+    uint32_t o[4];
+    for (uint32_t bc = 0; bc < kb; ++bc)
+        for (uint32_t j = 0; j < z; ++j) {
+            const uint32_t ctr[4] = {bc * zw + j / 64, (uint32_t)g, (uint32_t)(g >> 32), 0};
+            or_philox4x32(ctr, key, o);
+            const uint64_t w = (uint64_t)o[0] | ((uint64_t)o[1] << 32);
+            bit[bc * z + j] = (uint8_t)((w >> (j % 64)) & 1u);
+        }
+    for (uint32_t br = 0; br < mb; ++br)
+        for (uint32_t i = 0; i < z; ++i) {
+            uint8_t sacc = 0;
+            for (uint32_t bc = 0; bc < kb; ++bc) {
+                const int32_t sh = shifts[(size_t)br * nb + bc];
+                if (sh >= 0) sacc ^= bit[bc * z + (i + (uint32_t)sh) % z];
+            }
+            const uint8_t prev = br ? bit[(kb + br - 1) * z + i] : 0;
+            bit[(kb + br) * z + i] = sacc ^ prev;
+        }
+    const uint32_t c2[4] = {0, (uint32_t)g, (uint32_t)(g >> 32), 2};
+    or_philox4x32(c2, key, o);
+    const double rate = (double)(kb * z) / (double)n;
+    const double u = ((double)o[0] + 0.5) * (1.0 / 4294967296.0);
+    const double ebn0 = (double)ebn0_lo + ((double)ebn0_hi - (double)ebn0_lo) * u;
+    const double s2 = 1.0 / (2.0 * rate * pow(10.0, ebn0 / 10.0));
+    const double sigma = sqrt(s2);
+    for (uint32_t v = 0; v < n; ++v) {
+        const uint32_t c1[4] = {v / 2, (uint32_t)g, (uint32_t)(g >> 32), 1};
+        or_philox4x32(c1, key, o);
+        const double u1 = ((double)o[0] + 0.5) * (1.0 / 4294967296.0);
+        const double u2 = ((double)o[1] + 0.5) * (1.0 / 4294967296.0);
+        const double r = sqrt(-2.0 * log(u1));
+        const double th = 6.283185307179586476925286766559 * u2;
+        const double nz = (v % 2 == 0) ? r * cos(th) : r * sin(th);
+        const double x = bit[v] ? -1.0 : 1.0;
+        llr[v] = (float)(2.0 * (x + sigma * nz) / s2);
+    }
+    if (cw_words) {
+        memset(cw_words, 0, ((n + 63) / 64) * sizeof(uint64_t));
+        for (uint32_t v = 0; v < n; ++v)
+            if (bit[v]) cw_words[v >> 6] |= (uint64_t)1 << (v & 63);
+    }
+    free(bit);
+}

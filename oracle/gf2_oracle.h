@@ -108,6 +108,11 @@ double or_nms_loss_grad(const or_code *c, const float *llr, const uint8_t *bits,
 /* Philox4x32-10 counter-based generator (Salmon et al., SC'11). */
 void or_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+/* Restatement of the device frame generator for one frame with global id g
+ * (see gf2l_frames_generate): writes n LLRs and ceil(n/64) codeword words. */
+void or_framegen(const int32_t *shifts, uint32_t mb, uint32_t nb, uint32_t z, uint64_t seed,
+                 uint64_t g, float ebn0_lo, float ebn0_hi, float *llr, uint64_t *cw_words);
+
 #ifdef __cplusplus
 }
 #endif

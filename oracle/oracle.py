@@ -67,6 +67,10 @@ def load():
         lib.or_nms_loss_grad.argtypes = [ctypes.POINTER(OrCode), vp, vp, ctypes.c_uint32, vp, vp]
         lib.or_philox4x32.restype = None
         lib.or_philox4x32.argtypes = [vp, vp, vp]
+        lib.or_framegen.restype = None
+        lib.or_framegen.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                    ctypes.c_uint64, ctypes.c_uint64, ctypes.c_float,
+                                    ctypes.c_float, vp, vp]
         lib.port_code_new.restype = vp
         lib.port_code_new.argtypes = [ctypes.c_uint32, ctypes.c_uint32, vp, ctypes.c_size_t]
         lib.port_code_free.restype = None
@@ -179,6 +183,18 @@ class Code:
         g = np.zeros(T * self.nnz, dtype=np.float64)
         loss = load().or_nms_loss_grad(self._p, _p(llr), _p(b), T, _p(w), _p(g))
         return loss, g.reshape(T, self.nnz)
+
+
+def framegen(base: np.ndarray, z: int, seed: int, frame_id: int, ebn0_lo: float,
+             ebn0_hi: float):
+    """(llr [n] float32, codeword words [ceil(n/64)] uint64) of one frame."""
+    base = np.ascontiguousarray(base, dtype=np.int32)
+    mb, nb = base.shape
+    n = nb * z
+    llr = np.zeros(n, dtype=np.float32)
+    cw = np.zeros((n + 63) // 64, dtype=np.uint64)
+    load().or_framegen(_p(base), mb, nb, z, seed, frame_id, ebn0_lo, ebn0_hi, _p(llr), _p(cw))
+    return llr, cw
 
 
 def minsum_avx2(x) -> np.float32:

@@ -360,3 +360,32 @@ def test_encoder_produces_codewords():
     assert np.array_equal(cw[:, :spec.k], info)
     for f in range(8):
         assert code.syndrome_weight(cw[f]) == 0
+
+
+# ---------------------------------------------------------------- frame generator
+def test_framegen_restatement_is_a_coded_awgn_channel():
+    from gf2_amd import codes
+    spec = codes.n2304_r12()
+    base = spec.base_matrix()
+    code = orc.Code.from_qc(base, spec.z)
+    zw = (spec.z + 63) // 64
+    s2 = frames.sigma2(2.0, spec.k / spec.n)
+    ys = []
+    for g in range(6):
+        llr, cw = orc.framegen(base, spec.z, 99, g, 2.0, 2.0)
+        bits = frames.unpack_words(cw[None, :], spec.n)[0]
+        assert code.syndrome_weight(bits) == 0          # a codeword of H
+        # message bit 70 of block 0 comes from Philox({0*zw + 1, g, 0, 0})
+        o = orc.philox4x32([0 * zw + 70 // 64, g, 0, 0], [99, 0])
+        w = int(o[0]) | (int(o[1]) << 32)
+        assert bits[70] == (w >> (70 % 64)) & 1
+        ys.append(llr * (1 - 2 * bits.astype(np.float32)))
+    y = np.concatenate(ys).astype(np.float64)
+    # LLR given the sent symbol is N(2/s2, 4/s2)
+    assert abs(y.mean() - 2 / s2) < 0.05 * (2 / s2)
+    assert abs(y.var() - 4 / s2) < 0.05 * (4 / s2)
+    # different frames and seeds give different draws
+    a, _ = orc.framegen(base, spec.z, 99, 0, 2.0, 2.0)
+    b, _ = orc.framegen(base, spec.z, 99, 1, 2.0, 2.0)
+    c, _ = orc.framegen(base, spec.z, 98, 0, 2.0, 2.0)
+    assert not np.array_equal(a, b) and not np.array_equal(a, c)
