@@ -57,6 +57,8 @@ SIGNATURES = {
                                          ctypes.c_void_p]),
     "gf2l_code_qc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                     ctypes.c_uint32, _i32p, _c_void_pp]),
+    "gf2l_code_edges": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.c_void_p, ctypes.c_size_t, _c_void_pp]),
     "gf2l_code_info": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p, _u32p, _u64p]),
     "gf2l_code_destroy": (None, [ctypes.c_void_p]),
     "gf2l_qc_ira_shifts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,

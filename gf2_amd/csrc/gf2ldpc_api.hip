@@ -99,6 +99,12 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
         if (d_fb) GF2L_HIP(ctx, hipMemsetAsync(d_fb, 0, F, s));
         return GF2L_OK;
     }
+    if (code->is_csr) {
+        if (d_w || h_trace_c2v)
+            return fail(ctx, GF2L_ERR_UNSUPPORTED, "weights and traces need a quasi-cyclic code");
+        if (d_fb) GF2L_HIP(ctx, hipMemsetAsync(d_fb, 0, F, s));
+        return decode_csr(ctx, code, d_llr, F, max_it, d_msg, d_iters, d_conv, d_synd);
+    }
     const int path = resolve_path(ctx, cv);
     if (path < 0) return fail(ctx, GF2L_ERR_UNSUPPORTED, "the forced decoder path cannot run this code");
     if (path == gf2l::kLds) {
@@ -515,7 +521,7 @@ int gf2l_code_info(const gf2l_code *code, uint32_t *n, uint32_t *m, uint32_t *k,
     if (n) *n = code->view.n;
     if (m) *m = code->view.m;
     if (k) *k = code->view.k;
-    if (nnz) *nnz = (uint64_t)code->view.nbe * code->view.z;
+    if (nnz) *nnz = code->is_csr ? (uint64_t)code->csr.nnz : (uint64_t)code->view.nbe * code->view.z;
     return GF2L_OK;
 }
 

@@ -37,6 +37,17 @@ struct CodeView {
     const uint4 *col_info;       // nbe: {block-row, position in row, shift, row-order edge id}
 };
 
+// Device view of a general sparse code (LdpcCode::from_edges): CSR rows with
+// ascending columns and, per variable, its CSR edges in ascending check
+// order (the stable transpose of SpBitMatrix::transpose).
+struct CsrView {
+    uint32_t m, n, k, nnz;
+    const uint32_t *row_ptr;   // m+1
+    const uint32_t *col_idx;   // nnz
+    const uint32_t *col_ptr;   // n+1
+    const uint32_t *csc_edge;  // nnz: CSR edge index
+};
+
 // Decoder paths (values of gf2l_set_path).
 enum Path : int { kAuto = 0, kLds = 1, kStream = 2, kState = 3 };
 
@@ -88,6 +99,8 @@ struct IterArgs {
 }  // namespace gf2l
 
 struct gf2l_code {
+    bool is_csr = false;         // built by gf2l_code_edges (general sparse path)
+    gf2l::CsrView csr{};
     gf2l::CodeView view;         // device pointers
     std::vector<int32_t> shifts; // host copy of the base matrix
     std::vector<uint32_t> h_row_ptr, h_col_ptr;
@@ -137,6 +150,10 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
                   uint32_t max_it, const float *d_w, uint32_t fb_it, uint64_t *d_msg,
                   uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb,
                   float *h_trace_c2v, float *h_trace_bel, uint32_t trace_iters);
+
+// Plain min-sum on a general sparse code (csr.hip).
+int decode_csr(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, uint32_t F, uint32_t max_it,
+               uint64_t *d_msg, uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd);
 
 // Kernel launchers (gf2ldpc_kernels.hip).  All enqueue on `s`.
 size_t lds_path_bytes(const CodeView &cv);

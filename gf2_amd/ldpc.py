@@ -202,19 +202,37 @@ class QuasiCyclicLdpc:
 
 
 class LdpcCode:
-    """A quasi-cyclic LDPC code uploaded to one HIP device."""
+    """An LDPC code uploaded to one HIP device: quasi-cyclic (from a base
+    matrix) or general sparse (from an edge list)."""
 
-    def __init__(self, qc: QuasiCyclicLdpc, ctx: Optional[Context] = None):
-        qc._check_shifts()
+    def __init__(self, qc: Optional[QuasiCyclicLdpc], ctx: Optional[Context] = None,
+                 _edges: Optional[tuple] = None):
+        if qc is not None:
+            qc._check_shifts()
+        else:
+            m, n, edges = _edges
+            e = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+            bad = e[(e[:, 0] >= m) | (e[:, 0] < 0)]
+            if bad.size:  # SpBitMatrix::from_coo panics (sparse.rs:170-171)
+                raise ValueError(f"row index {bad[0, 0]} out of bounds (rows={m})")
+            bad = e[(e[:, 1] >= n) | (e[:, 1] < 0)]
+            if bad.size:
+                raise ValueError(f"col index {bad[0, 1]} out of bounds (cols={n})")
         self._ctx = ctx or default_context()
         self._qc = qc
         lib = _lib.load()
-        shifts = np.ascontiguousarray(qc._base, dtype=np.int32)
         h = ctypes.c_void_p()
-        mb, nb = shifts.shape
-        _lib.check(self._ctx.handle, lib.gf2l_code_qc(
-            self._ctx.handle, mb, nb, qc.expansion_factor(), _lib.ptr(shifts, ctypes.c_int32),
-            ctypes.byref(h)))
+        if qc is not None:
+            shifts = np.ascontiguousarray(qc._base, dtype=np.int32)
+            mb, nb = shifts.shape
+            _lib.check(self._ctx.handle, lib.gf2l_code_qc(
+                self._ctx.handle, mb, nb, qc.expansion_factor(), _lib.ptr(shifts, ctypes.c_int32),
+                ctypes.byref(h)))
+        else:
+            e32 = np.ascontiguousarray(e, dtype=np.uint32)
+            _lib.check(self._ctx.handle, lib.gf2l_code_edges(
+                self._ctx.handle, m, n, e32.ctypes.data if e32.size else None, e32.shape[0],
+                ctypes.byref(h)))
         self._h = h
         n, m, k = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
         nnz = ctypes.c_uint64()
@@ -224,6 +242,12 @@ class LdpcCode:
     @classmethod
     def from_quasi_cyclic(cls, qc: QuasiCyclicLdpc, ctx: Optional[Context] = None) -> "LdpcCode":
         return cls(qc, ctx)
+
+    @classmethod
+    def from_edges(cls, m: int, n: int, edges, ctx: Optional[Context] = None) -> "LdpcCode":
+        """LdpcCode::from_edges (core.rs:96-104): (check, variable) pairs,
+        duplicates cancel in pairs as in SpBitMatrix::from_coo."""
+        return cls(None, ctx, _edges=(int(m), int(n), edges))
 
     @property
     def handle(self):

@@ -105,6 +105,15 @@ int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value);
  * with the text of core.rs:589-596. */
 int gf2l_code_qc(gf2l_ctx *ctx, uint32_t mb, uint32_t nb, uint32_t z, const int32_t *shifts,
                  gf2l_code **out);
+/* General sparse code from (row, col) pairs: LdpcCode::from_edges
+ * (core.rs:96-104) -> SpBitMatrixDual::from_coo (crates/gf2-core/src/
+ * sparse.rs:166-204, 297-331): duplicates cancel in pairs (XOR), rows sorted,
+ * columns by stable transpose.  An out-of-range index returns
+ * GF2L_ERR_INVALID with the text of sparse.rs:170-171.  Decoded by a CSR/CSC
+ * path with the same arithmetic (plain min-sum only: no weights, no traces).
+ * Check degree <= 32. */
+int gf2l_code_edges(gf2l_ctx *ctx, uint32_t m, uint32_t n, const uint32_t *rc_pairs, size_t nnz,
+                    gf2l_code **out);
 /* n = nb*z (LdpcCode::n), m = mb*z (LdpcCode::m), k = n - m (LdpcCode::k,
  * core.rs:107-119), nnz = number of Tanner-graph edges. */
 int gf2l_code_info(const gf2l_code *code, uint32_t *n, uint32_t *m, uint32_t *k, uint64_t *nnz);

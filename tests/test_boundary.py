@@ -122,3 +122,11 @@ def test_qc_ira_rejects_bad_arguments():
         qc_ira_base_matrix(1, 4, 4, 8, [])
     with pytest.raises(ValueError):
         qc_ira_base_matrix(1, 2, 4, 8, [3, 1])  # weight above mb
+
+
+def test_from_edges_bounds_messages():  # sparse.rs:170-171 panics
+    from gf2_amd.ldpc import LdpcCode
+    with pytest.raises(ValueError, match=r"row index 3 out of bounds \(rows=2\)"):
+        LdpcCode.from_edges(2, 4, [(0, 0), (3, 1)])
+    with pytest.raises(ValueError, match=r"col index 9 out of bounds \(cols=4\)"):
+        LdpcCode.from_edges(2, 4, [(0, 9)])
