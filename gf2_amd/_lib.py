@@ -26,6 +26,7 @@ PATH_STATE = 3
 
 TUNE_WG_PER_CU = 1
 TUNE_XCD_MAP = 2
+TUNE_GRAPH_ITERS = 3
 
 _c_void_pp = ctypes.POINTER(ctypes.c_void_p)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -95,6 +96,13 @@ SIGNATURES = {
                                               ctypes.c_size_t, ctypes.c_uint32, ctypes.c_void_p,
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_void_p]),
+    "gf2l_codeset_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                           _c_void_pp]),
+    "gf2l_codeset_destroy": (None, [ctypes.c_void_p]),
+    "gf2l_decode_mixed_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_size_t, ctypes.c_void_p, ctypes.c_uint32,
+                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p]),
 }
 
 _lib = None

@@ -17,45 +17,11 @@
 #include <string>
 #include <vector>
 
+#include "device_common.h"
 #include "gf2ldpc.h"
-#include "internal.h"
 
 namespace gf2l {
 namespace {
-
-struct Acc {
-    float min1, min2;
-    uint32_t idx;
-    uint64_t sb, lt;
-    __device__ void init() {
-        min1 = __builtin_inff();
-        min2 = __builtin_inff();
-        idx = 0;
-        sb = 0;
-        lt = 0;
-    }
-    __device__ void add(uint32_t p, float x) {
-        const float a = fabsf(x);
-        if (a < min1) {
-            min2 = min1;
-            min1 = a;
-            idx = p;
-        } else if (a < min2) {
-            min2 = a;
-        }
-        sb |= (uint64_t)(__float_as_uint(x) >> 31) << p;
-        lt |= (uint64_t)(x < 0.0f) << p;
-    }
-    __device__ float out(uint32_t d, uint32_t p) const {
-        if (d == 1) return 0.0f;
-        const float mag = (p == idx) ? min2 : min1;
-        const uint32_t l8 = (d - 1) & ~7u;
-        const uint64_t pb = 1ull << p, all = (1ull << d) - 1;
-        const uint64_t lane = (p < l8) ? (((1ull << (l8 + 1)) - 1) & ~pb) : ((1ull << l8) - 1);
-        const uint64_t rem = all & ~lane & ~pb;
-        return ((__popcll(sb & lane) ^ __popcll(lt & rem)) & 1u) ? -mag : mag;
-    }
-};
 
 __global__ void __launch_bounds__(256)
 cn_csr_kernel(CsrView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
@@ -69,7 +35,7 @@ cn_csr_kernel(CsrView cv, const uint32_t *__restrict__ list, const uint32_t *__r
         float *fc = c2v + (size_t)f * cv.nnz;
         const float *fb = belief + (size_t)f * cv.n;
         const uint32_t e0 = cv.row_ptr[r], d = cv.row_ptr[r + 1] - e0;
-        Acc acc;
+        CheckAcc acc;
         acc.init();
         for (uint32_t p = 0; p < d; ++p) acc.add(p, fb[cv.col_idx[e0 + p]] - fc[e0 + p]);
         for (uint32_t p = 0; p < d; ++p) fc[e0 + p] = acc.out(d, p);
@@ -150,13 +116,6 @@ __global__ void csr_init_kernel(uint32_t *list, uint32_t *counts, uint32_t frame
         counts[1] = 0;
     }
 }
-
-struct CsrWork {
-    size_t cap = 0, cap_nnz = 0, cap_n = 0;
-    float *c2v = nullptr, *bel = nullptr;
-    uint64_t *hb = nullptr;
-    uint32_t *lists = nullptr, *counts = nullptr, *h_count = nullptr;
-};
 
 }  // namespace
 

@@ -1,0 +1,314 @@
+// Device helpers shared by the decoder kernels: the min-sum check rule, the
+// compressed check state, circulant index arithmetic, degree dispatch, and the
+// per-item bodies of the check update, variable update and syndrome that the
+// single-code and mixed-code kernels both run.
+//
+// Arithmetic contract (bit-exact with the reference decoder for finite LLRs,
+// crates/gf2-coding/src/ldpc/core.rs:790-938):
+//   * check message = sign_product * min |v2c| over the other d-1 edges, exact
+//     (abs / compare / negate only); the sign of an input counts as negative
+//     by its sign bit inside the AVX2 8-lane chunks and by `value < 0` in the
+//     scalar remainder, as in crates/gf2-kernels-simd/src/llr.rs:117-180
+//     (matters only for -0.0); a degree-1 check sends +0.0 (core.rs:804-806).
+//   * belief = ((L + c2v_0) + c2v_1) + ... in ascending check order, then
+//     v2c = belief - c2v (core.rs:819-838): sequential binary32 adds, built
+//     with -ffp-contract=off so nothing fuses into an FMA.
+//   * hard decision belief < 0 (llr.rs:73-75); syndrome over GF(2) by XOR.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "internal.h"
+
+namespace gf2l {
+namespace {
+
+// Running state of one check while its d inputs stream past.
+struct CheckAcc {
+    float min1, min2;
+    uint32_t idx;
+    uint64_t sb, lt;  // per-position sign-bit and (x < 0) masks
+    __device__ void init() {
+        min1 = __builtin_inff();
+        min2 = __builtin_inff();
+        idx = 0;
+        sb = 0;
+        lt = 0;
+    }
+    __device__ void add(uint32_t p, float x) {
+        float a = fabsf(x);
+        if (a < min1) {
+            min2 = min1;
+            min1 = a;
+            idx = p;
+        } else if (a < min2) {
+            min2 = a;
+        }
+        sb |= (uint64_t)(__float_as_uint(x) >> 31) << p;
+        lt |= (uint64_t)(x < 0.0f) << p;
+    }
+    // Sign of the message to position p: product of the other inputs' signs,
+    // sign bit for inputs inside full 8-lane chunks of the extrinsic list,
+    // `< 0` for the remainder.
+    __device__ uint32_t neg(uint32_t d, uint32_t p) const {
+        const uint32_t l8 = (d - 1) & ~7u;
+        const uint64_t pb = 1ull << p;
+        const uint64_t all = (1ull << d) - 1;
+        const uint64_t lane = (p < l8) ? (((1ull << (l8 + 1)) - 1) & ~pb) : ((1ull << l8) - 1);
+        const uint64_t rem = all & ~lane & ~pb;
+        return (__popcll(sb & lane) ^ __popcll(lt & rem)) & 1u;
+    }
+    // Output sign bits of all d positions.  When sign bit and `< 0` agree
+    // for every input (no -0.0) the rule reduces to parity(all) ^ own sign.
+    template <int DMAX>
+    __device__ uint32_t signs(uint32_t d) const {
+        const uint64_t all = (1ull << d) - 1;
+        if (sb == lt) return (uint32_t)(((__popcll(sb) & 1u) ? all : 0ull) ^ sb);
+        uint32_t s = 0;
+#pragma unroll
+        for (int p = 0; p < DMAX; ++p)
+            if (p < (int)d) s |= neg(d, p) << p;
+        return s;
+    }
+    __device__ float out(uint32_t d, uint32_t p) const {
+        if (d == 1) return 0.0f;
+        const float mag = (p == idx) ? min2 : min1;
+        return neg(d, p) ? -mag : mag;
+    }
+};
+
+// Message of position p rebuilt from a compressed check state.
+__device__ __forceinline__ float recon(float m1, float m2, uint32_t meta, uint32_t p) {
+    const float mag = (p == (meta & 31u)) ? m2 : m1;
+    return ((meta >> (5 + p)) & 1u) ? -mag : mag;
+}
+
+__device__ __forceinline__ uint32_t wrap_add(uint32_t a, uint32_t b, uint32_t z) {
+    uint32_t s = a + b;
+    return s >= z ? s - z : s;
+}
+__device__ __forceinline__ uint32_t wrap_sub(uint32_t a, uint32_t b, uint32_t z) {
+    return a >= b ? a - b : a + z - b;
+}
+
+// Calls f(std::integral_constant<int, d>) for a wave-uniform degree d in
+// [0, MAXD], so per-edge loops are fully unrolled with no guards and every
+// load of an item is issued before the first wait.  Degree 0 (a variable or
+// check outside every edge) is a valid case.
+#define GF2L_DEG_CASE(K)                                              \
+    case K:                                                           \
+        if constexpr (K <= MAXD) f(std::integral_constant<int, K>{}); \
+        break;
+template <int MAXD, typename F>
+__device__ __forceinline__ void dispatch_degree(uint32_t d, F &&f) {
+    switch (d) {
+        GF2L_DEG_CASE(0) GF2L_DEG_CASE(1) GF2L_DEG_CASE(2) GF2L_DEG_CASE(3) GF2L_DEG_CASE(4)
+        GF2L_DEG_CASE(5) GF2L_DEG_CASE(6) GF2L_DEG_CASE(7) GF2L_DEG_CASE(8)
+        GF2L_DEG_CASE(9) GF2L_DEG_CASE(10) GF2L_DEG_CASE(11) GF2L_DEG_CASE(12)
+        GF2L_DEG_CASE(13) GF2L_DEG_CASE(14) GF2L_DEG_CASE(15) GF2L_DEG_CASE(16)
+        GF2L_DEG_CASE(17) GF2L_DEG_CASE(18) GF2L_DEG_CASE(19) GF2L_DEG_CASE(20)
+        GF2L_DEG_CASE(21) GF2L_DEG_CASE(22) GF2L_DEG_CASE(23) GF2L_DEG_CASE(24)
+        GF2L_DEG_CASE(25) GF2L_DEG_CASE(26) GF2L_DEG_CASE(27) GF2L_DEG_CASE(28)
+        GF2L_DEG_CASE(29) GF2L_DEG_CASE(30) GF2L_DEG_CASE(31) GF2L_DEG_CASE(32)
+        default: break;
+    }
+}
+#undef GF2L_DEG_CASE
+
+// ---------------------------------------------------------------------------
+// STATE-path item bodies.  A frame's check state is [3][mb][zp] u32
+// (min1, min2, argmin | output sign bits << 5), its beliefs [nb][zp] floats.
+
+// Check update of block-row br of one frame: threads map to the z lifted
+// checks; belief reads of a circulant are a rotated contiguous run.  si / st
+// point at the block-row's first min1 word (plane stride mb*zp).  In the
+// first iteration the inputs are the channel LLRs fb (stride z) with
+// c2v = 0; later fb is the frame's belief array (stride zp) and the previous
+// message is rebuilt from si (times its weight w_prev[e] for NMS).
+template <int MAXD, bool FIRST, bool NMS>
+__device__ __forceinline__ void cn_state_item(const CodeView &cv, uint32_t br, const uint32_t *si,
+                                              uint32_t *st, const float *__restrict__ fb,
+                                              const float *__restrict__ w_prev) {
+    const size_t plane = (size_t)cv.mb * cv.zp;
+    const uint32_t bstride = FIRST ? cv.z : cv.zp;
+    const uint32_t e0 = cv.row_ptr[br];
+    const uint32_t d = cv.row_ptr[br + 1] - e0;
+    dispatch_degree<MAXD>(d, [&](auto DC) {
+        constexpr int D = decltype(DC)::value, DA = D > 0 ? D : 1;
+        uint2 re[DA];
+        float wp[DA];
+#pragma unroll
+        for (int p = 0; p < D; ++p) {
+            re[p] = cv.row_edge[e0 + p];
+            if constexpr (NMS && !FIRST) wp[p] = w_prev[e0 + p];
+        }
+        for (uint32_t i = threadIdx.x; i < cv.z; i += blockDim.x) {
+            float xv[DA];
+#pragma unroll
+            for (int p = 0; p < D; ++p) xv[p] = fb[re[p].x * bstride + wrap_add(i, re[p].y, cv.z)];
+            float m1o = 0.0f, m2o = 0.0f;
+            uint32_t mo = 0;
+            if constexpr (!FIRST) {
+                m1o = __uint_as_float(si[i]);
+                m2o = __uint_as_float(si[plane + i]);
+                mo = si[2 * plane + i];
+            }
+            CheckAcc acc;
+            acc.init();
+#pragma unroll
+            for (int p = 0; p < D; ++p) {
+                float v = xv[p];
+                if constexpr (!FIRST) {
+                    float c = recon(m1o, m2o, mo, p);
+                    if constexpr (NMS) c = wp[p] * c;
+                    v = v - c;
+                }
+                acc.add(p, v);
+            }
+            float m1 = acc.min1, m2 = acc.min2;
+            uint32_t meta = acc.idx | (acc.signs<D>(D) << 5);
+            if constexpr (D == 1) {  // the lone message is +0.0
+                m1 = 0.0f;
+                m2 = 0.0f;
+                meta = 0;
+            }
+            st[i] = __float_as_uint(m1);
+            st[plane + i] = __float_as_uint(m2);
+            st[2 * plane + i] = meta;
+        }
+    });
+}
+
+// Variable update of the z variables of block-column bc of one frame:
+// belief = L + sum of the rebuilt check messages in ascending block-row
+// order; hard decisions packed by wave ballot into 64-bit words (fh may be
+// null).  fs: frame's state, fl: the block-column's LLRs, fb: its beliefs.
+// Needs blockDim.x % 64 == 0.
+template <int MAXD, bool NMS>
+__device__ __forceinline__ void vn_state_item(const CodeView &cv, uint32_t bc, const uint32_t *__restrict__ fs,
+                                              const float *__restrict__ fl, float *__restrict__ fb,
+                                              uint64_t *__restrict__ fh, const float *__restrict__ w_cur) {
+    const size_t plane = (size_t)cv.mb * cv.zp;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t c0 = cv.col_ptr[bc];
+    const uint32_t dv = cv.col_ptr[bc + 1] - c0;
+    dispatch_degree<MAXD>(dv, [&](auto DC) {
+        constexpr int D = decltype(DC)::value, DA = D > 0 ? D : 1;
+        uint4 ci[DA];
+        float wq[DA];
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+            ci[q] = cv.col_info[c0 + q];
+            if constexpr (NMS) wq[q] = w_cur[ci[q].w];
+        }
+        for (uint32_t j0 = 0; j0 < cv.z; j0 += blockDim.x) {
+            const uint32_t j = j0 + threadIdx.x;
+            const bool valid = j < cv.z;
+            float bel = 0.0f;
+            if (valid) {
+                float m1[DA], m2[DA];
+                uint32_t mt[DA];
+#pragma unroll
+                for (int q = 0; q < D; ++q) {
+                    const uint32_t *sp = fs + (size_t)ci[q].x * cv.zp + wrap_sub(j, ci[q].z, cv.z);
+                    m1[q] = __uint_as_float(sp[0]);
+                    m2[q] = __uint_as_float(sp[plane]);
+                    mt[q] = sp[2 * plane];
+                }
+                bel = fl[j];
+#pragma unroll
+                for (int q = 0; q < D; ++q) {
+                    float c = recon(m1[q], m2[q], mt[q], ci[q].y);
+                    if constexpr (NMS) c = wq[q] * c;
+                    bel = bel + c;
+                }
+                fb[j] = bel;
+            }
+            if (fh) {
+                const uint64_t mask = __ballot(valid && bel < 0.0f);
+                const uint32_t word_bit = j0 + wave * 64;
+                if (lane == 0 && word_bit < cv.z) fh[word_bit >> 6] = mask;
+            }
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Syndrome and message extraction of one frame by one 64-lane workgroup.
+
+// Bits pos .. pos+63 (cyclically mod z) of a z-bit vector stored in zw
+// words whose bits at z and above are zero.  Needs z >= 64.
+__device__ __forceinline__ uint64_t cyc64(const uint64_t *v, uint32_t z, uint32_t zw, uint32_t pos) {
+    const uint32_t wi = pos >> 6, sh = pos & 63u;
+    uint64_t r = v[wi] >> sh;
+    if (sh && wi + 1 < zw) r |= v[wi + 1] << (64u - sh);
+    const uint32_t avail = z - pos;
+    if (avail < 64u) r |= v[0] << avail;
+    return r;
+}
+
+// Copies the frame's nb*zw hard-decision words hbf into sh (LDS) and
+// returns nonzero (uniform over the workgroup) when a check is unsatisfied.
+__device__ __forceinline__ int frame_unsat(const CodeView &cv, uint64_t *sh, const uint64_t *__restrict__ hbf) {
+    const uint32_t words = cv.nb * cv.zw;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t z = cv.z, zw = cv.zw;
+    for (uint32_t w = lane; w < words; w += blockDim.x) sh[w] = hbf[w];
+    __syncthreads();
+    uint64_t bad = 0;
+    if (z >= 64) {
+        for (uint32_t t = lane; t < cv.mb * zw; t += blockDim.x) {
+            const uint32_t br = t / zw, w = t - br * zw;
+            uint64_t acc = 0;
+            for (uint32_t e = cv.row_ptr[br]; e < cv.row_ptr[br + 1]; ++e) {
+                const uint2 re = cv.row_edge[e];
+                acc ^= cyc64(sh + re.x * zw, z, zw, wrap_add(64u * w, re.y, z));
+            }
+            if (w == zw - 1 && (z & 63u)) acc &= (1ull << (z & 63u)) - 1;
+            bad |= acc;
+        }
+    } else {
+        for (uint32_t c = lane; c < cv.m; c += blockDim.x) {
+            const uint32_t br = c / z, i = c - br * z;
+            uint32_t par = 0;
+            for (uint32_t e = cv.row_ptr[br]; e < cv.row_ptr[br + 1]; ++e) {
+                const uint2 re = cv.row_edge[e];
+                par ^= (uint32_t)(sh[re.x * zw] >> wrap_add(i, re.y, z)) & 1u;
+            }
+            bad |= par;
+        }
+    }
+    return __syncthreads_or(bad != 0);
+}
+
+// Writes the first k hard decisions (the message of a systematic code) from
+// sh as ceil(k/64) packed words to msgf.
+__device__ __forceinline__ void frame_message(const CodeView &cv, const uint64_t *sh, uint64_t *__restrict__ msgf) {
+    const uint32_t kw = (cv.k + 63) / 64;
+    const uint32_t z = cv.z, zw = cv.zw;
+    for (uint32_t w = threadIdx.x; w < kw; w += blockDim.x) {
+        const uint32_t v0 = 64u * w;
+        uint64_t r = 0;
+        if (z >= 64) {
+            const uint32_t bc = v0 / z, j = v0 - bc * z;
+            const uint64_t *blk = sh + bc * zw;
+            const uint32_t wi = j >> 6, s = j & 63u;
+            r = blk[wi] >> s;
+            if (s && wi + 1 < zw) r |= blk[wi + 1] << (64u - s);
+            const uint32_t avail = z - j;
+            if (avail < 64u && bc + 1 < cv.nb) r |= blk[zw] << avail;
+        } else {
+            for (uint32_t b = 0; b < 64 && v0 + b < cv.k; ++b) {
+                const uint32_t v = v0 + b, bc = v / z, j = v - bc * z;
+                r |= ((sh[bc * zw] >> j) & 1ull) << b;
+            }
+        }
+        if (v0 + 64u > cv.k) r &= (1ull << (cv.k - v0)) - 1;
+        msgf[w] = r;
+    }
+}
+
+}  // namespace
+}  // namespace gf2l

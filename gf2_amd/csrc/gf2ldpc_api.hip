@@ -422,6 +422,10 @@ int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value) {
         case GF2L_TUNE_XCD_MAP:
             ctx->tune.xcd_map = value != 0;
             return GF2L_OK;
+        case GF2L_TUNE_GRAPH_ITERS:
+            if (value < 1 || value > 64) return fail(ctx, GF2L_ERR_INVALID, "graph_iters out of [1,64]");
+            ctx->tune.graph_iters = value;
+            return GF2L_OK;
         default:
             return fail(ctx, GF2L_ERR_INVALID, "unknown tuning key");
     }

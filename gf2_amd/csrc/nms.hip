@@ -22,8 +22,8 @@
 #include <type_traits>
 #include <vector>
 
+#include "device_common.h"
 #include "gf2ldpc.h"
-#include "internal.h"
 
 struct gf2l_nms {
     const gf2l_code *code = nullptr;
@@ -48,39 +48,6 @@ struct gf2l_nms {
 
 namespace gf2l {
 namespace {
-
-__device__ __forceinline__ uint32_t wrap_add(uint32_t a, uint32_t b, uint32_t z) {
-    uint32_t s = a + b;
-    return s >= z ? s - z : s;
-}
-__device__ __forceinline__ uint32_t wrap_sub(uint32_t a, uint32_t b, uint32_t z) {
-    return a >= b ? a - b : a + z - b;
-}
-__device__ __forceinline__ float recon(float m1, float m2, uint32_t meta, uint32_t p) {
-    const float mag = (p == (meta & 31u)) ? m2 : m1;
-    return ((meta >> (5 + p)) & 1u) ? -mag : mag;
-}
-
-#define GF2L_DEG_CASE(K)                                              \
-    case K:                                                           \
-        if constexpr (K <= MAXD) f(std::integral_constant<int, K>{}); \
-        break;
-template <int MAXD, typename F>
-__device__ __forceinline__ void dispatch_degree(uint32_t d, F &&f) {
-    switch (d) {
-        GF2L_DEG_CASE(0) GF2L_DEG_CASE(1) GF2L_DEG_CASE(2) GF2L_DEG_CASE(3)
-        GF2L_DEG_CASE(4) GF2L_DEG_CASE(5) GF2L_DEG_CASE(6) GF2L_DEG_CASE(7)
-        GF2L_DEG_CASE(8) GF2L_DEG_CASE(9) GF2L_DEG_CASE(10) GF2L_DEG_CASE(11)
-        GF2L_DEG_CASE(12) GF2L_DEG_CASE(13) GF2L_DEG_CASE(14) GF2L_DEG_CASE(15)
-        GF2L_DEG_CASE(16) GF2L_DEG_CASE(17) GF2L_DEG_CASE(18) GF2L_DEG_CASE(19)
-        GF2L_DEG_CASE(20) GF2L_DEG_CASE(21) GF2L_DEG_CASE(22) GF2L_DEG_CASE(23)
-        GF2L_DEG_CASE(24) GF2L_DEG_CASE(25) GF2L_DEG_CASE(26) GF2L_DEG_CASE(27)
-        GF2L_DEG_CASE(28) GF2L_DEG_CASE(29) GF2L_DEG_CASE(30) GF2L_DEG_CASE(31)
-        GF2L_DEG_CASE(32)
-        default: break;
-    }
-}
-#undef GF2L_DEG_CASE
 
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll

@@ -110,6 +110,9 @@ class Context:
     def set_path(self, path: int) -> None:
         _lib.check(self._h, _lib.load().gf2l_set_path(self._h, path))
 
+    def set_tuning(self, key: int, value: int) -> None:
+        _lib.check(self._h, _lib.load().gf2l_set_tuning(self._h, key, int(value)))
+
     def synchronize(self) -> None:
         _lib.check(self._h, _lib.load().gf2l_synchronize(self._h))
 

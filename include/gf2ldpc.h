@@ -89,9 +89,13 @@ int gf2l_set_path(gf2l_ctx *ctx, int path);
  *   GF2L_TUNE_WG_PER_CU  persistent workgroups per compute unit (1..32, default 2)
  *   GF2L_TUNE_XCD_MAP    1 (default): each frame's block-rows / block-columns
  *                        run on one XCD so its beliefs and check states are
- *                        re-read from that XCD's L2; 0: frame-major over the grid */
+ *                        re-read from that XCD's L2; 0: frame-major over the grid
+ *   GF2L_TUNE_GRAPH_ITERS iterations captured per graph replay of the mixed-code
+ *                        decoder (1..64, default 4): the host checks for running
+ *                        frames once per replay */
 #define GF2L_TUNE_WG_PER_CU 1
 #define GF2L_TUNE_XCD_MAP 2
+#define GF2L_TUNE_GRAPH_ITERS 3
 int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value);
 
 /* ---- codes ------------------------------------------------------------ */
@@ -238,6 +242,31 @@ int gf2l_nms_train_step(gf2l_ctx *ctx, gf2l_nms *nms, const float *d_llr, const 
 int gf2l_nms_decode_device(gf2l_ctx *ctx, const gf2l_nms *nms, const float *d_llr, size_t frames,
                            uint32_t fallback_max_it, uint64_t *d_msg_words, uint32_t *d_iterations,
                            uint8_t *d_converged, uint8_t *d_syndrome_ok, uint8_t *d_fallback);
+
+/* ---- mixed-code batches ----------------------------------------------- */
+
+typedef struct gf2l_codeset gf2l_codeset;
+
+/* A set of 1..16 quasi-cyclic codes (each check degree <= 27; the codes must
+ * outlive the set) whose frames can be decoded together.  No reference
+ * counterpart: the reference decodes one code per decode_batch call
+ * (core.rs:729-756); a caller with frames of several codes calls it once per
+ * code.  The set owns a workspace and a captured iteration graph, so calls on
+ * one set must not overlap. */
+int gf2l_codeset_create(gf2l_ctx *ctx, const gf2l_code *const *codes, uint32_t count, gf2l_codeset **out);
+void gf2l_codeset_destroy(gf2l_codeset *set);
+/* Plain min-sum decode of `frames` frames, frame f of code code_ids[f]
+ * (host array, index into the set), each with exactly the results of
+ * gf2l_decode_batch_device on its own code: per-frame early stop, at most
+ * max_it iterations.  Device buffers, frames packed back to back in input
+ * order: d_llr holds n(code_ids[f]) floats per frame, d_msg_words
+ * ceil(k/64) words per frame; d_iterations, d_converged, d_syndrome_ok one
+ * entry per frame.  The iteration loop runs as replays of a hipGraph of
+ * GF2L_TUNE_GRAPH_ITERS iterations.  Errors: a code id outside the set
+ * returns GF2L_ERR_INVALID. */
+int gf2l_decode_mixed_device(gf2l_ctx *ctx, gf2l_codeset *set, const uint16_t *code_ids, size_t frames,
+                             const float *d_llr, uint32_t max_it, uint64_t *d_msg_words,
+                             uint32_t *d_iterations, uint8_t *d_converged, uint8_t *d_syndrome_ok);
 
 #ifdef __cplusplus
 }
