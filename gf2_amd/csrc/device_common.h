@@ -93,6 +93,37 @@ __device__ __forceinline__ uint32_t wrap_sub(uint32_t a, uint32_t b, uint32_t z)
     return a >= b ? a - b : a + z - b;
 }
 
+// Element idx of a wave-uniform base pointer, addressed as base + 32-bit byte
+// offset so the load / store takes the scalar-base form (one offset VGPR per
+// access instead of a 64-bit address pair).  Per-frame arrays stay below
+// 4 GiB (codes are limited to 2^28 checks and variables).
+// Wave-uniform value moved to a scalar register (graph tables are read by
+// every lane of a workgroup at the same address).
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Load through the constant address space: for a wave-uniform address this is
+// a scalar load (s_load) even when the pointer itself came from memory.  Only
+// for data no workgroup of the running kernel writes (code tables, weights,
+// the running list being read).
+template <typename T>
+__device__ __forceinline__ T ldc(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)p;
+}
+// Vector types are read component-wise (one s_load_dwordx2 after merging).
+__device__ __forceinline__ uint2 ldc(const uint2 *p) {
+    const auto *q = (const __attribute__((address_space(4))) uint32_t *)p;
+    return make_uint2(q[0], q[1]);
+}
+
+template <typename T>
+__device__ __forceinline__ T ldu(const T *base, uint32_t idx) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + idx * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void stu(T *base, uint32_t idx, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(T)) = v;
+}
+
 // Calls f(std::integral_constant<int, d>) for a wave-uniform degree d in
 // [0, MAXD], so per-edge loops are fully unrolled with no guards and every
 // load of an item is issued before the first wait.  Degree 0 (a variable or
@@ -131,29 +162,32 @@ template <int MAXD, bool FIRST, bool NMS>
 __device__ __forceinline__ void cn_state_item(const CodeView &cv, uint32_t br, const uint32_t *si,
                                               uint32_t *st, const float *__restrict__ fb,
                                               const float *__restrict__ w_prev) {
-    const size_t plane = (size_t)cv.mb * cv.zp;
+    const uint32_t plane = cv.mb * cv.zp;
     const uint32_t bstride = FIRST ? cv.z : cv.zp;
-    const uint32_t e0 = cv.row_ptr[br];
-    const uint32_t d = cv.row_ptr[br + 1] - e0;
+    const uint32_t e0 = uni(ldc(cv.row_ptr + br));
+    const uint32_t d = uni(ldc(cv.row_ptr + br + 1)) - e0;
     dispatch_degree<MAXD>(d, [&](auto DC) {
         constexpr int D = decltype(DC)::value, DA = D > 0 ? D : 1;
         uint2 re[DA];
         float wp[DA];
 #pragma unroll
         for (int p = 0; p < D; ++p) {
-            re[p] = cv.row_edge[e0 + p];
-            if constexpr (NMS && !FIRST) wp[p] = w_prev[e0 + p];
+            // uniform: scalar registers even where the compiler picks a
+            // vector load (the state stores of earlier items may alias)
+            const uint2 r = ldc(cv.row_edge + e0 + p);
+            re[p] = make_uint2(uni(r.x), uni(r.y));
+            if constexpr (NMS && !FIRST) wp[p] = __uint_as_float(uni(__float_as_uint(ldc(w_prev + e0 + p))));
         }
         for (uint32_t i = threadIdx.x; i < cv.z; i += blockDim.x) {
             float xv[DA];
 #pragma unroll
-            for (int p = 0; p < D; ++p) xv[p] = fb[re[p].x * bstride + wrap_add(i, re[p].y, cv.z)];
+            for (int p = 0; p < D; ++p) xv[p] = ldu(fb, re[p].x * bstride + wrap_add(i, re[p].y, cv.z));
             float m1o = 0.0f, m2o = 0.0f;
             uint32_t mo = 0;
             if constexpr (!FIRST) {
-                m1o = __uint_as_float(si[i]);
-                m2o = __uint_as_float(si[plane + i]);
-                mo = si[2 * plane + i];
+                m1o = __uint_as_float(ldu(si, i));
+                m2o = __uint_as_float(ldu(si, plane + i));
+                mo = ldu(si, 2 * plane + i);
             }
             CheckAcc acc;
             acc.init();
@@ -174,9 +208,9 @@ __device__ __forceinline__ void cn_state_item(const CodeView &cv, uint32_t br, c
                 m2 = 0.0f;
                 meta = 0;
             }
-            st[i] = __float_as_uint(m1);
-            st[plane + i] = __float_as_uint(m2);
-            st[2 * plane + i] = meta;
+            stu(st, i, __float_as_uint(m1));
+            stu(st, plane + i, __float_as_uint(m2));
+            stu(st, 2 * plane + i, meta);
         }
     });
 }
@@ -190,18 +224,21 @@ template <int MAXD, bool NMS>
 __device__ __forceinline__ void vn_state_item(const CodeView &cv, uint32_t bc, const uint32_t *__restrict__ fs,
                                               const float *__restrict__ fl, float *__restrict__ fb,
                                               uint64_t *__restrict__ fh, const float *__restrict__ w_cur) {
-    const size_t plane = (size_t)cv.mb * cv.zp;
+    const uint32_t plane = cv.mb * cv.zp;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t c0 = cv.col_ptr[bc];
-    const uint32_t dv = cv.col_ptr[bc + 1] - c0;
+    const uint32_t c0 = uni(ldc(cv.col_ptr + bc));
+    const uint32_t dv = uni(ldc(cv.col_ptr + bc + 1)) - c0;
     dispatch_degree<MAXD>(dv, [&](auto DC) {
         constexpr int D = decltype(DC)::value, DA = D > 0 ? D : 1;
         uint4 ci[DA];
         float wq[DA];
 #pragma unroll
         for (int q = 0; q < D; ++q) {
-            ci[q] = cv.col_info[c0 + q];
-            if constexpr (NMS) wq[q] = w_cur[ci[q].w];
+            // two 8-byte scalar loads: a 12-byte load (w unused) has no scalar form
+            const uint2 *cp = reinterpret_cast<const uint2 *>(cv.col_info + c0 + q);
+            const uint2 lo = ldc(cp), hi = ldc(cp + 1);
+            ci[q] = make_uint4(uni(lo.x), uni(lo.y), uni(hi.x), uni(hi.y));
+            if constexpr (NMS) wq[q] = __uint_as_float(uni(__float_as_uint(ldc(w_cur + ci[q].w))));
         }
         for (uint32_t j0 = 0; j0 < cv.z; j0 += blockDim.x) {
             const uint32_t j = j0 + threadIdx.x;
@@ -212,19 +249,19 @@ __device__ __forceinline__ void vn_state_item(const CodeView &cv, uint32_t bc, c
                 uint32_t mt[DA];
 #pragma unroll
                 for (int q = 0; q < D; ++q) {
-                    const uint32_t *sp = fs + (size_t)ci[q].x * cv.zp + wrap_sub(j, ci[q].z, cv.z);
-                    m1[q] = __uint_as_float(sp[0]);
-                    m2[q] = __uint_as_float(sp[plane]);
-                    mt[q] = sp[2 * plane];
+                    const uint32_t sp = ci[q].x * cv.zp + wrap_sub(j, ci[q].z, cv.z);
+                    m1[q] = __uint_as_float(ldu(fs, sp));
+                    m2[q] = __uint_as_float(ldu(fs, sp + plane));
+                    mt[q] = ldu(fs, sp + 2 * plane);
                 }
-                bel = fl[j];
+                bel = ldu(fl, j);
 #pragma unroll
                 for (int q = 0; q < D; ++q) {
                     float c = recon(m1[q], m2[q], mt[q], ci[q].y);
                     if constexpr (NMS) c = wq[q] * c;
                     bel = bel + c;
                 }
-                fb[j] = bel;
+                stu(fb, j, bel);
             }
             if (fh) {
                 const uint64_t mask = __ballot(valid && bel < 0.0f);

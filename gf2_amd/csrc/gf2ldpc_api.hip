@@ -439,8 +439,8 @@ int gf2l_code_qc(gf2l_ctx *ctx, uint32_t mb, uint32_t nb, uint32_t z, const int3
     if (mb == 0) return fail(ctx, GF2L_ERR_INVALID, "Base matrix must have at least one row");
     if (z == 0) return fail(ctx, GF2L_ERR_INVALID, "Expansion factor must be positive");
     if (nb == 0 || !shifts) return fail(ctx, GF2L_ERR_INVALID, "empty base matrix");
-    if ((uint64_t)mb * z > 0x7fffffffull || (uint64_t)nb * z > 0x7fffffffull)
-        return fail(ctx, GF2L_ERR_UNSUPPORTED, "code too large");
+    if ((uint64_t)mb * z > (1ull << 28) || (uint64_t)nb * z > (1ull << 28))
+        return fail(ctx, GF2L_ERR_UNSUPPORTED, "code too large (more than 2^28 checks or variables)");
     for (uint32_t br = 0; br < mb; ++br)
         for (uint32_t bc = 0; bc < nb; ++bc) {
             const int32_t s = shifts[(size_t)br * nb + bc];

@@ -421,6 +421,28 @@ uint32_t hbm_grid(uint32_t items, int num_cus, int wg_per_cu) {
     return g < 8u ? 8u : g;
 }
 
+// Degree classes of the STATE kernels: registers are allocated for the
+// largest degree a kernel dispatches, so a code runs the smallest class that
+// covers its check (variable) degrees.
+#define GF2L_STATE_CLASS(DEG, LAUNCH, A, B) \
+    do {                                     \
+        if ((DEG) <= 8)                      \
+            LAUNCH(8, A, B);                 \
+        else if ((DEG) <= 16)                \
+            LAUNCH(16, A, B);                \
+        else                                 \
+            LAUNCH(27, A, B);                \
+    } while (0)
+#define GF2L_VAR_CLASS(DEG, LAUNCH, A, B) \
+    do {                                   \
+        if ((DEG) <= 8)                    \
+            LAUNCH(8, A, B);               \
+        else if ((DEG) <= 16)              \
+            LAUNCH(16, A, B);              \
+        else                               \
+            LAUNCH(32, A, B);              \
+    } while (0)
+
 template <int DMAX, int DVMAX>
 hipError_t launch_lds_t(const CodeView &cv, const float *llr, uint32_t frames, uint32_t max_it,
                         const float *w, uint32_t fb_it, uint64_t *msg, uint32_t *iters,
@@ -494,21 +516,21 @@ hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, con
     hipLaunchKernelGGL((cn_state_kernel<MD, FI, NM>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
                        ws.state, ws.state, ws.belief, a.llr, a.w_prev, xcd)
         if (first) {
-            if (wide_c) GF2L_CN(27, true, false); else GF2L_CN(16, true, false);
+            GF2L_STATE_CLASS(cv.max_dc, GF2L_CN, true, false);
         } else if (nms_prev) {
-            if (wide_c) GF2L_CN(27, false, true); else GF2L_CN(16, false, true);
+            GF2L_STATE_CLASS(cv.max_dc, GF2L_CN, false, true);
         } else {
-            if (wide_c) GF2L_CN(27, false, false); else GF2L_CN(16, false, false);
+            GF2L_STATE_CLASS(cv.max_dc, GF2L_CN, false, false);
         }
 #undef GF2L_CN
         if (ev) hipEventRecord(ev[1], s);
-#define GF2L_VN(MD, NM)                                                                        \
+#define GF2L_VN(MD, NM, UNUSED)                                                                \
     hipLaunchKernelGGL((vn_state_kernel<MD, NM>), dim3(g_vn), dim3(block), 0, s, cv, list, count, \
                        ws.state, ws.belief, a.llr, ws.hb, next_count, a.w_cur, xcd)
         if (nms_cur) {
-            if (wide_v) GF2L_VN(32, true); else GF2L_VN(16, true);
+            GF2L_VAR_CLASS(cv.max_dv, GF2L_VN, true, 0);
         } else {
-            if (wide_v) GF2L_VN(32, false); else GF2L_VN(16, false);
+            GF2L_VAR_CLASS(cv.max_dv, GF2L_VN, false, 0);
         }
 #undef GF2L_VN
     } else {
@@ -547,26 +569,25 @@ hipError_t launch_state_forward(const CodeView &cv, const uint32_t *list, const 
     const uint32_t g_cn = hbm_grid(frames * cv.mb, num_cus, tune.wg_per_cu);
     const uint32_t g_vn = hbm_grid(frames * cv.nb, num_cus, tune.wg_per_cu);
     const int xcd = tune.xcd_map;
-    const bool wide_c = cv.max_dc > 16, wide_v = cv.max_dv > 16;
     const bool nms_prev = w_prev != nullptr, nms_cur = w_cur != nullptr;
 #define GF2L_CN(MD, FI, NM)                                                                    \
     hipLaunchKernelGGL((cn_state_kernel<MD, FI, NM>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
                        state_in, state_out, belief_in, llr, w_prev, xcd)
     if (first) {
-        if (wide_c) GF2L_CN(27, true, false); else GF2L_CN(16, true, false);
+        GF2L_STATE_CLASS(cv.max_dc, GF2L_CN, true, false);
     } else if (nms_prev) {
-        if (wide_c) GF2L_CN(27, false, true); else GF2L_CN(16, false, true);
+        GF2L_STATE_CLASS(cv.max_dc, GF2L_CN, false, true);
     } else {
-        if (wide_c) GF2L_CN(27, false, false); else GF2L_CN(16, false, false);
+        GF2L_STATE_CLASS(cv.max_dc, GF2L_CN, false, false);
     }
 #undef GF2L_CN
-#define GF2L_VN(MD, NM)                                                                        \
+#define GF2L_VN(MD, NM, UNUSED)                                                                \
     hipLaunchKernelGGL((vn_state_kernel<MD, NM>), dim3(g_vn), dim3(block), 0, s, cv, list, count, \
                        state_out, belief_out, llr, nullptr, nullptr, w_cur, xcd)
     if (nms_cur) {
-        if (wide_v) GF2L_VN(32, true); else GF2L_VN(16, true);
+        GF2L_VAR_CLASS(cv.max_dv, GF2L_VN, true, 0);
     } else {
-        if (wide_v) GF2L_VN(32, false); else GF2L_VN(16, false);
+        GF2L_VAR_CLASS(cv.max_dv, GF2L_VN, false, 0);
     }
 #undef GF2L_VN
     return hipGetLastError();

@@ -77,7 +77,7 @@ struct Profile {
 // Launch-shape tunables of the HBM paths (set by gf2l_set_tuning or the
 // autotuner).
 struct Tuning {
-    int wg_per_cu = 2;   // persistent workgroups per compute unit
+    int wg_per_cu = 3;   // persistent workgroups per compute unit
     int xcd_map = 1;     // 1: frames pinned to XCDs (L2 reuse), 0: frame-major over the grid
     int graph_iters = 4; // iterations per graph replay of the mixed-code decoder
 };

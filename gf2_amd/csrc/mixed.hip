@@ -3,21 +3,26 @@
 // plain min-sum flooding decoder and its own early stop (crates/gf2-coding/
 // src/ldpc/core.rs:889-938), bit-exact with decoding each code on its own.
 //
-// Each iteration is three launches over all running frames of all codes:
-// check update, variable update (STATE layout: compressed check state, padded
-// beliefs) and syndrome/retirement.  Every value that changes during a decode
-// lives on the device -- each frame's completed-iteration count, one running
-// list per code (double-buffered), the list parity -- and every value that
-// changes between calls is read from a device argument block, so U iterations
-// are captured once per code set as a hipGraph and replayed until no frame is
-// running; the host reads the running counts back once per replay.
+// Layout.  At the start of a call the frames of code c get the slots
+// list_off[c] .. list_off[c] + frames_c - 1 (input order within the code), and
+// their LLRs are gathered into a per-code workspace segment, so every per-frame
+// array is segment base + rank * stride:
+//   llr     [frames_c][n] floats
+//   state   [frames_c][3][mb][zp] u32 (min1, min2, argmin | output signs << 5)
+//   belief  [frames_c][nb][zp] floats
+//   hb      [frames_c][nb][zw] hard-decision words
+// Results go to the caller's packed arrays (frames back to back in input
+// order) through the slot table.
 //
-// Per frame f (element offsets in FrameInfo):
-//   llr     n floats of the caller's packed input (frame after frame)
-//   state   [3][mb][zp] u32 (min1, min2, argmin | output signs << 5)
-//   belief  [nb][zp] floats
-//   hb      [nb][zw] hard-decision words
-//   msg     ceil(k/64) words of the caller's packed output
+// Iterations.  All frames start together, so one device counter holds the
+// iteration number of every running frame.  An iteration is a check-update
+// and a variable-update launch per degree class present in the set
+// (registers sized per class) and one syndrome/retirement launch, each over
+// the running frames of every code; the running lists (one per code,
+// double-buffered) and their counts live on the device.  The first iteration
+// (inputs = channel LLRs) is launched directly; later iterations replay a
+// hipGraph of U iterations captured once per code set, and the host reads the
+// running counts back once per replay.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,18 +37,26 @@ namespace gf2l {
 
 constexpr uint32_t kMaxCodes = 16;
 
-struct FrameInfo {
-    uint32_t code, pad;
-    uint64_t llr, st, bel, hb, msg;
+// Workspace segment of one code for one call (element offsets).
+struct CodeSeg {
+    uint64_t llr, st, bel, hb;
+    uint32_t list_off, frames;
 };
 
 // Per-call arguments, uploaded before each decode (the graph reads them).
 struct CallArgs {
-    const float *llr;
+    const float *llr;  // caller's packed LLRs
     uint64_t *msg;
     uint32_t *iters;
     uint8_t *conv, *synd;
     uint32_t max_it, frames;
+    CodeSeg seg[kMaxCodes];
+};
+
+// Caller-side position of the frame in a slot.
+struct SlotInfo {
+    uint32_t frame, code;
+    uint64_t llr, msg;  // element offsets in the caller's packed arrays
 };
 
 // Device pointers of a code set's workspace; fixed until the workspace grows.
@@ -51,13 +64,12 @@ struct MixShared {
     const CodeView *codes;
     uint32_t ncodes;
     const CallArgs *args;
-    const FrameInfo *fi;
-    const uint32_t *list_off;  // [kMaxCodes] start of each code's list
-    uint32_t *lists;           // [2][list_cap]
+    const SlotInfo *slots;
+    uint32_t *lists;   // [2][list_cap] slots of running frames, code c from list_off[c]
     uint32_t list_cap;
-    uint32_t *counts;          // [2][kMaxCodes] running frames per code
-    uint32_t *fit;             // completed iterations per frame
-    uint32_t *misc;            // [0] list parity, [1] finished syndrome workgroups
+    uint32_t *counts;  // [2][kMaxCodes] running frames per code
+    uint32_t *misc;    // [0] list parity, [1] finished syndrome workgroups, [2] iterations done
+    float *llr;
     uint32_t *state;
     float *belief;
     uint64_t *hb;
@@ -69,160 +81,270 @@ struct gf2l_codeset {
     std::vector<const gf2l_code *> codes;
     gf2l::CodeView *d_views = nullptr;
     uint32_t max_dc = 0, max_dv = 0, max_z = 0, max_words = 0;
+    uint32_t cn_mask[3] = {0, 0, 0};  // codes per check-degree class (8, 16, 27)
+    uint32_t vn_mask[3] = {0, 0, 0};  // codes per variable-degree class (8, 16, 32)
     int device = 0;
     // workspace (one device allocation) and pinned upload / readback staging
     char *d_base = nullptr;
-    size_t cap_frames = 0, cap_state = 0, cap_bel = 0, cap_hb = 0;
+    size_t cap_frames = 0, cap_llr = 0, cap_state = 0, cap_bel = 0, cap_hb = 0;
     char *h_stage = nullptr;
     gf2l::MixShared m{};
     // captured iteration graph and the launch shape it was captured with
     hipGraphExec_t exec = nullptr;
     uint32_t g_grid = 0, g_unroll = 0;
+    int g_xcd = -1;
 };
 
 namespace gf2l {
 namespace {
 
-// Maps item j of a pass with `per` items per frame (mb for check items, nb
-// for variable items) to (code, frame, block index); false past the end.
-template <bool ROWS>
-__device__ __forceinline__ bool locate(const MixShared &m, const uint32_t *cnt, const uint32_t *list,
-                                       uint32_t j, uint32_t &c, uint32_t &f, uint32_t &r) {
-    for (c = 0; c < m.ncodes; ++c) {
-        const uint32_t per = ROWS ? m.codes[c].mb : m.codes[c].nb;
-        const uint32_t n = cnt[c] * per;
-        if (j < n) {
-            f = list[m.list_off[c] + j / per];
-            r = j % per;
-            return true;
+// The code table, the argument block and the running list a launch reads are
+// not written while it runs: they are read with scalar loads (ldc), so the
+// per-frame base pointers built from them stay in scalar registers and the
+// item bodies use scalar-base addressing as with a kernel-argument CodeView.
+__device__ __forceinline__ CodeView uniform_view(const CodeView *g) {
+    CodeView v;
+    v.mb = ldc(&g->mb);
+    v.nb = ldc(&g->nb);
+    v.z = ldc(&g->z);
+    v.n = ldc(&g->n);
+    v.m = ldc(&g->m);
+    v.k = ldc(&g->k);
+    v.nbe = ldc(&g->nbe);
+    v.max_dc = ldc(&g->max_dc);
+    v.max_dv = ldc(&g->max_dv);
+    v.zw = ldc(&g->zw);
+    v.zp = ldc(&g->zp);
+    v.row_ptr = ldc(&g->row_ptr);
+    v.row_edge = ldc(&g->row_edge);
+    v.col_ptr = ldc(&g->col_ptr);
+    v.col_edge = ldc(&g->col_edge);
+    v.col_info = ldc(&g->col_info);
+    return v;
+}
+__device__ __forceinline__ CodeSeg uniform_seg(const CodeSeg *g) {
+    CodeSeg s;
+    s.llr = ldc(&g->llr);
+    s.st = ldc(&g->st);
+    s.bel = ldc(&g->bel);
+    s.hb = ldc(&g->hb);
+    s.list_off = ldc(&g->list_off);
+    s.frames = ldc(&g->frames);
+    return s;
+}
+
+// Work split of one launch: with xcd != 0 the running frames of each code
+// are dealt round-robin to the 8 XCDs (list index x, x+8, ... to XCD
+// x = blockIdx % 8) and the workgroups of an XCD stride over their items in
+// frame order, as in the single-code kernels (needs gridDim.x % 8 == 0);
+// with xcd == 0 the whole grid strides over all items.
+struct MixSplit {
+    uint32_t x, q, Q, step;
+    __device__ explicit MixSplit(int xcd) {
+        if (xcd) {
+            x = blockIdx.x & 7u;
+            q = blockIdx.x >> 3;
+            Q = gridDim.x >> 3;
+            step = 8u;
+        } else {
+            x = 0;
+            q = blockIdx.x;
+            Q = gridDim.x;
+            step = 1u;
         }
-        j -= n;
     }
-    return false;
+    __device__ uint32_t share(uint32_t count) const {
+        return step == 1u ? count : (count > x ? (count - x + 7u) >> 3 : 0u);
+    }
+};
+
+enum ItemKind { kRows, kCols, kFrames };
+
+// First item >= start that the workgroup at grid position q (stride Q) owns.
+__device__ __forceinline__ uint32_t first_item(uint32_t start, uint32_t q, uint32_t Q) {
+    return start <= q ? q : q + (start - q + Q - 1) / Q * Q;
 }
 
-template <bool ROWS>
-__device__ __forceinline__ uint32_t total_items(const MixShared &m, const uint32_t *cnt) {
-    uint32_t t = 0;
-    for (uint32_t c = 0; c < m.ncodes; ++c) t += cnt[c] * (ROWS ? m.codes[c].mb : m.codes[c].nb);
-    return t;
+// Item prefix over the codes of one launch in LDS: s_start[c] = first item of
+// code c; returns the total.  Items per running frame: mb, nb or 1.
+template <ItemKind KIND>
+__device__ uint32_t build_prefix(const MixShared &m, uint32_t mask, const uint32_t *cnt, const MixSplit &sp,
+                                 uint32_t *s_start) {
+    if (threadIdx.x < kMaxCodes) {
+        const uint32_t c = threadIdx.x;
+        uint32_t n = 0;
+        if (c < m.ncodes && ((mask >> c) & 1u)) {
+            const uint32_t per = KIND == kRows ? m.codes[c].mb : KIND == kCols ? m.codes[c].nb : 1u;
+            n = sp.share(cnt[c]) * per;
+        }
+        s_start[c + 1] = n;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_start[0] = 0;
+        for (uint32_t c = 0; c < kMaxCodes; ++c) s_start[c + 1] += s_start[c];
+    }
+    __syncthreads();
+    return uni(s_start[kMaxCodes]);
+}
+
+// Check update of the running frames of the codes in `mask` (one degree
+// class: the kernel's registers are sized for MAXD).  FIRST: inputs are the
+// channel LLRs, no previous messages.
+template <int MAXD, bool FIRST>
+__global__ void __launch_bounds__(512) cn_mix_kernel(MixShared m, uint32_t mask, int xcd) {
+    __shared__ uint32_t s_start[kMaxCodes + 1];
+    const uint32_t cur = ldc(m.misc);
+    const uint32_t *cnt = m.counts + cur * kMaxCodes;
+    const uint32_t *list = m.lists + (size_t)cur * m.list_cap;
+    const MixSplit sp(xcd);
+    const uint32_t items = build_prefix<kRows>(m, mask, cnt, sp, s_start);
+    (void)items;
+    for (uint32_t c = 0; c < m.ncodes; ++c) {
+        const uint32_t start = uni(s_start[c]), end = uni(s_start[c + 1]);
+        if (start == end) continue;
+        const CodeView cv = uniform_view(m.codes + c);
+        const CodeSeg seg = uniform_seg(m.args->seg + c);
+        for (uint32_t j = first_item(start, sp.q, sp.Q); j < end; j += sp.Q) {
+            const uint32_t local = j - start;
+            const uint32_t br = local % cv.mb;
+            const uint32_t slot = ldc(list + seg.list_off + sp.x + sp.step * (local / cv.mb));
+            const uint64_t r = slot - seg.list_off;
+            uint32_t *st = m.state + seg.st + r * (3ull * cv.mb * cv.zp) + (size_t)br * cv.zp;
+            if constexpr (FIRST)
+                cn_state_item<MAXD, true, false>(cv, br, st, st, m.llr + seg.llr + r * cv.n, nullptr);
+            else
+                cn_state_item<MAXD, false, false>(cv, br, st, st,
+                                                  m.belief + seg.bel + r * ((uint64_t)cv.nb * cv.zp), nullptr);
+        }
+    }
 }
 
 template <int MAXD>
-__global__ void __launch_bounds__(512) cn_mix_kernel(MixShared m) {
-    const uint32_t cur = m.misc[0];
+__global__ void __launch_bounds__(512) vn_mix_kernel(MixShared m, uint32_t mask, int xcd) {
+    __shared__ uint32_t s_start[kMaxCodes + 1];
+    const uint32_t cur = ldc(m.misc);
     const uint32_t *cnt = m.counts + cur * kMaxCodes;
     const uint32_t *list = m.lists + (size_t)cur * m.list_cap;
-    const float *llr = m.args->llr;
-    const uint32_t items = total_items<true>(m, cnt);
-    for (uint32_t j = blockIdx.x; j < items; j += gridDim.x) {
-        uint32_t c, f, br;
-        if (!locate<true>(m, cnt, list, j, c, f, br)) break;
-        const CodeView &cv = m.codes[c];
-        const FrameInfo fi = m.fi[f];
-        uint32_t *st = m.state + fi.st + (size_t)br * cv.zp;
-        if (m.fit[f] == 0)
-            cn_state_item<MAXD, true, false>(cv, br, st, st, llr + fi.llr, nullptr);
-        else
-            cn_state_item<MAXD, false, false>(cv, br, st, st, m.belief + fi.bel, nullptr);
+    const MixSplit sp(xcd);
+    const uint32_t items = build_prefix<kCols>(m, mask, cnt, sp, s_start);
+    (void)items;
+    for (uint32_t c = 0; c < m.ncodes; ++c) {
+        const uint32_t start = uni(s_start[c]), end = uni(s_start[c + 1]);
+        if (start == end) continue;
+        const CodeView cv = uniform_view(m.codes + c);
+        const CodeSeg seg = uniform_seg(m.args->seg + c);
+        for (uint32_t j = first_item(start, sp.q, sp.Q); j < end; j += sp.Q) {
+            const uint32_t local = j - start;
+            const uint32_t bc = local % cv.nb;
+            const uint32_t slot = ldc(list + seg.list_off + sp.x + sp.step * (local / cv.nb));
+            const uint64_t r = slot - seg.list_off;
+            vn_state_item<MAXD, false>(cv, bc, m.state + seg.st + r * (3ull * cv.mb * cv.zp),
+                                       m.llr + seg.llr + r * cv.n + (size_t)bc * cv.z,
+                                       m.belief + seg.bel + r * ((uint64_t)cv.nb * cv.zp) + (size_t)bc * cv.zp,
+                                       m.hb + seg.hb + r * ((uint64_t)cv.nb * cv.zw) + (size_t)bc * cv.zw, nullptr);
+        }
     }
-}
-
-template <int MAXD>
-__global__ void __launch_bounds__(512) vn_mix_kernel(MixShared m) {
-    const uint32_t cur = m.misc[0];
-    const uint32_t *cnt = m.counts + cur * kMaxCodes;
-    const uint32_t *list = m.lists + (size_t)cur * m.list_cap;
-    const float *llr = m.args->llr;
-    // the syndrome pass of this iteration appends to the other list
+    // The syndrome pass of this iteration appends to the other list: every
+    // variable launch of the iteration clears its counts (the syndrome pass
+    // follows them all).  Stored last: a global store ahead of the loop would
+    // keep the table loads above from being scalar loads.
     if (blockIdx.x == 0 && threadIdx.x < kMaxCodes) m.counts[(cur ^ 1u) * kMaxCodes + threadIdx.x] = 0;
-    const uint32_t items = total_items<false>(m, cnt);
-    for (uint32_t j = blockIdx.x; j < items; j += gridDim.x) {
-        uint32_t c, f, bc;
-        if (!locate<false>(m, cnt, list, j, c, f, bc)) break;
-        const CodeView &cv = m.codes[c];
-        const FrameInfo fi = m.fi[f];
-        vn_state_item<MAXD, false>(cv, bc, m.state + fi.st, llr + fi.llr + (size_t)bc * cv.z,
-                                   m.belief + fi.bel + (size_t)bc * cv.zp, m.hb + fi.hb + (size_t)bc * cv.zw,
-                                   nullptr);
-    }
 }
 
 // One 64-lane workgroup per running frame: syndrome, then retire the frame
 // (converged, or at the iteration cap) or append it to its code's next list.
-// The last workgroup to finish flips the list parity for the next iteration.
+// The last workgroup to finish advances the iteration counter and flips the
+// list parity.
 __global__ void __launch_bounds__(64) syn_mix_kernel(MixShared m) {
     extern __shared__ uint64_t sh[];
-    const uint32_t cur = m.misc[0];
+    __shared__ uint32_t s_start[kMaxCodes + 1];
+    const uint32_t cur = ldc(m.misc);
+    const uint32_t it = ldc(m.misc + 2) + 1;
     const uint32_t *cnt = m.counts + cur * kMaxCodes;
     const uint32_t *list = m.lists + (size_t)cur * m.list_cap;
     uint32_t *next_cnt = m.counts + (cur ^ 1u) * kMaxCodes;
     uint32_t *next_list = m.lists + (size_t)(cur ^ 1u) * m.list_cap;
-    const CallArgs a = *m.args;
-    uint32_t total = 0;
-    for (uint32_t c = 0; c < m.ncodes; ++c) total += cnt[c];
-    for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
-        uint32_t c = 0, j = item;
-        while (j >= cnt[c]) j -= cnt[c++];
-        const uint32_t f = list[m.list_off[c] + j];
-        const CodeView &cv = m.codes[c];
-        const FrameInfo fi = m.fi[f];
-        const int unsat = frame_unsat(cv, sh, m.hb + fi.hb);
-        const uint32_t it = m.fit[f] + 1;
-        if (!unsat || it >= a.max_it) {
-            frame_message(cv, sh, a.msg + fi.msg);
-            if (threadIdx.x == 0) {
-                a.iters[f] = it;
-                a.conv[f] = (uint8_t)!unsat;
-                a.synd[f] = (uint8_t)!unsat;
+    const uint32_t max_it = ldc(&m.args->max_it);
+    const MixSplit sp(0);
+    const uint32_t total = build_prefix<kFrames>(m, 0xffffu, cnt, sp, s_start);
+    (void)total;
+    for (uint32_t c = 0; c < m.ncodes; ++c) {
+        const uint32_t start = uni(s_start[c]), end = uni(s_start[c + 1]);
+        if (start == end) continue;
+        const CodeView cv = uniform_view(m.codes + c);
+        const CodeSeg seg = uniform_seg(m.args->seg + c);
+        for (uint32_t item = first_item(start, blockIdx.x, gridDim.x); item < end; item += gridDim.x) {
+            const uint32_t slot = ldc(list + seg.list_off + item - start);
+            const uint64_t r = slot - seg.list_off;
+            const int unsat = frame_unsat(cv, sh, m.hb + seg.hb + r * ((uint64_t)cv.nb * cv.zw));
+            if (!unsat || it >= max_it) {
+                const SlotInfo si = m.slots[slot];
+                const CallArgs *a = m.args;
+                frame_message(cv, sh, a->msg + si.msg);
+                if (threadIdx.x == 0) {
+                    a->iters[si.frame] = it;
+                    a->conv[si.frame] = (uint8_t)!unsat;
+                    a->synd[si.frame] = (uint8_t)!unsat;
+                }
+            } else if (threadIdx.x == 0) {
+                next_list[seg.list_off + atomicAdd(next_cnt + c, 1u)] = slot;
             }
-        } else if (threadIdx.x == 0) {
-            m.fit[f] = it;
-            next_list[m.list_off[c] + atomicAdd(next_cnt + c, 1u)] = f;
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (threadIdx.x == 0) {
         __threadfence();
         if (atomicAdd(m.misc + 1, 1u) == gridDim.x - 1) {
             m.misc[1] = 0;
+            m.misc[2] = it;
             m.misc[0] = cur ^ 1u;
         }
     }
 }
 
-// Start of a call: every frame on its code's list, zero iterations done.
-__global__ void mix_init_kernel(MixShared m, const uint32_t *__restrict__ init_list,
-                                const uint32_t *__restrict__ init_counts) {
+// Start of a call: every slot on its code's list, lists and counters reset.
+__global__ void mix_init_kernel(MixShared m) {
     const uint32_t frames = m.args->frames;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < frames; t += gridDim.x * blockDim.x) {
-        m.lists[t] = init_list[t];
-        m.fit[t] = 0;
-    }
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < frames; t += gridDim.x * blockDim.x)
+        m.lists[t] = t;
     if (blockIdx.x == 0 && threadIdx.x < kMaxCodes) {
-        m.counts[threadIdx.x] = init_counts[threadIdx.x];
+        m.counts[threadIdx.x] = threadIdx.x < m.ncodes ? m.args->seg[threadIdx.x].frames : 0u;
         m.counts[kMaxCodes + threadIdx.x] = 0;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         m.misc[0] = 0;
         m.misc[1] = 0;
+        m.misc[2] = 0;
+    }
+}
+
+// Copies each slot's LLRs from the caller's packed array into its code's
+// workspace segment (one workgroup per slot, grid-stride).
+__global__ void __launch_bounds__(256) mix_gather_kernel(MixShared m) {
+    const uint32_t frames = m.args->frames;
+    const float *src = m.args->llr;
+    for (uint32_t slot = blockIdx.x; slot < frames; slot += gridDim.x) {
+        const SlotInfo si = m.slots[slot];
+        const uint32_t n = m.codes[si.code].n;
+        const CodeSeg &seg = m.args->seg[si.code];
+        const float *s = src + si.llr;
+        float *d = m.llr + seg.llr + (uint64_t)(slot - seg.list_off) * n;
+        for (uint32_t v = threadIdx.x; v < n; v += blockDim.x) d[v] = s[v];
     }
 }
 
 uint64_t align256(uint64_t x) { return (x + 255) & ~(uint64_t)255; }
 
 // Staging layout, the same on the device and in the pinned host mirror:
-//   CallArgs | list_off[16] | init_counts[16] | FrameInfo[cap] | init_list[cap]
+//   CallArgs | SlotInfo[cap]
 // followed on the host only by the readback area (counts[2][16], parity).
 struct Upload {
-    uint64_t args, off, icnt, fi, ilist, bytes, rb, host_bytes;
+    uint64_t args, slots, bytes, rb, host_bytes;
     explicit Upload(uint64_t cap) {
         args = 0;
-        off = align256(sizeof(CallArgs));
-        icnt = off + kMaxCodes * 4;
-        fi = align256(icnt + kMaxCodes * 4);
-        ilist = align256(fi + cap * sizeof(FrameInfo));
-        bytes = align256(ilist + cap * 4);
+        slots = align256(sizeof(CallArgs));
+        bytes = align256(slots + cap * sizeof(SlotInfo));
         rb = bytes;
         host_bytes = rb + 256;
     }
@@ -235,25 +357,26 @@ void free_work(gf2l_codeset *cs) {
     if (cs->h_stage) hipHostFree(cs->h_stage);
     cs->d_base = nullptr;
     cs->h_stage = nullptr;
-    cs->cap_frames = cs->cap_state = cs->cap_bel = cs->cap_hb = 0;
+    cs->cap_frames = cs->cap_llr = cs->cap_state = cs->cap_bel = cs->cap_hb = 0;
 }
 
 // Grows the workspace to hold `frames` frames with the given element totals
 // (a quarter of headroom, so batches of varying size rarely regrow it).
-int ensure_work(gf2l_ctx *ctx, gf2l_codeset *cs, uint64_t frames, uint64_t st, uint64_t bel, uint64_t hb) {
-    if (frames <= cs->cap_frames && st <= cs->cap_state && bel <= cs->cap_bel && hb <= cs->cap_hb)
+int ensure_work(gf2l_ctx *ctx, gf2l_codeset *cs, uint64_t frames, uint64_t llr, uint64_t st, uint64_t bel,
+                uint64_t hb) {
+    if (frames <= cs->cap_frames && llr <= cs->cap_llr && st <= cs->cap_state && bel <= cs->cap_bel &&
+        hb <= cs->cap_hb)
         return GF2L_OK;
     GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    const uint64_t cf = std::max<uint64_t>(frames + frames / 4, cs->cap_frames);
-    const uint64_t cst = std::max<uint64_t>(st + st / 4, cs->cap_state);
-    const uint64_t cbe = std::max<uint64_t>(bel + bel / 4, cs->cap_bel);
-    const uint64_t chb = std::max<uint64_t>(hb + hb / 4, cs->cap_hb);
+    auto grow = [](uint64_t need, uint64_t have) { return std::max<uint64_t>(need + need / 4, have); };
+    const uint64_t cf = grow(frames, cs->cap_frames), cl = grow(llr, cs->cap_llr), cst = grow(st, cs->cap_state),
+                   cbe = grow(bel, cs->cap_bel), chb = grow(hb, cs->cap_hb);
     free_work(cs);
     const Upload up(cf);
-    const uint64_t b_lists = align256(2 * cf * 4), b_cnt = align256(2 * kMaxCodes * 4),
-                   b_fit = align256(cf * 4), b_misc = 256, b_st = align256(cst * 4),
-                   b_bel = align256(cbe * 4), b_hb = align256(chb * 8);
-    const uint64_t total = up.bytes + b_lists + b_cnt + b_fit + b_misc + b_st + b_bel + b_hb;
+    const uint64_t b_lists = align256(2 * cf * 4), b_cnt = align256(2 * kMaxCodes * 4), b_misc = 256,
+                   b_llr = align256(cl * 4), b_st = align256(cst * 4), b_bel = align256(cbe * 4),
+                   b_hb = align256(chb * 8);
+    const uint64_t total = up.bytes + b_lists + b_cnt + b_misc + b_llr + b_st + b_bel + b_hb;
     GF2L_HIP(ctx, hipMalloc(&cs->d_base, total));
     const hipError_t e = hipHostMalloc(&cs->h_stage, up.host_bytes, hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -266,48 +389,65 @@ int ensure_work(gf2l_ctx *ctx, gf2l_codeset *cs, uint64_t frames, uint64_t st, u
     m.codes = cs->d_views;
     m.ncodes = (uint32_t)cs->codes.size();
     m.args = reinterpret_cast<const CallArgs *>(p + up.args);
-    m.list_off = reinterpret_cast<const uint32_t *>(p + up.off);
-    m.fi = reinterpret_cast<const FrameInfo *>(p + up.fi);
+    m.slots = reinterpret_cast<const SlotInfo *>(p + up.slots);
     p += up.bytes;
     m.lists = reinterpret_cast<uint32_t *>(p), p += b_lists;
     m.list_cap = (uint32_t)cf;
     m.counts = reinterpret_cast<uint32_t *>(p), p += b_cnt;
-    m.fit = reinterpret_cast<uint32_t *>(p), p += b_fit;
     m.misc = reinterpret_cast<uint32_t *>(p), p += b_misc;
+    m.llr = reinterpret_cast<float *>(p), p += b_llr;
     m.state = reinterpret_cast<uint32_t *>(p), p += b_st;
     m.belief = reinterpret_cast<float *>(p), p += b_bel;
     m.hb = reinterpret_cast<uint64_t *>(p);
     cs->cap_frames = cf;
+    cs->cap_llr = cl;
     cs->cap_state = cst;
     cs->cap_bel = cbe;
     cs->cap_hb = chb;
     return GF2L_OK;
 }
 
-// Captures `unroll` iterations (check, variable, syndrome launches) as a
-// graph over the code set's current workspace.
-int capture(gf2l_ctx *ctx, gf2l_codeset *cs, uint32_t grid, uint32_t unroll) {
-    if (cs->exec && cs->g_grid == grid && cs->g_unroll == unroll) return GF2L_OK;
+struct Shape {
+    uint32_t grid, block, g_syn;
+    size_t sh;
+    int xcd;
+};
+
+Shape launch_shape(const gf2l_ctx *ctx, const gf2l_codeset *cs) {
+    Shape s;
+    s.grid = ((uint32_t)ctx->num_cus * (uint32_t)ctx->tune.wg_per_cu + 7u) & ~7u;
+    s.block = std::min<uint32_t>(512, ((cs->max_z + 63) / 64) * 64);
+    s.g_syn = (uint32_t)ctx->num_cus * 8u;
+    s.sh = (size_t)cs->max_words * sizeof(uint64_t);
+    s.xcd = ctx->tune.xcd_map;
+    return s;
+}
+
+// Enqueues one iteration: check and variable launches per degree class, then
+// the syndrome launch.
+template <bool FIRST>
+void enqueue_iteration(const gf2l_codeset *cs, const Shape &sh, hipStream_t s) {
+    const MixShared m = cs->m;
+    const dim3 g(sh.grid), b(sh.block);
+    if (cs->cn_mask[0]) hipLaunchKernelGGL((cn_mix_kernel<8, FIRST>), g, b, 0, s, m, cs->cn_mask[0], sh.xcd);
+    if (cs->cn_mask[1]) hipLaunchKernelGGL((cn_mix_kernel<16, FIRST>), g, b, 0, s, m, cs->cn_mask[1], sh.xcd);
+    if (cs->cn_mask[2]) hipLaunchKernelGGL((cn_mix_kernel<27, FIRST>), g, b, 0, s, m, cs->cn_mask[2], sh.xcd);
+    if (cs->vn_mask[0]) hipLaunchKernelGGL(vn_mix_kernel<8>, g, b, 0, s, m, cs->vn_mask[0], sh.xcd);
+    if (cs->vn_mask[1]) hipLaunchKernelGGL(vn_mix_kernel<16>, g, b, 0, s, m, cs->vn_mask[1], sh.xcd);
+    if (cs->vn_mask[2]) hipLaunchKernelGGL(vn_mix_kernel<32>, g, b, 0, s, m, cs->vn_mask[2], sh.xcd);
+    hipLaunchKernelGGL(syn_mix_kernel, dim3(sh.g_syn), dim3(64), sh.sh, s, m);
+}
+
+// Captures `unroll` non-first iterations as a graph over the code set's
+// current workspace.
+int capture(gf2l_ctx *ctx, gf2l_codeset *cs, const Shape &sh, uint32_t unroll) {
+    if (cs->exec && cs->g_grid == sh.grid && cs->g_unroll == unroll && cs->g_xcd == sh.xcd) return GF2L_OK;
     if (cs->exec) hipGraphExecDestroy(cs->exec);
     cs->exec = nullptr;
-    const uint32_t block = std::min<uint32_t>(512, ((cs->max_z + 63) / 64) * 64);
-    const uint32_t g_syn = (uint32_t)ctx->num_cus * 8u;
-    const size_t sh = (size_t)cs->max_words * sizeof(uint64_t);
-    const MixShared m = cs->m;
     hipStream_t s = ctx->stream;
     hipGraph_t graph = nullptr;
     GF2L_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (uint32_t u = 0; u < unroll; ++u) {
-        if (cs->max_dc > 16)
-            hipLaunchKernelGGL(cn_mix_kernel<27>, dim3(grid), dim3(block), 0, s, m);
-        else
-            hipLaunchKernelGGL(cn_mix_kernel<16>, dim3(grid), dim3(block), 0, s, m);
-        if (cs->max_dv > 16)
-            hipLaunchKernelGGL(vn_mix_kernel<32>, dim3(grid), dim3(block), 0, s, m);
-        else
-            hipLaunchKernelGGL(vn_mix_kernel<16>, dim3(grid), dim3(block), 0, s, m);
-        hipLaunchKernelGGL(syn_mix_kernel, dim3(g_syn), dim3(64), sh, s, m);
-    }
+    for (uint32_t u = 0; u < unroll; ++u) enqueue_iteration<false>(cs, sh, s);
     const hipError_t launch = hipGetLastError();
     const hipError_t end = hipStreamEndCapture(s, &graph);
     if (launch != hipSuccess || end != hipSuccess) {
@@ -321,8 +461,24 @@ int capture(gf2l_ctx *ctx, gf2l_codeset *cs, uint32_t grid, uint32_t unroll) {
         cs->exec = nullptr;
         return fail(ctx, GF2L_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(inst));
     }
-    cs->g_grid = grid;
+    cs->g_grid = sh.grid;
     cs->g_unroll = unroll;
+    cs->g_xcd = sh.xcd;
+    return GF2L_OK;
+}
+
+// Frames still running after the work enqueued so far (synchronizes).
+int running_frames(gf2l_ctx *ctx, gf2l_codeset *cs, uint32_t *out) {
+    const Upload up(cs->cap_frames);
+    uint32_t *h_rb = reinterpret_cast<uint32_t *>(cs->h_stage + up.rb);
+    hipStream_t s = ctx->stream;
+    GF2L_HIP(ctx, hipMemcpyAsync(h_rb, cs->m.counts, 2 * kMaxCodes * 4, hipMemcpyDeviceToHost, s));
+    GF2L_HIP(ctx, hipMemcpyAsync(h_rb + 2 * kMaxCodes, cs->m.misc, 4, hipMemcpyDeviceToHost, s));
+    GF2L_HIP(ctx, hipStreamSynchronize(s));
+    const uint32_t *cur = h_rb + (h_rb[2 * kMaxCodes] & 1u) * kMaxCodes;
+    uint32_t running = 0;
+    for (uint32_t c = 0; c < kMaxCodes; ++c) running += cur[c];
+    *out = running;
     return GF2L_OK;
 }
 
@@ -362,6 +518,11 @@ int gf2l_codeset_create(gf2l_ctx *ctx, const gf2l_code *const *codes, uint32_t c
     cs->max_dv = max_dv;
     cs->max_z = max_z;
     cs->max_words = max_words;
+    for (uint32_t c = 0; c < count; ++c) {
+        const uint32_t dc = codes[c]->view.max_dc, dv = codes[c]->view.max_dv;
+        cs->cn_mask[dc <= 8 ? 0 : dc <= 16 ? 1 : 2] |= 1u << c;
+        cs->vn_mask[dv <= 8 ? 0 : dv <= 16 ? 1 : 2] |= 1u << c;
+    }
     cs->device = ctx->device;
     hipError_t e = hipMalloc(&cs->d_views, count * sizeof(gf2l::CodeView));
     if (e == hipSuccess)
@@ -400,13 +561,14 @@ int gf2l_decode_mixed_device(gf2l_ctx *ctx, gf2l_codeset *cs, const uint16_t *co
     hipStream_t s = ctx->stream;
     // per-code frame counts and workspace totals
     uint32_t per_code[kMaxCodes] = {0};
-    uint64_t t_st = 0, t_bel = 0, t_hb = 0, t_msg = 0, items_c = 0, items_v = 0;
+    uint64_t t_llr = 0, t_st = 0, t_bel = 0, t_hb = 0, t_msg = 0, items_c = 0, items_v = 0;
     for (uint32_t f = 0; f < F; ++f) {
         const uint32_t c = code_ids[f];
         if (c >= K)
             return fail(ctx, GF2L_ERR_INVALID, "code id " + std::to_string(c) + " of frame " + std::to_string(f) +
                                                    " outside the code set");
         const CodeView &v = cs->codes[c]->view;
+        t_llr += v.n;
         t_st += 3ull * v.mb * v.zp;
         t_bel += (uint64_t)v.nb * v.zp;
         t_hb += (uint64_t)v.nb * v.zw;
@@ -424,57 +586,62 @@ int gf2l_decode_mixed_device(gf2l_ctx *ctx, gf2l_codeset *cs, const uint16_t *co
         GF2L_HIP(ctx, hipMemsetAsync(d_syndrome_ok, 1, F, s));
         return GF2L_OK;
     }
-    int rc = ensure_work(ctx, cs, F, t_st, t_bel, t_hb);
+    int rc = ensure_work(ctx, cs, F, t_llr, t_st, t_bel, t_hb);
     if (rc != GF2L_OK) return rc;
     // fill the pinned staging block (the previous call has synchronized)
     const Upload up(cs->cap_frames);
     char *h = cs->h_stage;
-    *reinterpret_cast<CallArgs *>(h + up.args) =
-        CallArgs{d_llr, d_msg_words, d_iterations, d_converged, d_syndrome_ok, max_it, F};
-    uint32_t *h_off = reinterpret_cast<uint32_t *>(h + up.off);
-    uint32_t *h_icnt = reinterpret_cast<uint32_t *>(h + up.icnt);
-    FrameInfo *h_fi = reinterpret_cast<FrameInfo *>(h + up.fi);
-    uint32_t *h_il = reinterpret_cast<uint32_t *>(h + up.ilist);
-    uint32_t fill[kMaxCodes];
-    for (uint32_t c = 0; c < kMaxCodes; ++c) {
-        h_off[c] = c == 0 ? 0 : h_off[c - 1] + per_code[c - 1];
-        h_icnt[c] = per_code[c];
+    CallArgs *ha = reinterpret_cast<CallArgs *>(h + up.args);
+    *ha = CallArgs{};
+    ha->llr = d_llr;
+    ha->msg = d_msg_words;
+    ha->iters = d_iterations;
+    ha->conv = d_converged;
+    ha->synd = d_syndrome_ok;
+    ha->max_it = max_it;
+    ha->frames = F;
+    uint64_t o_llr = 0, o_st = 0, o_bel = 0, o_hb = 0;
+    uint32_t o_slot = 0, fill[kMaxCodes];
+    for (uint32_t c = 0; c < K; ++c) {
+        const CodeView &v = cs->codes[c]->view;
+        ha->seg[c] = CodeSeg{o_llr, o_st, o_bel, o_hb, o_slot, per_code[c]};
+        o_llr += (uint64_t)per_code[c] * v.n;
+        o_st += (uint64_t)per_code[c] * 3ull * v.mb * v.zp;
+        o_bel += (uint64_t)per_code[c] * v.nb * v.zp;
+        o_hb += (uint64_t)per_code[c] * v.nb * v.zw;
+        o_slot += per_code[c];
         fill[c] = 0;
     }
-    uint64_t o_llr = 0, o_st = 0, o_bel = 0, o_hb = 0, o_msg = 0;
+    SlotInfo *h_slots = reinterpret_cast<SlotInfo *>(h + up.slots);
+    uint64_t f_llr = 0, f_msg = 0;
     for (uint32_t f = 0; f < F; ++f) {
         const uint32_t c = code_ids[f];
         const CodeView &v = cs->codes[c]->view;
-        h_fi[f] = FrameInfo{c, 0, o_llr, o_st, o_bel, o_hb, o_msg};
-        h_il[h_off[c] + fill[c]++] = f;
-        o_llr += v.n;
-        o_st += 3ull * v.mb * v.zp;
-        o_bel += (uint64_t)v.nb * v.zp;
-        o_hb += (uint64_t)v.nb * v.zw;
-        o_msg += (v.k + 63) / 64;
+        h_slots[ha->seg[c].list_off + fill[c]++] = SlotInfo{f, c, f_llr, f_msg};
+        f_llr += v.n;
+        f_msg += (v.k + 63) / 64;
     }
-    char *d_up = cs->d_base;
-    GF2L_HIP(ctx, hipMemcpyAsync(d_up, h, up.fi + (size_t)F * sizeof(FrameInfo), hipMemcpyHostToDevice, s));
-    GF2L_HIP(ctx, hipMemcpyAsync(d_up + up.ilist, h + up.ilist, (size_t)F * 4, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(mix_init_kernel, dim3(std::min<uint32_t>((F + 255) / 256, 1024)), dim3(256), 0, s, cs->m,
-                       reinterpret_cast<const uint32_t *>(d_up + up.ilist),
-                       reinterpret_cast<const uint32_t *>(d_up + up.icnt));
+    GF2L_HIP(ctx, hipMemcpyAsync(cs->d_base, h, up.slots + (size_t)F * sizeof(SlotInfo), hipMemcpyHostToDevice, s));
+    const uint32_t g_init = std::min<uint32_t>((F + 255) / 256, 1024);
+    hipLaunchKernelGGL(mix_init_kernel, dim3(g_init), dim3(256), 0, s, cs->m);
+    hipLaunchKernelGGL(mix_gather_kernel, dim3(std::min<uint32_t>(F, 4096)), dim3(256), 0, s, cs->m);
+    const Shape sh = launch_shape(ctx, cs);
+    enqueue_iteration<true>(cs, sh, s);
     GF2L_HIP(ctx, hipGetLastError());
-    const uint32_t grid = ((uint32_t)ctx->num_cus * (uint32_t)ctx->tune.wg_per_cu + 7u) & ~7u;
-    rc = capture(ctx, cs, grid, (uint32_t)ctx->tune.graph_iters);
-    if (rc != GF2L_OK) return rc;
-    // replay until every frame has retired (at most ceil(max_it / U) replays)
-    uint32_t *h_rb = reinterpret_cast<uint32_t *>(h + up.rb);
-    for (uint32_t done_it = 0; done_it < max_it; done_it += cs->g_unroll) {
-        GF2L_HIP(ctx, hipGraphLaunch(cs->exec, s));
-        GF2L_HIP(ctx, hipMemcpyAsync(h_rb, cs->m.counts, 2 * kMaxCodes * 4, hipMemcpyDeviceToHost, s));
-        GF2L_HIP(ctx, hipMemcpyAsync(h_rb + 2 * kMaxCodes, cs->m.misc, 4, hipMemcpyDeviceToHost, s));
-        GF2L_HIP(ctx, hipStreamSynchronize(s));
-        const uint32_t *cur = h_rb + (h_rb[2 * kMaxCodes] & 1u) * kMaxCodes;
-        uint32_t running = 0;
-        for (uint32_t c = 0; c < K; ++c) running += cur[c];
-        if (running == 0) break;
+    uint32_t running = 0;
+    if (max_it > 1) {
+        rc = capture(ctx, cs, sh, (uint32_t)ctx->tune.graph_iters);
+        if (rc != GF2L_OK) return rc;
+        rc = running_frames(ctx, cs, &running);
+        if (rc != GF2L_OK) return rc;
     }
+    // replay until every frame has retired (at most ceil((max_it - 1) / U) replays)
+    for (uint32_t done_it = 1; running && done_it < max_it; done_it += cs->g_unroll) {
+        GF2L_HIP(ctx, hipGraphLaunch(cs->exec, s));
+        rc = running_frames(ctx, cs, &running);
+        if (rc != GF2L_OK) return rc;
+    }
+    GF2L_HIP(ctx, hipStreamSynchronize(s));
     return GF2L_OK;
 }
 
