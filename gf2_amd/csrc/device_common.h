@@ -149,20 +149,22 @@ __device__ __forceinline__ void dispatch_degree(uint32_t d, F &&f) {
 #undef GF2L_DEG_CASE
 
 // ---------------------------------------------------------------------------
-// STATE-path item bodies.  A frame's check state is [3][mb][zp] u32
-// (min1, min2, argmin | output sign bits << 5), its beliefs [nb][zp] floats.
+// STATE-path item bodies.  A frame's check state is [mb][zp] records of three
+// u32 (min1, min2, argmin | output sign bits << 5), read and written as one
+// 12-byte access; its beliefs are [nb][zp] floats.
 
 // Check update of block-row br of one frame: threads map to the z lifted
-// checks; belief reads of a circulant are a rotated contiguous run.  si / st
-// point at the block-row's first min1 word (plane stride mb*zp).  In the
-// first iteration the inputs are the channel LLRs fb (stride z) with
+// checks; belief reads of a circulant are a rotated contiguous run.  fsi /
+// fso: the frame's state (read / written; the same array in the decoder).  In
+// the first iteration the inputs are the channel LLRs fb (stride z) with
 // c2v = 0; later fb is the frame's belief array (stride zp) and the previous
-// message is rebuilt from si (times its weight w_prev[e] for NMS).
+// message is rebuilt from fsi (times its weight w_prev[e] for NMS).
 template <int MAXD, bool FIRST, bool NMS>
-__device__ __forceinline__ void cn_state_item(const CodeView &cv, uint32_t br, const uint32_t *si,
-                                              uint32_t *st, const float *__restrict__ fb,
+__device__ __forceinline__ void cn_state_item(const CodeView &cv, uint32_t br, const uint32_t *fsi,
+                                              uint32_t *fso, const float *__restrict__ fb,
                                               const float *__restrict__ w_prev) {
-    const uint32_t plane = cv.mb * cv.zp;
+    const uint3 *si = reinterpret_cast<const uint3 *>(fsi) + br * cv.zp;
+    uint3 *st = reinterpret_cast<uint3 *>(fso) + br * cv.zp;
     const uint32_t bstride = FIRST ? cv.z : cv.zp;
     const uint32_t e0 = uni(ldc(cv.row_ptr + br));
     const uint32_t d = uni(ldc(cv.row_ptr + br + 1)) - e0;
@@ -185,9 +187,10 @@ __device__ __forceinline__ void cn_state_item(const CodeView &cv, uint32_t br, c
             float m1o = 0.0f, m2o = 0.0f;
             uint32_t mo = 0;
             if constexpr (!FIRST) {
-                m1o = __uint_as_float(ldu(si, i));
-                m2o = __uint_as_float(ldu(si, plane + i));
-                mo = ldu(si, 2 * plane + i);
+                const uint3 o = ldu(si, i);
+                m1o = __uint_as_float(o.x);
+                m2o = __uint_as_float(o.y);
+                mo = o.z;
             }
             CheckAcc acc;
             acc.init();
@@ -208,9 +211,7 @@ __device__ __forceinline__ void cn_state_item(const CodeView &cv, uint32_t br, c
                 m2 = 0.0f;
                 meta = 0;
             }
-            stu(st, i, __float_as_uint(m1));
-            stu(st, plane + i, __float_as_uint(m2));
-            stu(st, 2 * plane + i, meta);
+            stu(st, i, make_uint3(__float_as_uint(m1), __float_as_uint(m2), meta));
         }
     });
 }
@@ -224,7 +225,7 @@ template <int MAXD, bool NMS>
 __device__ __forceinline__ void vn_state_item(const CodeView &cv, uint32_t bc, const uint32_t *__restrict__ fs,
                                               const float *__restrict__ fl, float *__restrict__ fb,
                                               uint64_t *__restrict__ fh, const float *__restrict__ w_cur) {
-    const uint32_t plane = cv.mb * cv.zp;
+    const uint3 *fst = reinterpret_cast<const uint3 *>(fs);
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t c0 = uni(ldc(cv.col_ptr + bc));
     const uint32_t dv = uni(ldc(cv.col_ptr + bc + 1)) - c0;
@@ -249,10 +250,10 @@ __device__ __forceinline__ void vn_state_item(const CodeView &cv, uint32_t bc, c
                 uint32_t mt[DA];
 #pragma unroll
                 for (int q = 0; q < D; ++q) {
-                    const uint32_t sp = ci[q].x * cv.zp + wrap_sub(j, ci[q].z, cv.z);
-                    m1[q] = __uint_as_float(ldu(fs, sp));
-                    m2[q] = __uint_as_float(ldu(fs, sp + plane));
-                    mt[q] = ldu(fs, sp + 2 * plane);
+                    const uint3 o = ldu(fst, ci[q].x * cv.zp + wrap_sub(j, ci[q].z, cv.z));
+                    m1[q] = __uint_as_float(o.x);
+                    m2[q] = __uint_as_float(o.y);
+                    mt[q] = o.z;
                 }
                 bel = ldu(fl, j);
 #pragma unroll

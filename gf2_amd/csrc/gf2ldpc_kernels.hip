@@ -67,9 +67,8 @@ cn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *
     for (uint32_t j = xs.q; j < xs.items; j += xs.Q) {
         const uint32_t f = list[xs.list_index(j, cv.mb)];
         const uint32_t br = j % cv.mb;
-        const size_t off = f * fstride + (size_t)br * cv.zp;
         const float *fb = FIRST ? llr + (size_t)f * cv.n : belief + (size_t)f * cv.nb * cv.zp;
-        cn_state_item<MAXD, FIRST, NMS>(cv, br, state_in + off, state_out + off, fb, w_prev);
+        cn_state_item<MAXD, FIRST, NMS>(cv, br, state_in + f * fstride, state_out + f * fstride, fb, w_prev);
     }
 }
 
@@ -230,8 +229,7 @@ __global__ void state_expand_kernel(CodeView cv, const uint32_t *__restrict__ st
                                     const float *__restrict__ belief, uint32_t frames,
                                     const float *__restrict__ w, float *__restrict__ c2v_out,
                                     float *__restrict__ bel_out) {
-    const size_t E = (size_t)cv.nbe * cv.z, fstride = 3ull * cv.mb * cv.zp,
-                 plane = (size_t)cv.mb * cv.zp;
+    const size_t E = (size_t)cv.nbe * cv.z, fstride = 3ull * cv.mb * cv.zp;
     const size_t total = (size_t)frames * E;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
          t += (size_t)gridDim.x * blockDim.x) {
@@ -239,9 +237,8 @@ __global__ void state_expand_kernel(CodeView cv, const uint32_t *__restrict__ st
         const uint32_t r = (uint32_t)(t - (size_t)f * E), e = r / cv.z, i = r - e * cv.z;
         uint32_t br = 0;
         while (cv.row_ptr[br + 1] <= e) ++br;
-        const uint32_t *s = state + f * fstride + (size_t)br * cv.zp + i;
-        float v = recon(__uint_as_float(s[0]), __uint_as_float(s[plane]), s[2 * plane],
-                        e - cv.row_ptr[br]);
+        const uint32_t *s = state + f * fstride + 3 * ((size_t)br * cv.zp + i);
+        float v = recon(__uint_as_float(s[0]), __uint_as_float(s[1]), s[2], e - cv.row_ptr[br]);
         if (w) v = w[e] * v;
         c2v_out[t] = v;
     }

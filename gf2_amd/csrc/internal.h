@@ -56,7 +56,7 @@ struct Workspace {
     size_t cap_frames = 0;
     size_t cap_c2v = 0, cap_state = 0, cap_bel = 0, cap_hb = 0;
     float *c2v = nullptr;        // explicit path: [frames][nbe*z]
-    uint32_t *state = nullptr;   // state path: [frames][3][mb][zp] min1 | min2 | meta
+    uint32_t *state = nullptr;   // state path: [frames][mb][zp][3] min1, min2, meta
     float *belief = nullptr;     // explicit: [frames][n]; state: [frames][nb][zp]
     uint64_t *hb = nullptr;      // [frames][nb][zw] hard decisions
     uint32_t *lists = nullptr;   // [2][frames] running frame lists

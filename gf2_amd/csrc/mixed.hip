@@ -8,7 +8,7 @@
 // their LLRs are gathered into a per-code workspace segment, so every per-frame
 // array is segment base + rank * stride:
 //   llr     [frames_c][n] floats
-//   state   [frames_c][3][mb][zp] u32 (min1, min2, argmin | output signs << 5)
+//   state   [frames_c][mb][zp][3] u32 (min1, min2, argmin | output signs << 5)
 //   belief  [frames_c][nb][zp] floats
 //   hb      [frames_c][nb][zw] hard-decision words
 // Results go to the caller's packed arrays (frames back to back in input
@@ -210,11 +210,11 @@ __global__ void __launch_bounds__(512) cn_mix_kernel(MixShared m, uint32_t mask,
             const uint32_t br = local % cv.mb;
             const uint32_t slot = ldc(list + seg.list_off + sp.x + sp.step * (local / cv.mb));
             const uint64_t r = slot - seg.list_off;
-            uint32_t *st = m.state + seg.st + r * (3ull * cv.mb * cv.zp) + (size_t)br * cv.zp;
+            uint32_t *fs = m.state + seg.st + r * (3ull * cv.mb * cv.zp);
             if constexpr (FIRST)
-                cn_state_item<MAXD, true, false>(cv, br, st, st, m.llr + seg.llr + r * cv.n, nullptr);
+                cn_state_item<MAXD, true, false>(cv, br, fs, fs, m.llr + seg.llr + r * cv.n, nullptr);
             else
-                cn_state_item<MAXD, false, false>(cv, br, st, st,
+                cn_state_item<MAXD, false, false>(cv, br, fs, fs,
                                                   m.belief + seg.bel + r * ((uint64_t)cv.nb * cv.zp), nullptr);
         }
     }

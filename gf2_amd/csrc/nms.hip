@@ -35,7 +35,7 @@ struct gf2l_nms {
     uint64_t step = 0;
     // training workspace, grown on demand
     size_t cap = 0;
-    uint32_t *tape_state = nullptr;  // [T][F][3][mb][zp]
+    uint32_t *tape_state = nullptr;  // [T][F][mb][zp][3]
     float *tape_bel = nullptr;       // [T][F][nb][zp]
     float *g_a = nullptr, *g_b = nullptr;  // [F][nbe][z] dL/dv2c
     float *g_bel = nullptr;          // [F][nb][zp]
@@ -126,7 +126,7 @@ cn_bwd_kernel(CodeView cv, uint32_t frames, uint32_t t, const float *__restrict_
               float *__restrict__ g_prev, float *__restrict__ gpart) {
     __shared__ float red[MAXD > 0 ? MAXD : 1][8];
     const uint32_t items = frames * cv.mb;
-    const size_t fstride = 3ull * cv.mb * cv.zp, plane = (size_t)cv.mb * cv.zp;
+    const size_t fstride = 3ull * cv.mb * cv.zp;
     const size_t E = (size_t)cv.nbe * cv.z;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
     for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
@@ -143,15 +143,15 @@ cn_bwd_kernel(CodeView cv, uint32_t frames, uint32_t t, const float *__restrict_
                 wt[p] = w_t[e0 + p];
                 gw[p] = 0.0f;
             }
-            const uint32_t *si = st_prev + f * fstride + (size_t)br * cv.zp;
+            const uint32_t *si = st_prev + f * fstride + 3ull * br * cv.zp;
             for (uint32_t i = threadIdx.x; i < cv.z; i += blockDim.x) {
                 float x[DA], gb[DA], gc[DA];
                 float m1o = 0.0f, m2o = 0.0f;
                 uint32_t mo = 0;
                 if (t >= 2) {
-                    m1o = __uint_as_float(si[i]);
-                    m2o = __uint_as_float(si[plane + i]);
-                    mo = si[2 * plane + i];
+                    m1o = __uint_as_float(si[3 * i]);
+                    m2o = __uint_as_float(si[3 * i + 1]);
+                    mo = si[3 * i + 2];
                 }
 #pragma unroll
                 for (int p = 0; p < D; ++p) {
