@@ -1,4 +1,4 @@
-"""Frame-error-rate sweep harness around the GPU decoder.
+"""Frame-error-rate harness around the GPU decoders.
 
 Mirrors the reference's FER loop (crates/gf2-coding/examples/ldpc_awgn.rs:121-175):
 a frame is in error when the decoder did not converge or its syndrome check
@@ -6,10 +6,17 @@ failed (ldpc_awgn.rs:157); here a converged frame whose message differs from
 the sent information bits also counts (undetected error).  Results are
 reported per Eb/N0 like SimulationResult (crates/gf2-coding/src/simulation.rs:13-116):
 frames, frame errors, bit errors, FER, BER, mean iterations, plus info
-Gbit/s of the decode calls.
+Gbit/s of the decode calls.  Frames come from the on-device generator
+(gf2l_frames_generate: seeded Philox info bits, staircase encoder, BPSK/AWGN
+LLRs), so a sweep never leaves the GPU except for the per-batch error count.
 
-    python -m gf2_amd.fer --code n64800 --ebn0 1.5:2.5:0.25 --frames 4096 \
+    python -m gf2_amd.fer --code n64800 --ebn0 1.5:2.5:0.25 --frames 4096 \\
         --max-frames 65536 --min-errors 50 --json out.json
+    python -m gf2_amd.fer --code n16200-k9720 --target-fer 1e-2
+
+A decoder is any callable (d_llr, frames, outs) -> None that fills the
+device outputs [msg words, iterations, converged, syndrome_ok]; plain BP and
+the learned decoder with fallback are provided.
 """
 from __future__ import annotations
 
@@ -18,12 +25,13 @@ import json
 import sys
 import time
 from dataclasses import asdict, dataclass
-from typing import List
+from typing import Callable, List, Optional
 
 import numpy as np
 
-from . import channel, codes
-from .ldpc import Context, LdpcCode, decode_batch_arrays
+from . import _lib, codes
+from .device import DeviceBuffer, generate_frames
+from .ldpc import Context, LdpcCode
 
 
 @dataclass
@@ -39,39 +47,88 @@ class FerPoint:
     info_gbps: float
 
 
-def sweep(spec: codes.QcIraSpec, ebn0_list: List[float], frames_per_batch: int,
-          max_frames: int, min_errors: int, max_it: int = 50, seed: int = 1,
-          ctx: Context = None, log=sys.stderr) -> List[FerPoint]:
-    ctx = ctx or Context(0)
-    base = spec.base_matrix()
-    code = LdpcCode(spec.qc(), ctx)
+def plain_decoder(code: LdpcCode, max_it: int = 50) -> Callable:
+    lib = _lib.load()
+    ctx = code.context
+
+    def run(d_llr, frames, outs):
+        _lib.check(ctx.handle, lib.gf2l_decode_batch_device(
+            ctx.handle, code.handle, d_llr.ptr, frames, max_it, outs[0].ptr, outs[1].ptr,
+            outs[2].ptr, outs[3].ptr))
+    return run
+
+
+def learned_decoder(nms, fallback_max_it: int = 50) -> Callable:
+    def run(d_llr, frames, outs):
+        nms.decode_device(d_llr, frames, fallback_max_it, outs[0], outs[1], outs[2], outs[3], None)
+    return run
+
+
+def count_errors(msg: np.ndarray, cw: np.ndarray, conv: np.ndarray, synd: np.ndarray, k: int):
+    """Frame and bit errors of decoded message words against the codewords'
+    first k bits (systematic codes)."""
+    kw = (k + 63) // 64
+    diff = (msg[:, :kw] ^ cw[:, :kw]).copy()
+    if k % 64:
+        diff[:, -1] &= np.uint64((1 << (k % 64)) - 1)
+    wrong = np.unpackbits(diff.view(np.uint8), axis=1).sum(axis=1)
+    err = (~conv) | (~synd) | (wrong > 0)
+    return int(err.sum()), int(wrong.sum())
+
+
+def sweep(code: LdpcCode, ebn0_list: List[float], frames_per_batch: int, max_frames: int,
+          min_errors: int, decoder: Optional[Callable] = None, seed: int = 1, name: str = "",
+          log=sys.stderr) -> List[FerPoint]:
+    """FER/BER per Eb/N0: batches until max_frames or min_errors."""
+    ctx = code.context
+    decoder = decoder or plain_decoder(code)
+    n, k = code.n(), code.k()
+    F = frames_per_batch
+    kw, nw = (k + 63) // 64, (n + 63) // 64
+    outs = [DeviceBuffer(ctx, F * kw * 8), DeviceBuffer(ctx, F * 4), DeviceBuffer(ctx, F),
+            DeviceBuffer(ctx, F)]
     out = []
     for ebn0 in ebn0_list:
-        nf = fe = be = 0
-        iters = 0
+        nf = fe = be = iters = 0
         t_dec = 0.0
         b = 0
         while nf < max_frames and fe < min_errors:
-            info, _, llr = channel.random_frames(base, spec.z, frames_per_batch, ebn0,
-                                                 seed=seed * 1000003 + b * 7919 + int(ebn0 * 1000))
+            d_llr, d_cw = generate_frames(code, seed * 1000003 + int(round(ebn0 * 1000)), b * F, F,
+                                          ebn0, ebn0)
             t0 = time.perf_counter()
-            msg, it, conv, synd = decode_batch_arrays(code, llr, max_it)
+            decoder(d_llr, F, outs)
+            ctx.synchronize()
             t_dec += time.perf_counter() - t0
-            bits = channel.unpack_words(msg, spec.k)
-            wrong = (bits != info).sum(axis=1)
-            err = (~conv) | (~synd) | (wrong > 0)
-            nf += frames_per_batch
-            fe += int(err.sum())
-            be += int(wrong.sum())
+            msg = outs[0].download(np.uint64, (F, kw))
+            it = outs[1].download(np.uint32, F)
+            conv = outs[2].download(np.uint8, F).astype(bool)
+            synd = outs[3].download(np.uint8, F).astype(bool)
+            cw = d_cw.download(np.uint64, (F, nw))
+            e, w = count_errors(msg, cw, conv, synd, k)
+            nf += F
+            fe += e
+            be += w
             iters += int(it.sum())
             b += 1
-        p = FerPoint(ebn0, nf, fe, be, fe / nf, be / (nf * spec.k), iters / nf, t_dec,
-                     nf * spec.k / t_dec / 1e9)
-        print(f"[fer] {spec.name} Eb/N0={ebn0:.2f} frames={nf} errors={fe} FER={p.fer:.3e} "
-              f"BER={p.ber:.3e} it={p.mean_iterations:.2f} {p.info_gbps:.3f} Gbit/s",
-              file=log, flush=True)
+        p = FerPoint(ebn0, nf, fe, be, fe / nf, be / (nf * k), iters / nf, t_dec, nf * k / t_dec / 1e9)
+        print(f"[fer] {name} Eb/N0={ebn0:.2f} frames={nf} errors={fe} FER={p.fer:.3e} "
+              f"BER={p.ber:.3e} it={p.mean_iterations:.2f} {p.info_gbps:.3f} Gbit/s", file=log, flush=True)
         out.append(p)
     return out
+
+
+def calibrate(code: LdpcCode, target_fer: float, lo: float, hi: float, frames: int = 2048,
+              steps: int = 7, decoder: Optional[Callable] = None, seed: int = 5, log=sys.stderr) -> float:
+    """Eb/N0 (dB) where the FER crosses target_fer, by bisection on [lo, hi]
+    with `frames` frames per probe (FER is decreasing in Eb/N0)."""
+    for _ in range(steps):
+        mid = 0.5 * (lo + hi)
+        p = sweep(code, [mid], frames, frames, frames + 1, decoder, seed, log=log)[0]
+        if p.fer > target_fer:
+            lo = mid
+        else:
+            hi = mid
+    return round(0.5 * (lo + hi), 3)
 
 
 def _parse_range(s: str) -> List[float]:
@@ -81,18 +138,45 @@ def _parse_range(s: str) -> List[float]:
     return [float(x) for x in s.split(",")]
 
 
+def spec_by_name(name: str) -> codes.QcIraSpec:
+    """n2304, n64800, or n<N>-k<K> for one of the twelve Z=360 codes."""
+    if name == "n2304":
+        return codes.n2304_r12()
+    if name == "n64800":
+        return codes.n64800_r12()
+    for s in codes.mixed_twelve():
+        if name == f"n{s.n}-k{s.k}":
+            return s
+    raise ValueError(f"unknown code {name!r}")
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--code", default="n64800", choices=["n64800", "n2304"])
+    ap.add_argument("--code", default="n64800")
     ap.add_argument("--ebn0", default="1.5:2.5:0.25")
     ap.add_argument("--frames", type=int, default=4096)
     ap.add_argument("--max-frames", type=int, default=16384)
     ap.add_argument("--min-errors", type=int, default=50)
     ap.add_argument("--max-it", type=int, default=50)
+    ap.add_argument("--weights", default=None, help="learned decoder weights (.npz); plain BP if absent")
+    ap.add_argument("--target-fer", type=float, default=None,
+                    help="find the Eb/N0 of this FER by bisection over the --ebn0 range ends")
     ap.add_argument("--json", default=None)
     a = ap.parse_args(argv)
-    spec = codes.n64800_r12() if a.code == "n64800" else codes.n2304_r12()
-    pts = sweep(spec, _parse_range(a.ebn0), a.frames, a.max_frames, a.min_errors, a.max_it)
+    spec = spec_by_name(a.code)
+    ctx = Context(0)
+    code = LdpcCode(spec.qc(), ctx)
+    if a.weights:
+        from .nms import NmsDecoder
+        decoder = learned_decoder(NmsDecoder.load(code, a.weights), a.max_it)
+    else:
+        decoder = plain_decoder(code, a.max_it)
+    if a.target_fer is not None:
+        r = _parse_range(a.ebn0)
+        e = calibrate(code, a.target_fer, min(r), max(r), a.frames, decoder=decoder)
+        print(json.dumps({"code": spec.name, "target_fer": a.target_fer, "ebn0_db": e}))
+        return 0
+    pts = sweep(code, _parse_range(a.ebn0), a.frames, a.max_frames, a.min_errors, decoder, name=spec.name)
     if a.json:
         with open(a.json, "w") as f:
             json.dump({"code": spec.name, "points": [asdict(p) for p in pts]}, f, indent=1)

@@ -1,0 +1,50 @@
+"""GPU checks of the autotuner: every candidate launch shape gives identical
+results (the tuner raises otherwise), the best one is applied and cached."""
+import numpy as np
+import pytest
+
+from gf2_amd import _lib, autotune, codes
+from gf2_amd.device import DeviceBuffer
+from gf2_amd.ldpc import Context, LdpcCode
+from gf2_amd.mixed import CodeSet
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    autotune.apply(c, {"path": _lib.PATH_AUTO, "wg_per_cu": 3, "xcd_map": 1, "graph_iters": 4})
+
+
+def test_tune_single_code(ctx, tmp_path):
+    code = LdpcCode(codes.n2304_r12().qc(), ctx)
+    cache = autotune.TuningCache(str(tmp_path / "t.json"))
+    r = autotune.tune_code(code, 256, 2.0, max_it=20, repeats=1,
+                           knobs=("path", "wg_per_cu", "xcd_map"), cache=cache)
+    assert len(r["results"]) == 3 * 4 * 2  # every path can run this code
+    assert r["config"] in [cfg for cfg, _ in r["results"]]
+    again = autotune.TuningCache(str(tmp_path / "t.json"))
+    assert autotune.apply_cached(ctx, [code], 200, cache=again) == r["config"]  # same power-of-two bucket
+
+
+def test_tune_code_set(ctx, tmp_path):
+    specs = codes.mixed_twelve()[:3]
+    gcodes = [LdpcCode(s.qc(), ctx) for s in specs]
+    cs = CodeSet(gcodes)
+    ids = np.repeat(np.arange(3, dtype=np.uint16), 40)
+    n = np.array([s.n for s in specs])
+    off = np.concatenate([[0], np.cumsum(n[ids])])
+    d_llr = DeviceBuffer(ctx, int(off[-1]) * 4)
+    lib = _lib.load()
+    for c in range(3):
+        f0 = int(np.nonzero(ids == c)[0][0])
+        _lib.check(ctx.handle, lib.gf2l_frames_generate(ctx.handle, gcodes[c].handle, 9 + c, 0, 40, 2.5, 2.5,
+                                                        d_llr.ptr + int(off[f0]) * 4, None))
+    ctx.synchronize()
+    cache = autotune.TuningCache(str(tmp_path / "m.json"))
+    r = autotune.tune_codeset(cs, ids, d_llr, max_it=20, repeats=1,
+                              knobs=("wg_per_cu", "graph_iters"), cache=cache)
+    assert len(r["results"]) == 4 * 3
+    assert r["key"].startswith("mixed:") and cache.get(r["key"]) == r["config"]
