@@ -38,6 +38,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per frame-iteration of the check-state kernels on this workload,
+# measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (weighted-iteration
+# dispatches, all 1024 frames running) and scaled by the counts of a plain
+# copy of known size (bytes read = 2 x FETCH_SIZE, written = 1 x WRITE_SIZE):
+# profiles/n64800_state_kernels_bench.md.
+MEASURED_TRAFFIC_PER_FI = {"check": 771_986 + 388_800, "variable": 1_653_778 + 285_781}
 # Plain min-sum FER on this code is about 1e-3 here: the gf2_amd.fer sweep
 # measured 2.1e-3 at 1.60 dB (16384 frames) and 0 errors in 16384 at 1.65 dB.
 DEFAULT_EBN0 = 1.62
@@ -191,7 +197,10 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": kernels[dom],
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": (round(MEASURED_TRAFFIC_PER_FI[names[dom]] * frame_iters / launches[dom])
+                        if a.path in ("auto", "state") and names[dom] in MEASURED_TRAFFIC_PER_FI else None),
+            "traffic_unit": "bytes per launch (measured bytes per frame-iteration x mean frame-iterations per launch)",
             "avg_launch_us": round(ms[dom] / launches[dom] * 1e3, 2),
             "launches": int(launches[dom]),
             "bytes_per_frame_iteration": per_fi[names[dom]],
