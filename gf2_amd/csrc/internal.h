@@ -111,6 +111,8 @@ struct gf2l_code {
     int device = 0;
 };
 
+struct gf2l_nms;
+
 struct gf2l_ctx {
     int device = 0;
     gf2l::Profile prof;
@@ -151,6 +153,10 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
                   uint32_t max_it, const float *d_w, uint32_t fb_it, uint64_t *d_msg,
                   uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb,
                   float *h_trace_c2v, float *h_trace_bel, uint32_t trace_iters);
+
+// Weights [iterations][nbe] (device), weighted iterations and code of a
+// learned decoder (nms.hip).
+const float *nms_weights(const gf2l_nms *q, uint32_t *iterations, const gf2l_code **code);
 
 // Plain min-sum on a general sparse code (csr.hip).
 int decode_csr(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, uint32_t F, uint32_t max_it,

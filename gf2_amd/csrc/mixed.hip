@@ -15,7 +15,11 @@
 // order) through the slot table.
 //
 // Iterations.  All frames start together, so one device counter holds the
-// iteration number of every running frame.  An iteration is a check-update
+// iteration number of every running frame.  With learned decoders (one weight
+// set per code) the first T iterations are the weighted rule, and frames
+// still failing the syndrome after T restart from their LLRs together at
+// iteration T + 1 with plain min-sum (the single-code learned decoder's
+// fallback, gf2l_nms_decode_device).  An iteration is a check-update
 // and a variable-update launch per degree class present in the set
 // (registers sized per class) and one syndrome/retirement launch, each over
 // the running frames of every code; the running lists (one per code,
@@ -49,8 +53,12 @@ struct CallArgs {
     uint64_t *msg;
     uint32_t *iters;
     uint8_t *conv, *synd;
-    uint32_t max_it, frames;
+    uint8_t *fb;                  // per frame: re-decoded by the fallback (may be null)
+    uint32_t max_total;           // a frame retires at this iteration count
+    uint32_t weighted;            // T: weighted iterations (0: plain min-sum only)
+    uint32_t frames, pad;
     CodeSeg seg[kMaxCodes];
+    const float *w[kMaxCodes];    // per code [T][nbe] weights (weighted decodes)
 };
 
 // Caller-side position of the frame in a slot.
@@ -190,9 +198,10 @@ __device__ uint32_t build_prefix(const MixShared &m, uint32_t mask, const uint32
 
 // Check update of the running frames of the codes in `mask` (one degree
 // class: the kernel's registers are sized for MAXD).  FIRST: inputs are the
-// channel LLRs, no previous messages.
-template <int MAXD, bool FIRST>
-__global__ void __launch_bounds__(512) cn_mix_kernel(MixShared m, uint32_t mask, int xcd) {
+// channel LLRs, no previous messages.  NMS: the previous messages were
+// weighted by iteration t-1's weights (t = this iteration, 1-based).
+template <int MAXD, bool FIRST, bool NMS>
+__global__ void __launch_bounds__(512) cn_mix_kernel(MixShared m, uint32_t mask, int xcd, uint32_t t) {
     __shared__ uint32_t s_start[kMaxCodes + 1];
     const uint32_t cur = ldc(m.misc);
     const uint32_t *cnt = m.counts + cur * kMaxCodes;
@@ -205,6 +214,8 @@ __global__ void __launch_bounds__(512) cn_mix_kernel(MixShared m, uint32_t mask,
         if (start == end) continue;
         const CodeView cv = uniform_view(m.codes + c);
         const CodeSeg seg = uniform_seg(m.args->seg + c);
+        const float *w_prev = nullptr;
+        if constexpr (NMS) w_prev = ldc(m.args->w + c) + (size_t)(t - 2) * cv.nbe;
         for (uint32_t j = first_item(start, sp.q, sp.Q); j < end; j += sp.Q) {
             const uint32_t local = j - start;
             const uint32_t br = local % cv.mb;
@@ -214,14 +225,15 @@ __global__ void __launch_bounds__(512) cn_mix_kernel(MixShared m, uint32_t mask,
             if constexpr (FIRST)
                 cn_state_item<MAXD, true, false>(cv, br, fs, fs, m.llr + seg.llr + r * cv.n, nullptr);
             else
-                cn_state_item<MAXD, false, false>(cv, br, fs, fs,
-                                                  m.belief + seg.bel + r * ((uint64_t)cv.nb * cv.zp), nullptr);
+                cn_state_item<MAXD, false, NMS>(cv, br, fs, fs,
+                                                m.belief + seg.bel + r * ((uint64_t)cv.nb * cv.zp), w_prev);
         }
     }
 }
 
-template <int MAXD>
-__global__ void __launch_bounds__(512) vn_mix_kernel(MixShared m, uint32_t mask, int xcd) {
+// Variable update; NMS: messages weighted by iteration t's weights.
+template <int MAXD, bool NMS>
+__global__ void __launch_bounds__(512) vn_mix_kernel(MixShared m, uint32_t mask, int xcd, uint32_t t) {
     __shared__ uint32_t s_start[kMaxCodes + 1];
     const uint32_t cur = ldc(m.misc);
     const uint32_t *cnt = m.counts + cur * kMaxCodes;
@@ -234,15 +246,17 @@ __global__ void __launch_bounds__(512) vn_mix_kernel(MixShared m, uint32_t mask,
         if (start == end) continue;
         const CodeView cv = uniform_view(m.codes + c);
         const CodeSeg seg = uniform_seg(m.args->seg + c);
+        const float *w_cur = nullptr;
+        if constexpr (NMS) w_cur = ldc(m.args->w + c) + (size_t)(t - 1) * cv.nbe;
         for (uint32_t j = first_item(start, sp.q, sp.Q); j < end; j += sp.Q) {
             const uint32_t local = j - start;
             const uint32_t bc = local % cv.nb;
             const uint32_t slot = ldc(list + seg.list_off + sp.x + sp.step * (local / cv.nb));
             const uint64_t r = slot - seg.list_off;
-            vn_state_item<MAXD, false>(cv, bc, m.state + seg.st + r * (3ull * cv.mb * cv.zp),
-                                       m.llr + seg.llr + r * cv.n + (size_t)bc * cv.z,
-                                       m.belief + seg.bel + r * ((uint64_t)cv.nb * cv.zp) + (size_t)bc * cv.zp,
-                                       m.hb + seg.hb + r * ((uint64_t)cv.nb * cv.zw) + (size_t)bc * cv.zw, nullptr);
+            vn_state_item<MAXD, NMS>(cv, bc, m.state + seg.st + r * (3ull * cv.mb * cv.zp),
+                                     m.llr + seg.llr + r * cv.n + (size_t)bc * cv.z,
+                                     m.belief + seg.bel + r * ((uint64_t)cv.nb * cv.zp) + (size_t)bc * cv.zp,
+                                     m.hb + seg.hb + r * ((uint64_t)cv.nb * cv.zw) + (size_t)bc * cv.zw, w_cur);
         }
     }
     // The syndrome pass of this iteration appends to the other list: every
@@ -254,6 +268,8 @@ __global__ void __launch_bounds__(512) vn_mix_kernel(MixShared m, uint32_t mask,
 
 // One 64-lane workgroup per running frame: syndrome, then retire the frame
 // (converged, or at the iteration cap) or append it to its code's next list.
+// A frame failing at the end of the weighted iterations (it == T < cap)
+// stays on the list: the next iteration restarts it from its LLRs.
 // The last workgroup to finish advances the iteration counter and flips the
 // list parity.
 __global__ void __launch_bounds__(64) syn_mix_kernel(MixShared m) {
@@ -265,7 +281,8 @@ __global__ void __launch_bounds__(64) syn_mix_kernel(MixShared m) {
     const uint32_t *list = m.lists + (size_t)cur * m.list_cap;
     uint32_t *next_cnt = m.counts + (cur ^ 1u) * kMaxCodes;
     uint32_t *next_list = m.lists + (size_t)(cur ^ 1u) * m.list_cap;
-    const uint32_t max_it = ldc(&m.args->max_it);
+    const uint32_t max_total = ldc(&m.args->max_total);
+    const uint32_t weighted = ldc(&m.args->weighted);
     const MixSplit sp(0);
     const uint32_t total = build_prefix<kFrames>(m, 0xffffu, cnt, sp, s_start);
     (void)total;
@@ -278,7 +295,7 @@ __global__ void __launch_bounds__(64) syn_mix_kernel(MixShared m) {
             const uint32_t slot = ldc(list + seg.list_off + item - start);
             const uint64_t r = slot - seg.list_off;
             const int unsat = frame_unsat(cv, sh, m.hb + seg.hb + r * ((uint64_t)cv.nb * cv.zw));
-            if (!unsat || it >= max_it) {
+            if (!unsat || it >= max_total) {
                 const SlotInfo si = m.slots[slot];
                 const CallArgs *a = m.args;
                 frame_message(cv, sh, a->msg + si.msg);
@@ -286,6 +303,7 @@ __global__ void __launch_bounds__(64) syn_mix_kernel(MixShared m) {
                     a->iters[si.frame] = it;
                     a->conv[si.frame] = (uint8_t)!unsat;
                     a->synd[si.frame] = (uint8_t)!unsat;
+                    if (a->fb) a->fb[si.frame] = (uint8_t)(weighted && it > weighted);
                 }
             } else if (threadIdx.x == 0) {
                 next_list[seg.list_off + atomicAdd(next_cnt + c, 1u)] = slot;
@@ -423,18 +441,19 @@ Shape launch_shape(const gf2l_ctx *ctx, const gf2l_codeset *cs) {
     return s;
 }
 
-// Enqueues one iteration: check and variable launches per degree class, then
-// the syndrome launch.
-template <bool FIRST>
-void enqueue_iteration(const gf2l_codeset *cs, const Shape &sh, hipStream_t s) {
+// Enqueues iteration t: check and variable launches per degree class, then
+// the syndrome launch.  FIRST: inputs are the LLRs; NMS: weighted rule.
+template <bool FIRST, bool NMS>
+void enqueue_iteration(const gf2l_codeset *cs, const Shape &sh, hipStream_t s, uint32_t t) {
     const MixShared m = cs->m;
     const dim3 g(sh.grid), b(sh.block);
-    if (cs->cn_mask[0]) hipLaunchKernelGGL((cn_mix_kernel<8, FIRST>), g, b, 0, s, m, cs->cn_mask[0], sh.xcd);
-    if (cs->cn_mask[1]) hipLaunchKernelGGL((cn_mix_kernel<16, FIRST>), g, b, 0, s, m, cs->cn_mask[1], sh.xcd);
-    if (cs->cn_mask[2]) hipLaunchKernelGGL((cn_mix_kernel<27, FIRST>), g, b, 0, s, m, cs->cn_mask[2], sh.xcd);
-    if (cs->vn_mask[0]) hipLaunchKernelGGL(vn_mix_kernel<8>, g, b, 0, s, m, cs->vn_mask[0], sh.xcd);
-    if (cs->vn_mask[1]) hipLaunchKernelGGL(vn_mix_kernel<16>, g, b, 0, s, m, cs->vn_mask[1], sh.xcd);
-    if (cs->vn_mask[2]) hipLaunchKernelGGL(vn_mix_kernel<32>, g, b, 0, s, m, cs->vn_mask[2], sh.xcd);
+    constexpr bool CN_NMS = NMS && !FIRST;
+    if (cs->cn_mask[0]) hipLaunchKernelGGL((cn_mix_kernel<8, FIRST, CN_NMS>), g, b, 0, s, m, cs->cn_mask[0], sh.xcd, t);
+    if (cs->cn_mask[1]) hipLaunchKernelGGL((cn_mix_kernel<16, FIRST, CN_NMS>), g, b, 0, s, m, cs->cn_mask[1], sh.xcd, t);
+    if (cs->cn_mask[2]) hipLaunchKernelGGL((cn_mix_kernel<27, FIRST, CN_NMS>), g, b, 0, s, m, cs->cn_mask[2], sh.xcd, t);
+    if (cs->vn_mask[0]) hipLaunchKernelGGL((vn_mix_kernel<8, NMS>), g, b, 0, s, m, cs->vn_mask[0], sh.xcd, t);
+    if (cs->vn_mask[1]) hipLaunchKernelGGL((vn_mix_kernel<16, NMS>), g, b, 0, s, m, cs->vn_mask[1], sh.xcd, t);
+    if (cs->vn_mask[2]) hipLaunchKernelGGL((vn_mix_kernel<32, NMS>), g, b, 0, s, m, cs->vn_mask[2], sh.xcd, t);
     hipLaunchKernelGGL(syn_mix_kernel, dim3(sh.g_syn), dim3(64), sh.sh, s, m);
 }
 
@@ -447,7 +466,7 @@ int capture(gf2l_ctx *ctx, gf2l_codeset *cs, const Shape &sh, uint32_t unroll) {
     hipStream_t s = ctx->stream;
     hipGraph_t graph = nullptr;
     GF2L_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (uint32_t u = 0; u < unroll; ++u) enqueue_iteration<false>(cs, sh, s);
+    for (uint32_t u = 0; u < unroll; ++u) enqueue_iteration<false, false>(cs, sh, s, 0);
     const hipError_t launch = hipGetLastError();
     const hipError_t end = hipStreamEndCapture(s, &graph);
     if (launch != hipSuccess || end != hipSuccess) {
@@ -483,6 +502,152 @@ int running_frames(gf2l_ctx *ctx, gf2l_codeset *cs, uint32_t *out) {
 }
 
 }  // namespace
+}  // namespace gf2l
+
+namespace gf2l {
+
+// Decodes a mixed batch.  Plain (decoders == NULL): min-sum for up to
+// tail_it iterations.  Learned: decoders[c] (code c's learned decoder, all
+// with the same T) for T weighted iterations, then plain min-sum from the
+// LLRs for up to tail_it iterations for frames still failing.
+int decode_mixed(gf2l_ctx *ctx, gf2l_codeset *cs, const gf2l_nms *const *decoders, const uint16_t *code_ids,
+                 size_t frames, const float *d_llr, uint32_t tail_it, uint64_t *d_msg_words,
+                 uint32_t *d_iterations, uint8_t *d_converged, uint8_t *d_syndrome_ok, uint8_t *d_fallback) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    if (!cs) return fail(ctx, GF2L_ERR_INVALID, "null code set");
+    if (cs->device != ctx->device) return fail(ctx, GF2L_ERR_INVALID, "code set built on another device");
+    if (frames == 0) return GF2L_OK;
+    if (!code_ids || !d_llr || !d_msg_words || !d_iterations || !d_converged || !d_syndrome_ok)
+        return fail(ctx, GF2L_ERR_INVALID, "null buffer");
+    if (frames > (1u << 24)) return fail(ctx, GF2L_ERR_INVALID, "more than 2^24 frames in one call");
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    const uint32_t F = (uint32_t)frames, K = (uint32_t)cs->codes.size();
+    // learned decoders: one per code of the set, same weighted iteration count
+    uint32_t T = 0;
+    const float *w[kMaxCodes] = {nullptr};
+    if (decoders) {
+        for (uint32_t c = 0; c < K; ++c) {
+            if (!decoders[c]) return fail(ctx, GF2L_ERR_INVALID, "null learned decoder for code " + std::to_string(c));
+            uint32_t Tc = 0;
+            const gf2l_code *code = nullptr;
+            w[c] = nms_weights(decoders[c], &Tc, &code);
+            if (code != cs->codes[c])
+                return fail(ctx, GF2L_ERR_INVALID, "learned decoder " + std::to_string(c) + " is for another code");
+            if (c > 0 && Tc != T)
+                return fail(ctx, GF2L_ERR_INVALID, "learned decoders differ in their iteration count");
+            T = Tc;
+        }
+    }
+    const uint32_t max_total = T + tail_it;
+    hipStream_t s = ctx->stream;
+    // per-code frame counts and workspace totals
+    uint32_t per_code[kMaxCodes] = {0};
+    uint64_t t_llr = 0, t_st = 0, t_bel = 0, t_hb = 0, t_msg = 0, items_c = 0, items_v = 0;
+    for (uint32_t f = 0; f < F; ++f) {
+        const uint32_t c = code_ids[f];
+        if (c >= K)
+            return fail(ctx, GF2L_ERR_INVALID, "code id " + std::to_string(c) + " of frame " + std::to_string(f) +
+                                                   " outside the code set");
+        const CodeView &v = cs->codes[c]->view;
+        t_llr += v.n;
+        t_st += 3ull * v.mb * v.zp;
+        t_bel += (uint64_t)v.nb * v.zp;
+        t_hb += (uint64_t)v.nb * v.zw;
+        t_msg += (v.k + 63) / 64;
+        items_c += v.mb;
+        items_v += v.nb;
+        per_code[c]++;
+    }
+    if (items_c > 0xffffffffull || items_v > 0xffffffffull)
+        return fail(ctx, GF2L_ERR_UNSUPPORTED, "too many block items in one call");
+    if (max_total == 0) {  // fresh decoders: zero word, which satisfies every check (core.rs:906-937)
+        GF2L_HIP(ctx, hipMemsetAsync(d_msg_words, 0, t_msg * sizeof(uint64_t), s));
+        GF2L_HIP(ctx, hipMemsetAsync(d_iterations, 0, (size_t)F * sizeof(uint32_t), s));
+        GF2L_HIP(ctx, hipMemsetAsync(d_converged, 0, F, s));
+        GF2L_HIP(ctx, hipMemsetAsync(d_syndrome_ok, 1, F, s));
+        if (d_fallback) GF2L_HIP(ctx, hipMemsetAsync(d_fallback, 0, F, s));
+        return GF2L_OK;
+    }
+    int rc = ensure_work(ctx, cs, F, t_llr, t_st, t_bel, t_hb);
+    if (rc != GF2L_OK) return rc;
+    // fill the pinned staging block (the previous call has synchronized)
+    const Upload up(cs->cap_frames);
+    char *h = cs->h_stage;
+    CallArgs *ha = reinterpret_cast<CallArgs *>(h + up.args);
+    *ha = CallArgs{};
+    ha->llr = d_llr;
+    ha->msg = d_msg_words;
+    ha->iters = d_iterations;
+    ha->conv = d_converged;
+    ha->synd = d_syndrome_ok;
+    ha->fb = d_fallback;
+    ha->max_total = max_total;
+    ha->weighted = T;
+    ha->frames = F;
+    for (uint32_t c = 0; c < K; ++c) ha->w[c] = w[c];
+    uint64_t o_llr = 0, o_st = 0, o_bel = 0, o_hb = 0;
+    uint32_t o_slot = 0, fill[kMaxCodes];
+    for (uint32_t c = 0; c < K; ++c) {
+        const CodeView &v = cs->codes[c]->view;
+        ha->seg[c] = CodeSeg{o_llr, o_st, o_bel, o_hb, o_slot, per_code[c]};
+        o_llr += (uint64_t)per_code[c] * v.n;
+        o_st += (uint64_t)per_code[c] * 3ull * v.mb * v.zp;
+        o_bel += (uint64_t)per_code[c] * v.nb * v.zp;
+        o_hb += (uint64_t)per_code[c] * v.nb * v.zw;
+        o_slot += per_code[c];
+        fill[c] = 0;
+    }
+    SlotInfo *h_slots = reinterpret_cast<SlotInfo *>(h + up.slots);
+    uint64_t f_llr = 0, f_msg = 0;
+    for (uint32_t f = 0; f < F; ++f) {
+        const uint32_t c = code_ids[f];
+        const CodeView &v = cs->codes[c]->view;
+        h_slots[ha->seg[c].list_off + fill[c]++] = SlotInfo{f, c, f_llr, f_msg};
+        f_llr += v.n;
+        f_msg += (v.k + 63) / 64;
+    }
+    GF2L_HIP(ctx, hipMemcpyAsync(cs->d_base, h, up.slots + (size_t)F * sizeof(SlotInfo), hipMemcpyHostToDevice, s));
+    const uint32_t g_init = std::min<uint32_t>((F + 255) / 256, 1024);
+    hipLaunchKernelGGL(mix_init_kernel, dim3(g_init), dim3(256), 0, s, cs->m);
+    hipLaunchKernelGGL(mix_gather_kernel, dim3(std::min<uint32_t>(F, 4096)), dim3(256), 0, s, cs->m);
+    const Shape sh = launch_shape(ctx, cs);
+    // weighted iterations 1..T (direct launches: each reads its own weights)
+    for (uint32_t t = 1; t <= T; ++t) {
+        if (t == 1)
+            enqueue_iteration<true, true>(cs, sh, s, t);
+        else
+            enqueue_iteration<false, true>(cs, sh, s, t);
+    }
+    if (tail_it == 0) {
+        GF2L_HIP(ctx, hipGetLastError());
+        GF2L_HIP(ctx, hipStreamSynchronize(s));
+        return GF2L_OK;
+    }
+    // plain iterations T+1 .. T+tail_it: the first from the LLRs (a fresh
+    // decode, or the fallback's restart), later ones replayed from the graph
+    uint32_t running = F;
+    if (T) {
+        rc = running_frames(ctx, cs, &running);
+        if (rc != GF2L_OK) return rc;
+    }
+    if (running) enqueue_iteration<true, false>(cs, sh, s, T + 1);
+    GF2L_HIP(ctx, hipGetLastError());
+    if (running && tail_it > 1) {
+        rc = capture(ctx, cs, sh, (uint32_t)ctx->tune.graph_iters);
+        if (rc != GF2L_OK) return rc;
+        rc = running_frames(ctx, cs, &running);
+        if (rc != GF2L_OK) return rc;
+        // replay until every frame has retired (at most ceil((tail_it - 1) / U) replays)
+        for (uint32_t done_it = 1; running && done_it < tail_it; done_it += cs->g_unroll) {
+            GF2L_HIP(ctx, hipGraphLaunch(cs->exec, s));
+            rc = running_frames(ctx, cs, &running);
+            if (rc != GF2L_OK) return rc;
+        }
+    }
+    GF2L_HIP(ctx, hipStreamSynchronize(s));
+    return GF2L_OK;
+}
+
 }  // namespace gf2l
 
 using gf2l::fail;
@@ -548,101 +713,18 @@ void gf2l_codeset_destroy(gf2l_codeset *cs) {
 int gf2l_decode_mixed_device(gf2l_ctx *ctx, gf2l_codeset *cs, const uint16_t *code_ids, size_t frames,
                              const float *d_llr, uint32_t max_it, uint64_t *d_msg_words,
                              uint32_t *d_iterations, uint8_t *d_converged, uint8_t *d_syndrome_ok) {
-    using namespace gf2l;
+    return gf2l::decode_mixed(ctx, cs, nullptr, code_ids, frames, d_llr, max_it, d_msg_words, d_iterations,
+                              d_converged, d_syndrome_ok, nullptr);
+}
+
+int gf2l_decode_mixed_nms_device(gf2l_ctx *ctx, gf2l_codeset *cs, const gf2l_nms *const *decoders,
+                                 const uint16_t *code_ids, size_t frames, const float *d_llr,
+                                 uint32_t fallback_max_it, uint64_t *d_msg_words, uint32_t *d_iterations,
+                                 uint8_t *d_converged, uint8_t *d_syndrome_ok, uint8_t *d_fallback) {
     if (!ctx) return GF2L_ERR_INVALID;
-    if (!cs) return fail(ctx, GF2L_ERR_INVALID, "null code set");
-    if (cs->device != ctx->device) return fail(ctx, GF2L_ERR_INVALID, "code set built on another device");
-    if (frames == 0) return GF2L_OK;
-    if (!code_ids || !d_llr || !d_msg_words || !d_iterations || !d_converged || !d_syndrome_ok)
-        return fail(ctx, GF2L_ERR_INVALID, "null buffer");
-    if (frames > (1u << 24)) return fail(ctx, GF2L_ERR_INVALID, "more than 2^24 frames in one call");
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    const uint32_t F = (uint32_t)frames, K = (uint32_t)cs->codes.size();
-    hipStream_t s = ctx->stream;
-    // per-code frame counts and workspace totals
-    uint32_t per_code[kMaxCodes] = {0};
-    uint64_t t_llr = 0, t_st = 0, t_bel = 0, t_hb = 0, t_msg = 0, items_c = 0, items_v = 0;
-    for (uint32_t f = 0; f < F; ++f) {
-        const uint32_t c = code_ids[f];
-        if (c >= K)
-            return fail(ctx, GF2L_ERR_INVALID, "code id " + std::to_string(c) + " of frame " + std::to_string(f) +
-                                                   " outside the code set");
-        const CodeView &v = cs->codes[c]->view;
-        t_llr += v.n;
-        t_st += 3ull * v.mb * v.zp;
-        t_bel += (uint64_t)v.nb * v.zp;
-        t_hb += (uint64_t)v.nb * v.zw;
-        t_msg += (v.k + 63) / 64;
-        items_c += v.mb;
-        items_v += v.nb;
-        per_code[c]++;
-    }
-    if (items_c > 0xffffffffull || items_v > 0xffffffffull)
-        return fail(ctx, GF2L_ERR_UNSUPPORTED, "too many block items in one call");
-    if (max_it == 0) {  // fresh decoders: zero word, which satisfies every check (core.rs:906-937)
-        GF2L_HIP(ctx, hipMemsetAsync(d_msg_words, 0, t_msg * sizeof(uint64_t), s));
-        GF2L_HIP(ctx, hipMemsetAsync(d_iterations, 0, (size_t)F * sizeof(uint32_t), s));
-        GF2L_HIP(ctx, hipMemsetAsync(d_converged, 0, F, s));
-        GF2L_HIP(ctx, hipMemsetAsync(d_syndrome_ok, 1, F, s));
-        return GF2L_OK;
-    }
-    int rc = ensure_work(ctx, cs, F, t_llr, t_st, t_bel, t_hb);
-    if (rc != GF2L_OK) return rc;
-    // fill the pinned staging block (the previous call has synchronized)
-    const Upload up(cs->cap_frames);
-    char *h = cs->h_stage;
-    CallArgs *ha = reinterpret_cast<CallArgs *>(h + up.args);
-    *ha = CallArgs{};
-    ha->llr = d_llr;
-    ha->msg = d_msg_words;
-    ha->iters = d_iterations;
-    ha->conv = d_converged;
-    ha->synd = d_syndrome_ok;
-    ha->max_it = max_it;
-    ha->frames = F;
-    uint64_t o_llr = 0, o_st = 0, o_bel = 0, o_hb = 0;
-    uint32_t o_slot = 0, fill[kMaxCodes];
-    for (uint32_t c = 0; c < K; ++c) {
-        const CodeView &v = cs->codes[c]->view;
-        ha->seg[c] = CodeSeg{o_llr, o_st, o_bel, o_hb, o_slot, per_code[c]};
-        o_llr += (uint64_t)per_code[c] * v.n;
-        o_st += (uint64_t)per_code[c] * 3ull * v.mb * v.zp;
-        o_bel += (uint64_t)per_code[c] * v.nb * v.zp;
-        o_hb += (uint64_t)per_code[c] * v.nb * v.zw;
-        o_slot += per_code[c];
-        fill[c] = 0;
-    }
-    SlotInfo *h_slots = reinterpret_cast<SlotInfo *>(h + up.slots);
-    uint64_t f_llr = 0, f_msg = 0;
-    for (uint32_t f = 0; f < F; ++f) {
-        const uint32_t c = code_ids[f];
-        const CodeView &v = cs->codes[c]->view;
-        h_slots[ha->seg[c].list_off + fill[c]++] = SlotInfo{f, c, f_llr, f_msg};
-        f_llr += v.n;
-        f_msg += (v.k + 63) / 64;
-    }
-    GF2L_HIP(ctx, hipMemcpyAsync(cs->d_base, h, up.slots + (size_t)F * sizeof(SlotInfo), hipMemcpyHostToDevice, s));
-    const uint32_t g_init = std::min<uint32_t>((F + 255) / 256, 1024);
-    hipLaunchKernelGGL(mix_init_kernel, dim3(g_init), dim3(256), 0, s, cs->m);
-    hipLaunchKernelGGL(mix_gather_kernel, dim3(std::min<uint32_t>(F, 4096)), dim3(256), 0, s, cs->m);
-    const Shape sh = launch_shape(ctx, cs);
-    enqueue_iteration<true>(cs, sh, s);
-    GF2L_HIP(ctx, hipGetLastError());
-    uint32_t running = 0;
-    if (max_it > 1) {
-        rc = capture(ctx, cs, sh, (uint32_t)ctx->tune.graph_iters);
-        if (rc != GF2L_OK) return rc;
-        rc = running_frames(ctx, cs, &running);
-        if (rc != GF2L_OK) return rc;
-    }
-    // replay until every frame has retired (at most ceil((max_it - 1) / U) replays)
-    for (uint32_t done_it = 1; running && done_it < max_it; done_it += cs->g_unroll) {
-        GF2L_HIP(ctx, hipGraphLaunch(cs->exec, s));
-        rc = running_frames(ctx, cs, &running);
-        if (rc != GF2L_OK) return rc;
-    }
-    GF2L_HIP(ctx, hipStreamSynchronize(s));
-    return GF2L_OK;
+    if (!decoders) return fail(ctx, GF2L_ERR_INVALID, "null decoder list");
+    return gf2l::decode_mixed(ctx, cs, decoders, code_ids, frames, d_llr, fallback_max_it, d_msg_words,
+                              d_iterations, d_converged, d_syndrome_ok, d_fallback);
 }
 
 }  // extern "C"

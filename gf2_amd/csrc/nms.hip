@@ -502,3 +502,13 @@ int gf2l_nms_decode_device(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr,
 }
 
 }  // extern "C"
+
+namespace gf2l {
+
+const float *nms_weights(const gf2l_nms *q, uint32_t *iterations, const gf2l_code **code) {
+    *iterations = q->T;
+    *code = q->code;
+    return q->d_w;
+}
+
+}  // namespace gf2l

@@ -10,7 +10,7 @@ converged, syndrome flag) for all frames in one device call
 from __future__ import annotations
 
 import ctypes
-from typing import List, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -62,6 +62,17 @@ class CodeSet:
             self.ctx.handle, self._h, ids.ctypes.data, ids.size, d_llr, int(max_iterations), d_msg,
             d_iters, d_conv, d_synd))
 
+    def decode_nms_device(self, decoders, code_ids: np.ndarray, d_llr: int, fallback_max_it: int, d_msg: int,
+                          d_iters: int, d_conv: int, d_synd: int, d_fb: Optional[int] = None) -> None:
+        """Learned decode: decoders[c] is an NmsDecoder of code c (same T for all)."""
+        if len(decoders) != len(self.codes):
+            raise ValueError("one learned decoder per code of the set")
+        ids = np.ascontiguousarray(code_ids, dtype=np.uint16)
+        arr = (ctypes.c_void_p * len(decoders))(*[d.handle for d in decoders])
+        _lib.check(self.ctx.handle, _lib.load().gf2l_decode_mixed_nms_device(
+            self.ctx.handle, self._h, arr, ids.ctypes.data, ids.size, d_llr, int(fallback_max_it), d_msg,
+            d_iters, d_conv, d_synd, d_fb))
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             _lib.load().gf2l_codeset_destroy(self._h)
@@ -74,9 +85,12 @@ class CodeSet:
             pass
 
 
-def decode_mixed_arrays(cs: CodeSet, code_ids, llr_packed: np.ndarray, max_iterations: int):
+def decode_mixed_arrays(cs: CodeSet, code_ids, llr_packed: np.ndarray, max_iterations: int,
+                        decoders=None):
     """Host-array form: llr_packed holds each frame's n LLRs back to back.
-    Returns (msg_words packed, msg offsets, iterations, converged, syndrome_ok)."""
+    Returns (msg_words packed, msg offsets, iterations, converged, syndrome_ok,
+    fallback).  With `decoders` (one NmsDecoder per code) the learned decode
+    runs and max_iterations is the fallback's cap."""
     ids = np.ascontiguousarray(code_ids, dtype=np.uint16)
     if ids.size and int(ids.max()) >= len(cs.codes):
         raise ValueError(f"code id {int(ids.max())} outside the code set")
@@ -89,6 +103,7 @@ def decode_mixed_arrays(cs: CodeSet, code_ids, llr_packed: np.ndarray, max_itera
     it = np.zeros(F, dtype=np.uint32)
     conv = np.zeros(F, dtype=np.uint8)
     synd = np.zeros(F, dtype=np.uint8)
+    fb = np.zeros(F, dtype=np.uint8)
     if F:
         ctx = cs.ctx
         d_llr = DeviceBuffer.from_array(ctx, llr)
@@ -96,13 +111,20 @@ def decode_mixed_arrays(cs: CodeSet, code_ids, llr_packed: np.ndarray, max_itera
         d_it = DeviceBuffer(ctx, it.nbytes)
         d_conv = DeviceBuffer(ctx, F)
         d_synd = DeviceBuffer(ctx, F)
-        cs.decode_device(ids, d_llr.ptr, max_iterations, d_msg.ptr, d_it.ptr, d_conv.ptr, d_synd.ptr)
+        d_fb = DeviceBuffer(ctx, F)
+        if decoders is None:
+            cs.decode_device(ids, d_llr.ptr, max_iterations, d_msg.ptr, d_it.ptr, d_conv.ptr, d_synd.ptr)
+        else:
+            cs.decode_nms_device(decoders, ids, d_llr.ptr, max_iterations, d_msg.ptr, d_it.ptr, d_conv.ptr,
+                                 d_synd.ptr, d_fb.ptr)
         ctx.synchronize()
         msg = d_msg.download(np.uint64, msg.shape)
         it = d_it.download(np.uint32, F)
         conv = d_conv.download(np.uint8, F)
         synd = d_synd.download(np.uint8, F)
-    return msg, msg_off, it, conv.astype(bool), synd.astype(bool)
+        if decoders is not None:
+            fb = d_fb.download(np.uint8, F)
+    return msg, msg_off, it, conv.astype(bool), synd.astype(bool), fb.astype(bool)
 
 
 def decode_mixed(cs: CodeSet, code_ids, llr_blocks, max_iterations: int) -> List[DecoderResult]:
@@ -117,6 +139,6 @@ def decode_mixed(cs: CodeSet, code_ids, llr_blocks, max_iterations: int) -> List
             raise ValueError("LLR length must equal n")
     packed = (np.concatenate([np.asarray(b, dtype=np.float32) for b in blocks])
               if blocks else np.zeros(0, dtype=np.float32))
-    msg, msg_off, it, conv, synd = decode_mixed_arrays(cs, ids, packed, max_iterations)
+    msg, msg_off, it, conv, synd, _ = decode_mixed_arrays(cs, ids, packed, max_iterations)
     return [DecoderResult(BitVec(msg[msg_off[f]:msg_off[f + 1]], int(cs.k[ids[f]])), int(it[f]),
                           bool(conv[f]), bool(synd[f])) for f in range(ids.size)]

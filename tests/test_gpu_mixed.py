@@ -13,6 +13,7 @@ import pytest
 from gf2_amd import _lib, codes
 from gf2_amd.ldpc import Context, LdpcCode, QuasiCyclicLdpc, decode_batch_arrays, qc_ira_base_matrix
 from gf2_amd.mixed import CodeSet, decode_mixed, decode_mixed_arrays
+from gf2_amd.nms import NmsDecoder
 from oracle import oracle as orc
 from tests import frames
 
@@ -161,3 +162,51 @@ def test_errors(ctx):
     csr = LdpcCode.from_edges(2, 4, [(0, 0), (0, 1), (1, 2), (1, 3)], ctx)
     with pytest.raises(_lib.Gf2lError, match="quasi-cyclic"):
         CodeSet([csr])
+
+
+@pytest.mark.parametrize("fb_it", [0, 20])
+def test_learned_mixed_matches_oracle(ctx, fb_it):
+    """One learned decoder per code (random weights), frames of four codes at
+    low, middle and high Eb/N0 in one call: every frame equals the oracle's
+    weighted decode followed, when it fails, by plain min-sum from its LLRs."""
+    specs = [codes.n2304_r12()] + [codes.mixed_twelve()[i] for i in (0, 6, 9)]
+    gcodes = [LdpcCode(QuasiCyclicLdpc(s.base_matrix(), s.z), ctx) for s in specs]
+    cs = CodeSet(gcodes)
+    rng = np.random.default_rng(41)
+    T = 4
+    weights = [rng.uniform(0.6, 0.95, size=(T, int((s.base_matrix() >= 0).sum()))).astype(np.float32)
+               for s in specs]
+    decs = [NmsDecoder(g, T, weights=w) for g, w in zip(gcodes, weights)]
+    ids, blocks = build_batch(specs, 2, seed=43, offsets=(0.2, 1.5, 3.5))
+    say("learned mixed", len(ids), "frames, fallback", fb_it)
+    n = np.array([s.n for s in specs])
+    packed = np.concatenate(blocks)
+    msg, msg_off, it, conv, synd, fb = decode_mixed_arrays(cs, ids, packed, fb_it, decoders=decs)
+    for c, spec in enumerate(specs):
+        sel = np.nonzero(ids == c)[0]
+        oc = orc.Code.from_qc(spec.base_matrix(), spec.z)
+        ref = frames.oracle_learned_with_fallback(oc, spec.base_matrix(), spec.z, weights[c],
+                                                  np.stack([blocks[i] for i in sel]), fb_it)
+        for j, f in enumerate(sel):
+            got = (msg[msg_off[f]:msg_off[f + 1]], it[f], conv[f], synd[f], fb[f])
+            exp = (ref[0][j], ref[1][j], ref[2][j], ref[3][j], ref[4][j])
+            assert np.array_equal(got[0], exp[0]) and tuple(got[1:]) == tuple(exp[1:]), \
+                f"code {c} frame {f}: got it {got[1]} conv {got[2]} fb {got[4]}, want {exp[1:]}"
+    assert (conv & ~fb & (it <= T)).any()  # some frames converge in the weighted phase
+    if fb_it:
+        assert fb.any() and (~fb).any()
+    assert n[ids].sum() == packed.size
+
+
+def test_learned_mixed_rejects_mismatched_decoders(ctx):
+    specs = codes.mixed_twelve()[:2]
+    gcodes = [LdpcCode(QuasiCyclicLdpc(s.base_matrix(), s.z), ctx) for s in specs]
+    cs = CodeSet(gcodes)
+    ids, blocks = build_batch(specs, 1, seed=5)
+    packed = np.concatenate(blocks)
+    swapped = [NmsDecoder(gcodes[1], 4, init=0.8), NmsDecoder(gcodes[0], 4, init=0.8)]
+    with pytest.raises(_lib.Gf2lError, match="another code"):
+        decode_mixed_arrays(cs, ids, packed, 10, decoders=swapped)
+    uneven = [NmsDecoder(gcodes[0], 4, init=0.8), NmsDecoder(gcodes[1], 5, init=0.8)]
+    with pytest.raises(_lib.Gf2lError, match="iteration count"):
+        decode_mixed_arrays(cs, ids, packed, 10, decoders=uneven)
