@@ -53,6 +53,16 @@ PROFILES: Dict[Tuple[int, int], List[Tuple[int, int]]] = {
 
 DEFAULT_SEED = 20241
 
+# Eb/N0 (dB) where plain min-sum BP (50 iterations) has FER about 1e-2 on the
+# twelve Z=360 codes of mixed_twelve(): gf2_amd.fer.calibrate, bisection over
+# 8 steps of 2048 device-generated frames each (seed 5) on an MI355X.
+FER_1E2_EBN0: Dict[Tuple[int, int], float] = {
+    (16200, 7200): 2.628, (16200, 9720): 2.362, (16200, 10800): 2.068,
+    (16200, 11880): 2.553, (16200, 12600): 2.895, (16200, 13320): 3.182,
+    (64800, 32400): 1.579, (64800, 38880): 2.212, (64800, 43200): 1.972,
+    (64800, 48600): 2.415, (64800, 51840): 2.779, (64800, 54000): 3.063,
+}
+
 
 @dataclass(frozen=True)
 class QcIraSpec:

@@ -3,7 +3,7 @@
 Batches of Poisson-distributed size (clipped to [256, 8192]) arrive one
 after another; each holds frames of all twelve codes (N in {16200, 64800},
 six rates each), every code at the Eb/N0 where plain min-sum BP has FER
-about 1e-2 (MIXED_EBN0, measured with gf2_amd.fer --target-fer 1e-2).  Frames
+about 1e-2 (codes.FER_1E2_EBN0, measured with gf2_amd.fer --target-fer 1e-2).  Frames
 are generated on the GPU before the timed region.  Each batch is one
 gf2l_decode_mixed_device call (one code set, per-frame early stop, graph-
 replayed iteration loop); the same batches are also decoded one code at a
@@ -18,7 +18,7 @@ import argparse
 import json
 import sys
 import time
-from typing import Dict, List, Tuple
+from typing import List
 
 import numpy as np
 
@@ -26,17 +26,6 @@ from . import _lib, codes
 from .device import DeviceBuffer
 from .ldpc import Context, LdpcCode
 from .mixed import CodeSet
-
-# Eb/N0 (dB) of FER about 1e-2 for plain min-sum, 50 iterations, per (N, K):
-# gf2_amd.fer.calibrate, bisection over 8 steps of 2048 device-generated
-# frames each (seed 5) on an MI355X.
-MIXED_EBN0: Dict[Tuple[int, int], float] = {
-    (16200, 7200): 2.628, (16200, 9720): 2.362, (16200, 10800): 2.068,
-    (16200, 11880): 2.553, (16200, 12600): 2.895, (16200, 13320): 3.182,
-    (64800, 32400): 1.579, (64800, 38880): 2.212, (64800, 43200): 1.972,
-    (64800, 48600): 2.415, (64800, 51840): 2.779, (64800, 54000): 3.063,
-}
-
 
 def poisson_batches(count: int, mean: int, seed: int, lo: int = 256, hi: int = 8192) -> List[int]:
     rng = np.random.default_rng(seed)
@@ -80,7 +69,7 @@ def main(argv=None) -> int:
     cs = CodeSet(gcodes)
     lib = _lib.load()
     sizes = poisson_batches(a.batches, a.mean_batch, a.seed)
-    stream = Stream(ctx, gcodes, sizes, [MIXED_EBN0[(s.n, s.k)] for s in specs], a.seed)
+    stream = Stream(ctx, gcodes, sizes, [codes.FER_1E2_EBN0[(s.n, s.k)] for s in specs], a.seed)
     k = np.array([s.k for s in specs])
     kw = (k + 63) // 64
     Fmax = max(sizes)

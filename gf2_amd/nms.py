@@ -81,7 +81,7 @@ class NmsDecoder:
                 "k": self.code.k(), "shifts": self.code.qc.base_matrix.tolist(),
                 "z": self.code.qc.expansion_factor()}
         info.update(meta or {})
-        np.savez(path, weights=self.weights, adam_m=m, adam_v=v, step=np.array(step, dtype=np.uint64),
+        np.savez_compressed(path, weights=self.weights, adam_m=m, adam_v=v, step=np.array(step, dtype=np.uint64),
                  meta=np.array(json.dumps(info)))
 
     @classmethod

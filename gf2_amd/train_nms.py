@@ -1,7 +1,9 @@
 """Trains the learned normalized min-sum decoder on the GPU and saves a checkpoint.
 
-    python -m gf2_amd.train_nms --code n2304  [--out gf2_amd/weights/nms_n2304_t10.npz]
-    python -m gf2_amd.train_nms --code n64800 [--out gf2_amd/weights/nms_n64800_t10.npz]
+    python -m gf2_amd.train_nms --code n2304   [--out gf2_amd/weights/nms_n2304_t10.npz]
+    python -m gf2_amd.train_nms --code n64800  [--out gf2_amd/weights/nms_n64800_t10.npz]
+    python -m gf2_amd.train_nms --code n16200-k9720   (any of the twelve Z=360 codes)
+    python -m gf2_amd.train_nms --all-mixed   (one checkpoint per Z=360 code)
 
 Recipes (defaults per code):
   n2304   N=2304 K=1152 Z=96 QC-IRA: 1024 steps x 4096 frames = 2^22 frames,
@@ -9,6 +11,9 @@ Recipes (defaults per code):
           BCE on the soft output of every iteration, weights start at 1.0.
   n64800  N=64800 K=32400 Z=360 QC-IRA: 400 steps x 256 frames, Adam lr 1e-2,
           Eb/N0 uniform in [1.0, 2.0] dB, 10 iterations, weights start at 1.0.
+  Z=360 codes of the mixed set: 300 steps x 256 frames (N=64800) or x 1024
+          frames (N=16200), Adam lr 1e-2, Eb/N0 uniform in [e - 1.0, e + 0.5]
+          dB around the code's plain-BP FER-1e-2 point e (codes.FER_1E2_EBN0).
 Frames come from the on-device generator (seeded Philox, staircase encoder,
 BPSK/AWGN); every step draws fresh frames.
 """
@@ -30,38 +35,63 @@ RECIPES = {
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
+def mixed_name(spec: codes.QcIraSpec) -> str:
+    return f"n{spec.n}-k{spec.k}"
+
+
+def recipe(name: str) -> dict:
+    if name in RECIPES:
+        return dict(RECIPES[name])
+    for s in codes.mixed_twelve():
+        if mixed_name(s) == name:
+            e = codes.FER_1E2_EBN0[(s.n, s.k)]
+            return dict(spec=lambda s=s: s, steps=300, batch=256 if s.n == 64800 else 1024, lr=1e-2,
+                        lo=round(e - 1.0, 3), hi=round(e + 0.5, 3))
+    raise ValueError(f"unknown code {name!r}")
+
+
 def default_path(code: str, iterations: int = 10) -> str:
     return os.path.join(HERE, "weights", f"nms_{code}_t{iterations}.npz")
 
 
+def train_one(name: str, iterations: int, ctx: Context, seed: int, out: str = None, **override) -> str:
+    r = recipe(name)
+    r.update({k: v for k, v in override.items() if v is not None})
+    spec = r["spec"]()
+    code = LdpcCode(spec.qc(), ctx)
+    dec = NmsDecoder(code, iterations, init=1.0)
+    t0 = time.perf_counter()
+    losses = train(dec, r["steps"], r["batch"], r["lr"], r["lo"], r["hi"], seed=seed,
+                   log_every=max(1, r["steps"] // 4))
+    dt = time.perf_counter() - t0
+    out = out or default_path(name, iterations)
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    dec.save(out, meta={"code_name": spec.name, "recipe": {k: v for k, v in r.items() if k != "spec"},
+                        "seed": seed, "final_loss": float(losses[-10:].mean()), "train_seconds": dt})
+    print(f"[nms-train] {spec.name}: {r['steps']} steps x {r['batch']} frames in {dt:.1f}s, "
+          f"loss {losses[:10].mean():.4f} -> {losses[-10:].mean():.4f}; saved {out}", flush=True)
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--code", choices=sorted(RECIPES), default="n2304")
+    ap.add_argument("--code", default="n2304")
+    ap.add_argument("--all-mixed", action="store_true", help="train one checkpoint per Z=360 code")
     ap.add_argument("--iterations", type=int, default=10)
     ap.add_argument("--steps", type=int)
     ap.add_argument("--batch", type=int)
     ap.add_argument("--lr", type=float)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--out")
+    ap.add_argument("--out-dir", help="directory for the checkpoints (default gf2_amd/weights)")
     a = ap.parse_args(argv)
-    r = dict(RECIPES[a.code])
-    for k in ("steps", "batch", "lr"):
-        if getattr(a, k) is not None:
-            r[k] = getattr(a, k)
-    spec = r["spec"]()
-    code = LdpcCode(spec.qc(), Context(0))
-    dec = NmsDecoder(code, a.iterations, init=1.0)
-    t0 = time.perf_counter()
-    losses = train(dec, r["steps"], r["batch"], r["lr"], r["lo"], r["hi"], seed=a.seed,
-                   log_every=max(1, r["steps"] // 8))
-    dt = time.perf_counter() - t0
-    out = a.out or default_path(a.code, a.iterations)
-    os.makedirs(os.path.dirname(out), exist_ok=True)
-    dec.save(out, meta={"code_name": spec.name, "recipe": {k: v for k, v in r.items() if k != "spec"},
-                        "seed": a.seed, "final_loss": float(losses[-10:].mean()),
-                        "train_seconds": dt})
-    print(f"[nms-train] {spec.name}: {r['steps']} steps x {r['batch']} frames in {dt:.1f}s, "
-          f"loss {losses[:10].mean():.4f} -> {losses[-10:].mean():.4f}; saved {out}", flush=True)
+    ctx = Context(0)
+    names = [mixed_name(s) for s in codes.mixed_twelve()] if a.all_mixed else [a.code]
+    for name in names:
+        out = a.out if not a.all_mixed else None
+        if a.out_dir:
+            out = os.path.join(a.out_dir, os.path.basename(default_path(name, a.iterations)))
+        train_one(name, a.iterations, ctx, a.seed, out, steps=a.steps, batch=a.batch, lr=a.lr)
     return 0
 
 
