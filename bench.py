@@ -19,7 +19,8 @@ The JSON line carries
                 launch in the timed region, against the 8 TB/s HBM3E peak
   cpu_baseline  the structural C++ restatement of the reference's rayon
                 decode_batch (oracle/ref_port.cpp) on a bounded sample of the
-                same frames, on this host's cores
+                same frames, on this host's cores; beside it the same
+                arithmetic on flat arrays (oracle/gf2_oracle.c)
 
     python bench.py [--steps K] [--warmup W] [--frames F] [--ebn0 X]
 """
@@ -78,9 +79,21 @@ def cpu_baseline(spec, base, llr, max_it, seconds):
         oc.port_decode_batch(llr[done:done + threads], max_it, threads)
         done += threads
     dt = time.perf_counter() - t0
-    return {"value": done * spec.k / dt / 1e9, "unit": "Gbit/s", "cores": threads, "kind": "port",
-            "sample": f"first {done} frames of the same batch, {dt:.1f} s, plain min-sum "
-                      f"{max_it} iterations, oracle/ref_port.cpp with {threads} worker threads"}
+    out = {"value": done * spec.k / dt / 1e9, "unit": "Gbit/s", "cores": threads, "kind": "port",
+           "sample": f"first {done} frames of the same batch, {dt:.1f} s, plain min-sum "
+                     f"{max_it} iterations, oracle/ref_port.cpp with {threads} worker threads"}
+    # the same arithmetic on flat CSR/CSC arrays (the oracle's decoder), so the
+    # GPU/CPU ratio does not rest on the reference's per-frame allocations
+    n2 = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds / 2 and n2 + 4 * threads <= llr.shape[0]:
+        oc.decode_batch(llr[n2:n2 + 4 * threads], max_it, threads=threads)
+        n2 += 4 * threads
+    dt2 = time.perf_counter() - t0
+    if n2:
+        out["flat_array_c"] = {"value": n2 * spec.k / dt2 / 1e9, "cores": threads,
+                               "sample": f"first {n2} frames, {dt2:.1f} s, oracle/gf2_oracle.c decode_batch"}
+    return out
 
 
 def main():
