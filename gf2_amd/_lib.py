@@ -27,6 +27,7 @@ PATH_STATE = 3
 TUNE_WG_PER_CU = 1
 TUNE_XCD_MAP = 2
 TUNE_GRAPH_ITERS = 3
+TUNE_CHUNK = 4
 
 _c_void_pp = ctypes.POINTER(ctypes.c_void_p)
 _u32p = ctypes.POINTER(ctypes.c_uint32)

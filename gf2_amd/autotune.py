@@ -12,6 +12,10 @@ the context tunables of gf2l_set_tuning / gf2l_set_path:
                frame-major over the whole grid
   graph_iters  iterations per graph replay of the mixed-code decoder (host
                round trips vs. empty launches after the last frame retires)
+  chunk        consecutive items per shared-counter fetch (0: fixed-stride
+               assignment): counters keep the items in flight close together
+               (less L2 traffic) at the cost of an atomic and a barrier per
+               chunk
   path         automatic (whole frame in LDS when it fits), compressed check
                state, or one stored message per edge (single-code decoder);
                a path that cannot run the code is skipped
@@ -41,10 +45,11 @@ CANDIDATES: Dict[str, List[int]] = {
     "wg_per_cu": [2, 3, 4, 6],
     "xcd_map": [1, 0],
     "graph_iters": [2, 4, 8],
+    "chunk": [0, 4, 8, 16],
     "path": [_lib.PATH_AUTO, _lib.PATH_STATE, _lib.PATH_STREAM],
 }
 _TUNE_KEYS = {"wg_per_cu": _lib.TUNE_WG_PER_CU, "xcd_map": _lib.TUNE_XCD_MAP,
-              "graph_iters": _lib.TUNE_GRAPH_ITERS}
+              "graph_iters": _lib.TUNE_GRAPH_ITERS, "chunk": _lib.TUNE_CHUNK}
 
 
 def default_cache_path() -> str:
@@ -125,7 +130,7 @@ def _time(fn, repeats: int) -> float:
 
 
 def tune_code(code: LdpcCode, frames: int, ebn0_db: float, max_it: int = 50, repeats: int = 2,
-              knobs: Sequence[str] = ("path", "wg_per_cu", "xcd_map"), cache: Optional[TuningCache] = None,
+              knobs: Sequence[str] = ("path", "wg_per_cu", "chunk"), cache: Optional[TuningCache] = None,
               seed: int = 11) -> dict:
     """Tunes the single-code decoder for batches of about `frames` frames.
     Returns {"config", "ms", "results": [(config, ms), ...]} and applies the
@@ -147,7 +152,7 @@ def tune_code(code: LdpcCode, frames: int, ebn0_db: float, max_it: int = 50, rep
 
 
 def tune_codeset(cs, code_ids: np.ndarray, d_llr: DeviceBuffer, max_it: int = 50, repeats: int = 2,
-                 knobs: Sequence[str] = ("wg_per_cu", "xcd_map", "graph_iters"),
+                 knobs: Sequence[str] = ("wg_per_cu", "chunk", "graph_iters"),
                  cache: Optional[TuningCache] = None) -> dict:
     """Tunes the mixed-code decoder of a code set on a resident batch (d_llr
     packed as gf2l_decode_mixed_device expects)."""

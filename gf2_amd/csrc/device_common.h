@@ -148,6 +148,42 @@ __device__ __forceinline__ void dispatch_degree(uint32_t d, F &&f) {
 }
 #undef GF2L_DEG_CASE
 
+// Item stream of a persistent workgroup.  Items differ in cost (a variable's
+// degree runs from 1 to 13), and with a fixed stride fast and slow
+// workgroups drift apart until the items in flight span many frames and
+// evict each other's check states from L2.  With a counter the workgroup
+// takes chunks of G consecutive items: its first chunk is q*G, each later
+// one the next value of the counter plus Q*G, fetched when the current chunk
+// starts so the atomic's latency hides behind G items.  That keeps the items
+// in flight within about G grid-widths.  ctr == nullptr: fixed stride Q.
+// Counters of different XCDs sit kCtrStride words apart (separate cache
+// lines and channels).  The caller's loop must be workgroup-uniform.
+constexpr uint32_t kCtrStride = 64;
+
+struct ItemStream {
+    uint32_t j, cend, G, QG, Q, nxt, par;
+    uint32_t *ctr;
+    __device__ ItemStream(uint32_t q, uint32_t stride, uint32_t *counter, uint32_t chunk)
+        : j(counter ? q * chunk : q), cend(counter ? q * chunk + chunk : 0), G(chunk), QG(stride * chunk),
+          Q(stride), nxt(0), par(0), ctr(counter) {
+        if (ctr && threadIdx.x == 0) nxt = atomicAdd(ctr, G) + QG;
+    }
+    // s_next: two LDS words (double-buffered: one barrier per chunk).
+    __device__ void advance(uint32_t *s_next) {
+        if (!ctr) {
+            j += Q;
+            return;
+        }
+        if (++j < cend) return;
+        if (threadIdx.x == 0) s_next[par] = nxt;
+        __syncthreads();
+        j = uni(s_next[par]);
+        par ^= 1u;
+        cend = j + G;
+        if (threadIdx.x == 0) nxt = atomicAdd(ctr, G) + QG;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // STATE-path item bodies.  A frame's check state is [mb][zp] records of three
 // u32 (min1, min2, argmin | output sign bits << 5), read and written as one
@@ -231,15 +267,13 @@ __device__ __forceinline__ void vn_state_item(const CodeView &cv, uint32_t bc, c
     const uint32_t dv = uni(ldc(cv.col_ptr + bc + 1)) - c0;
     dispatch_degree<MAXD>(dv, [&](auto DC) {
         constexpr int D = decltype(DC)::value, DA = D > 0 ? D : 1;
-        uint4 ci[DA];
+        uint2 ci[DA];  // {first state record of the block-row, shift << 5 | position}
         float wq[DA];
 #pragma unroll
         for (int q = 0; q < D; ++q) {
-            // two 8-byte scalar loads: a 12-byte load (w unused) has no scalar form
-            const uint2 *cp = reinterpret_cast<const uint2 *>(cv.col_info + c0 + q);
-            const uint2 lo = ldc(cp), hi = ldc(cp + 1);
-            ci[q] = make_uint4(uni(lo.x), uni(lo.y), uni(hi.x), uni(hi.y));
-            if constexpr (NMS) wq[q] = __uint_as_float(uni(__float_as_uint(ldc(w_cur + ci[q].w))));
+            const uint2 r = ldc(cv.col_pack + c0 + q);
+            ci[q] = make_uint2(uni(r.x), uni(r.y));
+            if constexpr (NMS) wq[q] = __uint_as_float(uni(__float_as_uint(ldc(w_cur + ldc(&cv.col_info[c0 + q].w)))));
         }
         for (uint32_t j0 = 0; j0 < cv.z; j0 += blockDim.x) {
             const uint32_t j = j0 + threadIdx.x;
@@ -250,7 +284,7 @@ __device__ __forceinline__ void vn_state_item(const CodeView &cv, uint32_t bc, c
                 uint32_t mt[DA];
 #pragma unroll
                 for (int q = 0; q < D; ++q) {
-                    const uint3 o = ldu(fst, ci[q].x * cv.zp + wrap_sub(j, ci[q].z, cv.z));
+                    const uint3 o = ldu(fst, ci[q].x + wrap_sub(j, ci[q].y >> 5, cv.z));
                     m1[q] = __uint_as_float(o.x);
                     m2[q] = __uint_as_float(o.y);
                     mt[q] = o.z;
@@ -258,7 +292,7 @@ __device__ __forceinline__ void vn_state_item(const CodeView &cv, uint32_t bc, c
                 bel = ldu(fl, j);
 #pragma unroll
                 for (int q = 0; q < D; ++q) {
-                    float c = recon(m1[q], m2[q], mt[q], ci[q].y);
+                    float c = recon(m1[q], m2[q], mt[q], ci[q].y & 31u);
                     if constexpr (NMS) c = wq[q] * c;
                     bel = bel + c;
                 }

@@ -45,7 +45,7 @@ int ensure_workspace(gf2l_ctx *ctx, const gf2l::CodeView &cv, int path, size_t f
     GF2L_HIP(ctx, hipMalloc(&ws.belief, bel * sizeof(float)));
     GF2L_HIP(ctx, hipMalloc(&ws.hb, hb * sizeof(uint64_t)));
     GF2L_HIP(ctx, hipMalloc(&ws.lists, 2 * f * sizeof(uint32_t)));
-    GF2L_HIP(ctx, hipMalloc(&ws.counts, 2 * sizeof(uint32_t)));
+    GF2L_HIP(ctx, hipMalloc(&ws.counts, gf2l::kCountsWords * sizeof(uint32_t)));  // counts + item counters
     GF2L_HIP(ctx, hipHostMalloc((void **)&ws.h_count, sizeof(uint32_t), hipHostMallocDefault));
     ws.cap_frames = f;
     ws.cap_c2v = c2v;
@@ -416,7 +416,7 @@ int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value) {
     if (!ctx) return GF2L_ERR_INVALID;
     switch (key) {
         case GF2L_TUNE_WG_PER_CU:
-            if (value < 1 || value > 32) return fail(ctx, GF2L_ERR_INVALID, "wg_per_cu out of [1,32]");
+            if (value < 0 || value > 32) return fail(ctx, GF2L_ERR_INVALID, "wg_per_cu out of [0,32]");
             ctx->tune.wg_per_cu = value;
             return GF2L_OK;
         case GF2L_TUNE_XCD_MAP:
@@ -425,6 +425,10 @@ int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value) {
         case GF2L_TUNE_GRAPH_ITERS:
             if (value < 1 || value > 64) return fail(ctx, GF2L_ERR_INVALID, "graph_iters out of [1,64]");
             ctx->tune.graph_iters = value;
+            return GF2L_OK;
+        case GF2L_TUNE_CHUNK:
+            if (value < -1 || value > 64) return fail(ctx, GF2L_ERR_INVALID, "chunk out of [-1,64]");
+            ctx->tune.chunk = value;
             return GF2L_OK;
         default:
             return fail(ctx, GF2L_ERR_INVALID, "unknown tuning key");
@@ -441,6 +445,7 @@ int gf2l_code_qc(gf2l_ctx *ctx, uint32_t mb, uint32_t nb, uint32_t z, const int3
     if (nb == 0 || !shifts) return fail(ctx, GF2L_ERR_INVALID, "empty base matrix");
     if ((uint64_t)mb * z > (1ull << 28) || (uint64_t)nb * z > (1ull << 28))
         return fail(ctx, GF2L_ERR_UNSUPPORTED, "code too large (more than 2^28 checks or variables)");
+    if (z > (1u << 26)) return fail(ctx, GF2L_ERR_UNSUPPORTED, "lift above 2^26");
     for (uint32_t br = 0; br < mb; ++br)
         for (uint32_t bc = 0; bc < nb; ++bc) {
             const int32_t s = shifts[(size_t)br * nb + bc];
@@ -480,17 +485,25 @@ int gf2l_code_qc(gf2l_ctx *ctx, uint32_t mb, uint32_t nb, uint32_t z, const int3
         return fail(ctx, GF2L_ERR_UNSUPPORTED, "check degree above 32");
     }
     const uint32_t nbe = (uint32_t)c->h_row_edge.size();
+    const uint32_t zp = ((z + 31) / 32) * 32;
+    std::vector<uint2> col_pack(nbe);  // {first state record of the block-row, shift << 5 | position}
+    for (uint32_t q = 0; q < nbe; ++q) {
+        const uint4 ci = c->h_col_info[q];
+        col_pack[q] = make_uint2(ci.x * zp, (ci.z << 5) | ci.y);
+    }
     const size_t b_rp = (mb + 1) * 4, b_cp = (nb + 1) * 4, b_e = (size_t)std::max(nbe, 1u) * 8,
                  b_ci = (size_t)std::max(nbe, 1u) * 16;
     auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
-    const size_t o_re = al(b_rp), o_cp = al(o_re + b_e), o_ce = al(o_cp + b_cp), o_ci = al(o_ce + b_e);
-    const size_t total = o_ci + b_ci;
+    const size_t o_re = al(b_rp), o_cp = al(o_re + b_e), o_ce = al(o_cp + b_cp), o_ci = al(o_ce + b_e),
+                 o_pk = al(o_ci + b_ci);
+    const size_t total = o_pk + b_e;
     std::vector<char> host(total, 0);
     std::memcpy(host.data(), c->h_row_ptr.data(), b_rp);
     if (nbe) std::memcpy(host.data() + o_re, c->h_row_edge.data(), nbe * 8);
     std::memcpy(host.data() + o_cp, c->h_col_ptr.data(), b_cp);
     if (nbe) std::memcpy(host.data() + o_ce, c->h_col_edge.data(), nbe * 8);
     if (nbe) std::memcpy(host.data() + o_ci, c->h_col_info.data(), nbe * 16);
+    if (nbe) std::memcpy(host.data() + o_pk, col_pack.data(), nbe * 8);
     hipError_t e = hipMalloc(&c->d_tables, total);
     if (e == hipSuccess) e = hipMemcpy(c->d_tables, host.data(), total, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -515,7 +528,8 @@ int gf2l_code_qc(gf2l_ctx *ctx, uint32_t mb, uint32_t nb, uint32_t z, const int3
     v.col_ptr = reinterpret_cast<const uint32_t *>(d + o_cp);
     v.col_edge = reinterpret_cast<const uint2 *>(d + o_ce);
     v.col_info = reinterpret_cast<const uint4 *>(d + o_ci);
-    v.zp = ((z + 31) / 32) * 32;
+    v.col_pack = reinterpret_cast<const uint2 *>(d + o_pk);
+    v.zp = zp;
     *out = c;
     return GF2L_OK;
 }

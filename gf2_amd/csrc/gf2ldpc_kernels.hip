@@ -57,38 +57,68 @@ struct XcdSplit {
 // checks; belief reads of a circulant are a rotated contiguous run, the
 // check's own state is contiguous.  In the first iteration the inputs are
 // the channel LLRs (belief = L, c2v = 0).
-template <int MAXD, bool FIRST, bool NMS>
+// DYN: items come in chunks of `chunk` from per-XCD counters at
+// sched + (kind * 8 + xcd) * kCtrStride (kind 0: check, 1: variable kernel);
+// otherwise a fixed stride.  Each kernel clears the other kind's counters
+// after its items, for the kernel that follows it.
+template <int MAXD, bool FIRST, bool NMS, bool DYN>
 __global__ void __launch_bounds__(512)
 cn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
                 const uint32_t *state_in, uint32_t *state_out, const float *__restrict__ belief,
-                const float *__restrict__ llr, const float *__restrict__ w_prev, int xcd) {
+                const float *__restrict__ llr, const float *__restrict__ w_prev, int xcd,
+                uint32_t *sched, uint32_t chunk) {
     const XcdSplit xs(*count_ptr, cv.mb, xcd);
     const size_t fstride = 3ull * cv.mb * cv.zp;
-    for (uint32_t j = xs.q; j < xs.items; j += xs.Q) {
+    auto item = [&](uint32_t j) {
         const uint32_t f = list[xs.list_index(j, cv.mb)];
         const uint32_t br = j % cv.mb;
         const float *fb = FIRST ? llr + (size_t)f * cv.n : belief + (size_t)f * cv.nb * cv.zp;
         cn_state_item<MAXD, FIRST, NMS>(cv, br, state_in + f * fstride, state_out + f * fstride, fb, w_prev);
+    };
+    if constexpr (DYN) {
+        __shared__ uint32_t s_next[2];
+        ItemStream st(xs.q, xs.Q, sched + xs.x * kCtrStride, chunk);
+        while (st.j < xs.items) {
+            item(st.j);
+            st.advance(s_next);
+        }
+        // the variable kernel that follows draws its items from these
+        if (blockIdx.x == 0 && threadIdx.x < 8) sched[(8 + threadIdx.x) * kCtrStride] = 0;
+    } else {
+        for (uint32_t j = xs.q; j < xs.items; j += xs.Q) item(j);
     }
 }
 
 // Variable update of one block-column of one frame per item.
-template <int MAXD, bool NMS>
+template <int MAXD, bool NMS, bool DYN>
 __global__ void __launch_bounds__(512)
 vn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
                 const uint32_t *__restrict__ state, float *__restrict__ belief,
                 const float *__restrict__ llr, uint64_t *__restrict__ hb,
-                uint32_t *__restrict__ next_count, const float *__restrict__ w_cur, int xcd) {
-    if (next_count && blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
+                uint32_t *__restrict__ next_count, const float *__restrict__ w_cur, int xcd,
+                uint32_t *sched, uint32_t chunk) {
     const XcdSplit xs(*count_ptr, cv.nb, xcd);
     const size_t fstride = 3ull * cv.mb * cv.zp;
-    for (uint32_t jj = xs.q; jj < xs.items; jj += xs.Q) {
+    auto item = [&](uint32_t jj) {
         const uint32_t f = list[xs.list_index(jj, cv.nb)];
         const uint32_t bc = jj % cv.nb;
         vn_state_item<MAXD, NMS>(cv, bc, state + f * fstride, llr + (size_t)f * cv.n + (size_t)bc * cv.z,
                                  belief + ((size_t)f * cv.nb + bc) * cv.zp,
                                  hb ? hb + ((size_t)f * cv.nb + bc) * cv.zw : nullptr, w_cur);
+    };
+    if constexpr (DYN) {
+        __shared__ uint32_t s_next[2];
+        ItemStream st(xs.q, xs.Q, sched + (8 + xs.x) * kCtrStride, chunk);
+        while (st.j < xs.items) {
+            item(st.j);
+            st.advance(s_next);
+        }
+        if (blockIdx.x == 0 && threadIdx.x < 8) sched[threadIdx.x * kCtrStride] = 0;
+    } else {
+        for (uint32_t jj = xs.q; jj < xs.items; jj += xs.Q) item(jj);
     }
+    // stored last: the running count the syndrome pass appends to
+    if (next_count && blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -215,6 +245,8 @@ syn_words_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t 
     }
 }
 
+// counts: [0..1] running-list counts; from kSchedOffset the check-kernel
+// (8) and variable-kernel (8) item counters, one per XCD, kCtrStride apart.
 __global__ void hbm_init_kernel(uint32_t *list, uint32_t *counts, uint32_t frames) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < frames) list[t] = t;
@@ -222,6 +254,7 @@ __global__ void hbm_init_kernel(uint32_t *list, uint32_t *counts, uint32_t frame
         counts[0] = frames;
         counts[1] = 0;
     }
+    if (t < 16) counts[kSchedOffset + t * kCtrStride] = 0;
 }
 
 // Trace support for the STATE path: explicit messages and unpadded beliefs.
@@ -498,9 +531,17 @@ hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, con
     uint32_t *next_list = ws.lists + (size_t)nxt * ws.cap_frames;
     const uint32_t *count = ws.counts + cur;
     uint32_t *next_count = ws.counts + nxt;
+    // items from shared counters (tune.chunk per fetch), or a fixed stride;
+    // automatic (tune.chunk < 0): counters with 8 items per fetch when the
+    // code's variable degrees fit the smallest class (state path only)
+    const int ch = path != kState ? 0 : tune.chunk >= 0 ? tune.chunk : (cv.max_dv <= 8 ? 8 : 0);
+    const bool dyn = ch > 0;
+    uint32_t *sched = ws.counts + kSchedOffset;
+    const uint32_t chunk = dyn ? (uint32_t)ch : 1u;
+    const int wg = tune.wg_per_cu ? tune.wg_per_cu : (dyn ? 4 : 3);
     const uint32_t block = round_block(cv.z);
-    const uint32_t g_cn = hbm_grid(a.frames * cv.mb, num_cus, tune.wg_per_cu);
-    const uint32_t g_vn = hbm_grid(a.frames * cv.nb, num_cus, tune.wg_per_cu);
+    const uint32_t g_cn = hbm_grid(a.frames * cv.mb, num_cus, wg);
+    const uint32_t g_vn = hbm_grid(a.frames * cv.nb, num_cus, wg);
     const int xcd = tune.xcd_map;
     const uint32_t g_syn = a.frames < (uint32_t)num_cus * 8u ? a.frames : (uint32_t)num_cus * 8u;
     const int first = a.it == 1;
@@ -509,9 +550,11 @@ hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, con
     const bool nms_prev = a.w_prev != nullptr, nms_cur = a.w_cur != nullptr;
     const bool wide_c = cv.max_dc > 16, wide_v = cv.max_dv > 16;
     if (path == kState) {
-#define GF2L_CN(MD, FI, NM)                                                                    \
-    hipLaunchKernelGGL((cn_state_kernel<MD, FI, NM>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
-                       ws.state, ws.state, ws.belief, a.llr, a.w_prev, xcd)
+#define GF2L_CN_DYN(MD, FI, NM, DY)                                                                \
+    hipLaunchKernelGGL((cn_state_kernel<MD, FI, NM, DY>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
+                       ws.state, ws.state, ws.belief, a.llr, a.w_prev, xcd, sched, chunk)
+#define GF2L_CN(MD, FI, NM) \
+    do { if (dyn) GF2L_CN_DYN(MD, FI, NM, true); else GF2L_CN_DYN(MD, FI, NM, false); } while (0)
         if (first) {
             GF2L_STATE_CLASS(cv.max_dc, GF2L_CN, true, false);
         } else if (nms_prev) {
@@ -520,16 +563,20 @@ hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, con
             GF2L_STATE_CLASS(cv.max_dc, GF2L_CN, false, false);
         }
 #undef GF2L_CN
+#undef GF2L_CN_DYN
         if (ev) hipEventRecord(ev[1], s);
-#define GF2L_VN(MD, NM, UNUSED)                                                                \
-    hipLaunchKernelGGL((vn_state_kernel<MD, NM>), dim3(g_vn), dim3(block), 0, s, cv, list, count, \
-                       ws.state, ws.belief, a.llr, ws.hb, next_count, a.w_cur, xcd)
+#define GF2L_VN_DYN(MD, NM, DY)                                                                    \
+    hipLaunchKernelGGL((vn_state_kernel<MD, NM, DY>), dim3(g_vn), dim3(block), 0, s, cv, list, count, \
+                       ws.state, ws.belief, a.llr, ws.hb, next_count, a.w_cur, xcd, sched, chunk)
+#define GF2L_VN(MD, NM, UNUSED) \
+    do { if (dyn) GF2L_VN_DYN(MD, NM, true); else GF2L_VN_DYN(MD, NM, false); } while (0)
         if (nms_cur) {
             GF2L_VAR_CLASS(cv.max_dv, GF2L_VN, true, 0);
         } else {
             GF2L_VAR_CLASS(cv.max_dv, GF2L_VN, false, 0);
         }
 #undef GF2L_VN
+#undef GF2L_VN_DYN
     } else {
 #define GF2L_CN(MD, NM)                                                                        \
     hipLaunchKernelGGL((cn_stream_kernel<MD, NM>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
@@ -563,13 +610,14 @@ hipError_t launch_state_forward(const CodeView &cv, const uint32_t *list, const 
                                 const float *llr, const float *w_prev, const float *w_cur,
                                 int num_cus, const Tuning &tune, hipStream_t s) {
     const uint32_t block = round_block(cv.z);
-    const uint32_t g_cn = hbm_grid(frames * cv.mb, num_cus, tune.wg_per_cu);
-    const uint32_t g_vn = hbm_grid(frames * cv.nb, num_cus, tune.wg_per_cu);
+    const int wg = tune.wg_per_cu ? tune.wg_per_cu : 3;
+    const uint32_t g_cn = hbm_grid(frames * cv.mb, num_cus, wg);
+    const uint32_t g_vn = hbm_grid(frames * cv.nb, num_cus, wg);
     const int xcd = tune.xcd_map;
     const bool nms_prev = w_prev != nullptr, nms_cur = w_cur != nullptr;
 #define GF2L_CN(MD, FI, NM)                                                                    \
-    hipLaunchKernelGGL((cn_state_kernel<MD, FI, NM>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
-                       state_in, state_out, belief_in, llr, w_prev, xcd)
+    hipLaunchKernelGGL((cn_state_kernel<MD, FI, NM, false>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
+                       state_in, state_out, belief_in, llr, w_prev, xcd, nullptr, 1u)
     if (first) {
         GF2L_STATE_CLASS(cv.max_dc, GF2L_CN, true, false);
     } else if (nms_prev) {
@@ -579,8 +627,8 @@ hipError_t launch_state_forward(const CodeView &cv, const uint32_t *list, const 
     }
 #undef GF2L_CN
 #define GF2L_VN(MD, NM, UNUSED)                                                                \
-    hipLaunchKernelGGL((vn_state_kernel<MD, NM>), dim3(g_vn), dim3(block), 0, s, cv, list, count, \
-                       state_out, belief_out, llr, nullptr, nullptr, w_cur, xcd)
+    hipLaunchKernelGGL((vn_state_kernel<MD, NM, false>), dim3(g_vn), dim3(block), 0, s, cv, list, count, \
+                       state_out, belief_out, llr, nullptr, nullptr, w_cur, xcd, nullptr, 1u)
     if (nms_cur) {
         GF2L_VAR_CLASS(cv.max_dv, GF2L_VN, true, 0);
     } else {

@@ -35,6 +35,7 @@ struct CodeView {
     const uint32_t *col_ptr;     // nb+1, into col-order list
     const uint2 *col_edge;       // nbe: {row-order edge id, shift}
     const uint4 *col_info;       // nbe: {block-row, position in row, shift, row-order edge id}
+    const uint2 *col_pack;       // nbe: {block-row * zp, shift << 5 | position} (z < 2^27)
 };
 
 // Device view of a general sparse code (LdpcCode::from_edges): CSR rows with
@@ -60,7 +61,7 @@ struct Workspace {
     float *belief = nullptr;     // explicit: [frames][n]; state: [frames][nb][zp]
     uint64_t *hb = nullptr;      // [frames][nb][zw] hard decisions
     uint32_t *lists = nullptr;   // [2][frames] running frame lists
-    uint32_t *counts = nullptr;  // [2]
+    uint32_t *counts = nullptr;  // [0..1] running counts; item counters from kSchedOffset
     uint32_t *h_count = nullptr; // pinned host mirror of one count
 };
 
@@ -77,10 +78,16 @@ struct Profile {
 // Launch-shape tunables of the HBM paths (set by gf2l_set_tuning or the
 // autotuner).
 struct Tuning {
-    int wg_per_cu = 3;   // persistent workgroups per compute unit
+    int wg_per_cu = 0;   // persistent workgroups per compute unit (0: 4 with counter-fed items, else 3)
     int xcd_map = 1;     // 1: frames pinned to XCDs (L2 reuse), 0: frame-major over the grid
     int graph_iters = 4; // iterations per graph replay of the mixed-code decoder
+    int chunk = -1;      // items per counter fetch of the check-state kernels (0: fixed stride, -1: automatic)
 };
+
+// Word offset of the item counters in Workspace::counts and its size (two
+// kernels x 8 XCDs, 64 words apart).
+constexpr uint32_t kSchedOffset = 64;
+constexpr uint32_t kCountsWords = kSchedOffset + 16 * 64;
 
 // Arguments of one iteration of an HBM path.
 struct IterArgs {

@@ -40,6 +40,12 @@
 namespace gf2l {
 
 constexpr uint32_t kMaxCodes = 16;
+// misc[kSchedBase + (8 * slot + xcd) * kCtrStride]: item counters of the
+// check (slots 0-2) and variable (slots 3-5) launches of one iteration, one
+// per degree class.
+constexpr uint32_t kSchedBase = 64;
+constexpr uint32_t kSchedCounters = 6 * 8;
+constexpr uint32_t kMiscWords = kSchedBase + kSchedCounters * 64;
 
 // Workspace segment of one code for one call (element offsets).
 struct CodeSeg {
@@ -76,7 +82,8 @@ struct MixShared {
     uint32_t *lists;   // [2][list_cap] slots of running frames, code c from list_off[c]
     uint32_t list_cap;
     uint32_t *counts;  // [2][kMaxCodes] running frames per code
-    uint32_t *misc;    // [0] list parity, [1] finished syndrome workgroups, [2] iterations done
+    uint32_t *misc;    // [0] list parity, [1] finished syndrome workgroups, [2] iterations done,
+                       // item counters from kSchedBase (kMiscWords in all)
     float *llr;
     uint32_t *state;
     float *belief;
@@ -99,7 +106,7 @@ struct gf2l_codeset {
     gf2l::MixShared m{};
     // captured iteration graph and the launch shape it was captured with
     hipGraphExec_t exec = nullptr;
-    uint32_t g_grid = 0, g_unroll = 0;
+    uint32_t g_grid = 0, g_unroll = 0, g_chunk = 0;
     int g_xcd = -1;
 };
 
@@ -128,6 +135,7 @@ __device__ __forceinline__ CodeView uniform_view(const CodeView *g) {
     v.col_ptr = ldc(&g->col_ptr);
     v.col_edge = ldc(&g->col_edge);
     v.col_info = ldc(&g->col_info);
+    v.col_pack = ldc(&g->col_pack);
     return v;
 }
 __device__ __forceinline__ CodeSeg uniform_seg(const CodeSeg *g) {
@@ -200,63 +208,103 @@ __device__ uint32_t build_prefix(const MixShared &m, uint32_t mask, const uint32
 // class: the kernel's registers are sized for MAXD).  FIRST: inputs are the
 // channel LLRs, no previous messages.  NMS: the previous messages were
 // weighted by iteration t-1's weights (t = this iteration, 1-based).
-template <int MAXD, bool FIRST, bool NMS>
-__global__ void __launch_bounds__(512) cn_mix_kernel(MixShared m, uint32_t mask, int xcd, uint32_t t) {
+template <int MAXD, bool FIRST, bool NMS, bool DYN>
+__global__ void __launch_bounds__(512) cn_mix_kernel(MixShared m, uint32_t mask, int xcd, uint32_t t, uint32_t slot,
+                                                     uint32_t chunk) {
     __shared__ uint32_t s_start[kMaxCodes + 1];
     const uint32_t cur = ldc(m.misc);
     const uint32_t *cnt = m.counts + cur * kMaxCodes;
     const uint32_t *list = m.lists + (size_t)cur * m.list_cap;
     const MixSplit sp(xcd);
     const uint32_t items = build_prefix<kRows>(m, mask, cnt, sp, s_start);
-    (void)items;
-    for (uint32_t c = 0; c < m.ncodes; ++c) {
-        const uint32_t start = uni(s_start[c]), end = uni(s_start[c + 1]);
-        if (start == end) continue;
-        const CodeView cv = uniform_view(m.codes + c);
-        const CodeSeg seg = uniform_seg(m.args->seg + c);
+    auto item = [&](const CodeView &cv, const CodeSeg &seg, const float *w_prev, uint32_t local) {
+        const uint32_t br = local % cv.mb;
+        const uint32_t slot_f = ldc(list + seg.list_off + sp.x + sp.step * (local / cv.mb));
+        const uint64_t r = slot_f - seg.list_off;
+        uint32_t *fs = m.state + seg.st + r * (3ull * cv.mb * cv.zp);
+        if constexpr (FIRST)
+            cn_state_item<MAXD, true, false>(cv, br, fs, fs, m.llr + seg.llr + r * cv.n, nullptr);
+        else
+            cn_state_item<MAXD, false, NMS>(cv, br, fs, fs, m.belief + seg.bel + r * ((uint64_t)cv.nb * cv.zp),
+                                            w_prev);
+    };
+    if constexpr (DYN) {
+        __shared__ uint32_t s_next[2];
+        ItemStream it(sp.q, sp.Q, m.misc + kSchedBase + (slot * 8 + sp.x) * kCtrStride, chunk);
+        uint32_t c = 0, cc = ~0u;
+        CodeView cv{};
+        CodeSeg seg{};
         const float *w_prev = nullptr;
-        if constexpr (NMS) w_prev = ldc(m.args->w + c) + (size_t)(t - 2) * cv.nbe;
-        for (uint32_t j = first_item(start, sp.q, sp.Q); j < end; j += sp.Q) {
-            const uint32_t local = j - start;
-            const uint32_t br = local % cv.mb;
-            const uint32_t slot = ldc(list + seg.list_off + sp.x + sp.step * (local / cv.mb));
-            const uint64_t r = slot - seg.list_off;
-            uint32_t *fs = m.state + seg.st + r * (3ull * cv.mb * cv.zp);
-            if constexpr (FIRST)
-                cn_state_item<MAXD, true, false>(cv, br, fs, fs, m.llr + seg.llr + r * cv.n, nullptr);
-            else
-                cn_state_item<MAXD, false, NMS>(cv, br, fs, fs,
-                                                m.belief + seg.bel + r * ((uint64_t)cv.nb * cv.zp), w_prev);
+        while (it.j < items) {
+            while (it.j >= uni(s_start[c + 1])) ++c;
+            if (c != cc) {
+                cv = uniform_view(m.codes + c);
+                seg = uniform_seg(m.args->seg + c);
+                if constexpr (NMS) w_prev = ldc(m.args->w + c) + (size_t)(t - 2) * cv.nbe;
+                cc = c;
+            }
+            item(cv, seg, w_prev, it.j - uni(s_start[c]));
+            it.advance(s_next);
+        }
+    } else {
+        for (uint32_t c = 0; c < m.ncodes; ++c) {
+            const uint32_t start = uni(s_start[c]), end = uni(s_start[c + 1]);
+            if (start == end) continue;
+            const CodeView cv = uniform_view(m.codes + c);
+            const CodeSeg seg = uniform_seg(m.args->seg + c);
+            const float *w_prev = nullptr;
+            if constexpr (NMS) w_prev = ldc(m.args->w + c) + (size_t)(t - 2) * cv.nbe;
+            for (uint32_t j = first_item(start, sp.q, sp.Q); j < end; j += sp.Q) item(cv, seg, w_prev, j - start);
         }
     }
 }
 
 // Variable update; NMS: messages weighted by iteration t's weights.
-template <int MAXD, bool NMS>
-__global__ void __launch_bounds__(512) vn_mix_kernel(MixShared m, uint32_t mask, int xcd, uint32_t t) {
+template <int MAXD, bool NMS, bool DYN>
+__global__ void __launch_bounds__(512) vn_mix_kernel(MixShared m, uint32_t mask, int xcd, uint32_t t, uint32_t slot,
+                                                     uint32_t chunk) {
     __shared__ uint32_t s_start[kMaxCodes + 1];
     const uint32_t cur = ldc(m.misc);
     const uint32_t *cnt = m.counts + cur * kMaxCodes;
     const uint32_t *list = m.lists + (size_t)cur * m.list_cap;
     const MixSplit sp(xcd);
     const uint32_t items = build_prefix<kCols>(m, mask, cnt, sp, s_start);
-    (void)items;
-    for (uint32_t c = 0; c < m.ncodes; ++c) {
-        const uint32_t start = uni(s_start[c]), end = uni(s_start[c + 1]);
-        if (start == end) continue;
-        const CodeView cv = uniform_view(m.codes + c);
-        const CodeSeg seg = uniform_seg(m.args->seg + c);
+    auto item = [&](const CodeView &cv, const CodeSeg &seg, const float *w_cur, uint32_t local) {
+        const uint32_t bc = local % cv.nb;
+        const uint32_t slot_f = ldc(list + seg.list_off + sp.x + sp.step * (local / cv.nb));
+        const uint64_t r = slot_f - seg.list_off;
+        vn_state_item<MAXD, NMS>(cv, bc, m.state + seg.st + r * (3ull * cv.mb * cv.zp),
+                                 m.llr + seg.llr + r * cv.n + (size_t)bc * cv.z,
+                                 m.belief + seg.bel + r * ((uint64_t)cv.nb * cv.zp) + (size_t)bc * cv.zp,
+                                 m.hb + seg.hb + r * ((uint64_t)cv.nb * cv.zw) + (size_t)bc * cv.zw, w_cur);
+    };
+    if constexpr (DYN) {
+        __shared__ uint32_t s_next[2];
+        ItemStream it(sp.q, sp.Q, m.misc + kSchedBase + (slot * 8 + sp.x) * kCtrStride, chunk);
+        uint32_t c = 0, cc = ~0u;
+        CodeView cv{};
+        CodeSeg seg{};
         const float *w_cur = nullptr;
-        if constexpr (NMS) w_cur = ldc(m.args->w + c) + (size_t)(t - 1) * cv.nbe;
-        for (uint32_t j = first_item(start, sp.q, sp.Q); j < end; j += sp.Q) {
-            const uint32_t local = j - start;
-            const uint32_t bc = local % cv.nb;
-            const uint32_t slot = ldc(list + seg.list_off + sp.x + sp.step * (local / cv.nb));
-            const uint64_t r = slot - seg.list_off;
-            vn_state_item<MAXD, NMS>(cv, bc, m.state + seg.st + r * (3ull * cv.mb * cv.zp),
-                                     m.llr + seg.llr + r * cv.n + (size_t)bc * cv.z,
-                                     m.belief + seg.bel + r * ((uint64_t)cv.nb * cv.zp) + (size_t)bc * cv.zp,
-                                     m.hb + seg.hb + r * ((uint64_t)cv.nb * cv.zw) + (size_t)bc * cv.zw, w_cur);
+        while (it.j < items) {
+            while (it.j >= uni(s_start[c + 1])) ++c;
+            if (c != cc) {
+                cv = uniform_view(m.codes + c);
+                seg = uniform_seg(m.args->seg + c);
+                if constexpr (NMS) w_cur = ldc(m.args->w + c) + (size_t)(t - 1) * cv.nbe;
+                cc = c;
+            }
+            item(cv, seg, w_cur, it.j - uni(s_start[c]));
+            it.advance(s_next);
+        }
+    } else {
+        for (uint32_t c = 0; c < m.ncodes; ++c) {
+            const uint32_t start = uni(s_start[c]), end = uni(s_start[c + 1]);
+            if (start == end) continue;
+            const CodeView cv = uniform_view(m.codes + c);
+            const CodeSeg seg = uniform_seg(m.args->seg + c);
+            const float *w_cur = nullptr;
+            if constexpr (NMS) w_cur = ldc(m.args->w + c) + (size_t)(t - 1) * cv.nbe;
+            for (uint32_t j = first_item(start, sp.q, sp.Q); j < end; j += sp.Q) item(cv, seg, w_cur, j - start);
         }
     }
     // The syndrome pass of this iteration appends to the other list: every
@@ -317,6 +365,7 @@ __global__ void __launch_bounds__(64) syn_mix_kernel(MixShared m) {
             m.misc[1] = 0;
             m.misc[2] = it;
             m.misc[0] = cur ^ 1u;
+            for (uint32_t k = 0; k < kSchedCounters; ++k) m.misc[kSchedBase + k * kCtrStride] = 0;
         }
     }
 }
@@ -330,11 +379,8 @@ __global__ void mix_init_kernel(MixShared m) {
         m.counts[threadIdx.x] = threadIdx.x < m.ncodes ? m.args->seg[threadIdx.x].frames : 0u;
         m.counts[kMaxCodes + threadIdx.x] = 0;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        m.misc[0] = 0;
-        m.misc[1] = 0;
-        m.misc[2] = 0;
-    }
+    if (blockIdx.x == 0 && threadIdx.x < 3) m.misc[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kSchedCounters) m.misc[kSchedBase + threadIdx.x * kCtrStride] = 0;
 }
 
 // Copies each slot's LLRs from the caller's packed array into its code's
@@ -391,7 +437,7 @@ int ensure_work(gf2l_ctx *ctx, gf2l_codeset *cs, uint64_t frames, uint64_t llr, 
                    cbe = grow(bel, cs->cap_bel), chb = grow(hb, cs->cap_hb);
     free_work(cs);
     const Upload up(cf);
-    const uint64_t b_lists = align256(2 * cf * 4), b_cnt = align256(2 * kMaxCodes * 4), b_misc = 256,
+    const uint64_t b_lists = align256(2 * cf * 4), b_cnt = align256(2 * kMaxCodes * 4), b_misc = align256(kMiscWords * 4),
                    b_llr = align256(cl * 4), b_st = align256(cst * 4), b_bel = align256(cbe * 4),
                    b_hb = align256(chb * 8);
     const uint64_t total = up.bytes + b_lists + b_cnt + b_misc + b_llr + b_st + b_bel + b_hb;
@@ -426,19 +472,45 @@ int ensure_work(gf2l_ctx *ctx, gf2l_codeset *cs, uint64_t frames, uint64_t llr, 
 }
 
 struct Shape {
-    uint32_t grid, block, g_syn;
+    uint32_t grid, block, g_syn, chunk;
     size_t sh;
     int xcd;
 };
 
 Shape launch_shape(const gf2l_ctx *ctx, const gf2l_codeset *cs) {
     Shape s;
-    s.grid = ((uint32_t)ctx->num_cus * (uint32_t)ctx->tune.wg_per_cu + 7u) & ~7u;
     s.block = std::min<uint32_t>(512, ((cs->max_z + 63) / 64) * 64);
     s.g_syn = (uint32_t)ctx->num_cus * 8u;
     s.sh = (size_t)cs->max_words * sizeof(uint64_t);
     s.xcd = ctx->tune.xcd_map;
+    // automatic: counter fetch (8 items) when every code's variable degrees fit
+    // the smallest class, where the lower L2 traffic pays for it
+    s.chunk = ctx->tune.chunk >= 0 ? (uint32_t)ctx->tune.chunk : (cs->max_dv <= 8 ? 8u : 0u);
+    const uint32_t wg = ctx->tune.wg_per_cu ? (uint32_t)ctx->tune.wg_per_cu : (s.chunk ? 4u : 3u);
+    s.grid = ((uint32_t)ctx->num_cus * wg + 7u) & ~7u;
     return s;
+}
+
+// One class launch: counter-fed items (sh.chunk > 0) or a fixed stride.
+template <int MAXD, bool FIRST, bool NMS>
+void launch_cn(const MixShared &m, uint32_t mask, const Shape &sh, hipStream_t s, uint32_t t, uint32_t slot) {
+    if (!mask) return;
+    if (sh.chunk)
+        hipLaunchKernelGGL((cn_mix_kernel<MAXD, FIRST, NMS, true>), dim3(sh.grid), dim3(sh.block), 0, s, m, mask,
+                           sh.xcd, t, slot, sh.chunk);
+    else
+        hipLaunchKernelGGL((cn_mix_kernel<MAXD, FIRST, NMS, false>), dim3(sh.grid), dim3(sh.block), 0, s, m, mask,
+                           sh.xcd, t, slot, 1u);
+}
+template <int MAXD, bool NMS>
+void launch_vn(const MixShared &m, uint32_t mask, const Shape &sh, hipStream_t s, uint32_t t, uint32_t slot) {
+    if (!mask) return;
+    if (sh.chunk)
+        hipLaunchKernelGGL((vn_mix_kernel<MAXD, NMS, true>), dim3(sh.grid), dim3(sh.block), 0, s, m, mask, sh.xcd,
+                           t, slot, sh.chunk);
+    else
+        hipLaunchKernelGGL((vn_mix_kernel<MAXD, NMS, false>), dim3(sh.grid), dim3(sh.block), 0, s, m, mask, sh.xcd,
+                           t, slot, 1u);
 }
 
 // Enqueues iteration t: check and variable launches per degree class, then
@@ -446,21 +518,22 @@ Shape launch_shape(const gf2l_ctx *ctx, const gf2l_codeset *cs) {
 template <bool FIRST, bool NMS>
 void enqueue_iteration(const gf2l_codeset *cs, const Shape &sh, hipStream_t s, uint32_t t) {
     const MixShared m = cs->m;
-    const dim3 g(sh.grid), b(sh.block);
     constexpr bool CN_NMS = NMS && !FIRST;
-    if (cs->cn_mask[0]) hipLaunchKernelGGL((cn_mix_kernel<8, FIRST, CN_NMS>), g, b, 0, s, m, cs->cn_mask[0], sh.xcd, t);
-    if (cs->cn_mask[1]) hipLaunchKernelGGL((cn_mix_kernel<16, FIRST, CN_NMS>), g, b, 0, s, m, cs->cn_mask[1], sh.xcd, t);
-    if (cs->cn_mask[2]) hipLaunchKernelGGL((cn_mix_kernel<27, FIRST, CN_NMS>), g, b, 0, s, m, cs->cn_mask[2], sh.xcd, t);
-    if (cs->vn_mask[0]) hipLaunchKernelGGL((vn_mix_kernel<8, NMS>), g, b, 0, s, m, cs->vn_mask[0], sh.xcd, t);
-    if (cs->vn_mask[1]) hipLaunchKernelGGL((vn_mix_kernel<16, NMS>), g, b, 0, s, m, cs->vn_mask[1], sh.xcd, t);
-    if (cs->vn_mask[2]) hipLaunchKernelGGL((vn_mix_kernel<32, NMS>), g, b, 0, s, m, cs->vn_mask[2], sh.xcd, t);
+    launch_cn<8, FIRST, CN_NMS>(m, cs->cn_mask[0], sh, s, t, 0u);
+    launch_cn<16, FIRST, CN_NMS>(m, cs->cn_mask[1], sh, s, t, 1u);
+    launch_cn<27, FIRST, CN_NMS>(m, cs->cn_mask[2], sh, s, t, 2u);
+    launch_vn<8, NMS>(m, cs->vn_mask[0], sh, s, t, 3u);
+    launch_vn<16, NMS>(m, cs->vn_mask[1], sh, s, t, 4u);
+    launch_vn<32, NMS>(m, cs->vn_mask[2], sh, s, t, 5u);
     hipLaunchKernelGGL(syn_mix_kernel, dim3(sh.g_syn), dim3(64), sh.sh, s, m);
 }
 
 // Captures `unroll` non-first iterations as a graph over the code set's
 // current workspace.
 int capture(gf2l_ctx *ctx, gf2l_codeset *cs, const Shape &sh, uint32_t unroll) {
-    if (cs->exec && cs->g_grid == sh.grid && cs->g_unroll == unroll && cs->g_xcd == sh.xcd) return GF2L_OK;
+    if (cs->exec && cs->g_grid == sh.grid && cs->g_unroll == unroll && cs->g_xcd == sh.xcd &&
+        cs->g_chunk == sh.chunk)
+        return GF2L_OK;
     if (cs->exec) hipGraphExecDestroy(cs->exec);
     cs->exec = nullptr;
     hipStream_t s = ctx->stream;
@@ -483,6 +556,7 @@ int capture(gf2l_ctx *ctx, gf2l_codeset *cs, const Shape &sh, uint32_t unroll) {
     cs->g_grid = sh.grid;
     cs->g_unroll = unroll;
     cs->g_xcd = sh.xcd;
+    cs->g_chunk = sh.chunk;
     return GF2L_OK;
 }
 

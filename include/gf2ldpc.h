@@ -86,16 +86,22 @@ int gf2l_profile_read(gf2l_ctx *ctx, double *ms, uint64_t *launches, uint64_t *f
 int gf2l_set_path(gf2l_ctx *ctx, int path);
 
 /* Launch-shape tunables of the HBM paths (results never depend on them).
- *   GF2L_TUNE_WG_PER_CU  persistent workgroups per compute unit (1..32, default 3)
+ *   GF2L_TUNE_WG_PER_CU  persistent workgroups per compute unit (1..32); 0 (default):
+ *                        4 with counter-fed items (GF2L_TUNE_CHUNK), else 3
  *   GF2L_TUNE_XCD_MAP    1 (default): each frame's block-rows / block-columns
  *                        run on one XCD so its beliefs and check states are
  *                        re-read from that XCD's L2; 0: frame-major over the grid
  *   GF2L_TUNE_GRAPH_ITERS iterations captured per graph replay of the mixed-code
  *                        decoder (1..64, default 4): the host checks for running
- *                        frames once per replay */
+ *                        frames once per replay
+ *   GF2L_TUNE_CHUNK      consecutive items a workgroup of the check-state
+ *                        kernels takes per shared-counter fetch (1..64);
+ *                        0: fixed-stride item assignment; -1 (default):
+ *                        8 when every variable degree is at most 8, else 0 */
 #define GF2L_TUNE_WG_PER_CU 1
 #define GF2L_TUNE_XCD_MAP 2
 #define GF2L_TUNE_GRAPH_ITERS 3
+#define GF2L_TUNE_CHUNK 4
 int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value);
 
 /* ---- codes ------------------------------------------------------------ */

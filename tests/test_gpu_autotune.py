@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 def ctx():
     c = Context(0)
     yield c
-    autotune.apply(c, {"path": _lib.PATH_AUTO, "wg_per_cu": 3, "xcd_map": 1, "graph_iters": 4})
+    autotune.apply(c, {"path": _lib.PATH_AUTO, "wg_per_cu": 3, "xcd_map": 1, "graph_iters": 4, "chunk": -1})
 
 
 def test_tune_single_code(ctx, tmp_path):

@@ -311,3 +311,22 @@ def test_mixed_rate_codes_match_oracle(ctx):
         gpu = decode_batch_arrays(code, llr, 30)
         ref = orc.Code.from_qc(base, spec.z).decode_batch(llr, 30, threads=8)
         assert_same(gpu, ref, spec.name)
+
+
+@pytest.mark.parametrize("chunk", [1, 3, 16, 0])
+def test_counter_fed_items_match_oracle(ctx, chunk):
+    """The check-state kernels give identical results whether items come from
+    the shared counters (any chunk size) or a fixed stride."""
+    spec = codes.n2304_r12()
+    base = spec.base_matrix()
+    _, _, llr = frames.random_frames(base, spec.z, 512, 1.75, seed=61)
+    say("counter-fed items, chunk", chunk)
+    code = gpu_code(ctx, base, spec.z, "state")
+    ctx.set_tuning(_lib.TUNE_CHUNK, chunk)
+    try:
+        gpu = decode_batch_arrays(code, llr, 30)
+    finally:
+        ctx.set_tuning(_lib.TUNE_CHUNK, -1)
+        ctx.set_path(_lib.PATH_AUTO)
+    ref = orc.Code.from_qc(base, spec.z).decode_batch(llr, 30, threads=8)
+    assert_same(gpu, ref, f"chunk {chunk}")

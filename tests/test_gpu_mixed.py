@@ -210,3 +210,17 @@ def test_learned_mixed_rejects_mismatched_decoders(ctx):
     uneven = [NmsDecoder(gcodes[0], 4, init=0.8), NmsDecoder(gcodes[1], 5, init=0.8)]
     with pytest.raises(_lib.Gf2lError, match="iteration count"):
         decode_mixed_arrays(cs, ids, packed, 10, decoders=uneven)
+
+
+@pytest.mark.parametrize("chunk", [1, 4])
+def test_counter_fed_mixed_items_match_oracle(ctx, chunk):
+    specs = [codes.n2304_r12()] + codes.mixed_twelve()[:2]
+    cs = CodeSet([LdpcCode(QuasiCyclicLdpc(s.base_matrix(), s.z), ctx) for s in specs])
+    ids, blocks = build_batch(specs, 3, seed=71)
+    say("mixed counter-fed items, chunk", chunk)
+    ctx.set_tuning(_lib.TUNE_CHUNK, chunk)
+    try:
+        res = decode_mixed(cs, ids, blocks, 20)
+    finally:
+        ctx.set_tuning(_lib.TUNE_CHUNK, -1)
+    check_against(res, oracle_results(specs, ids, blocks, 20), f"chunk {chunk}")

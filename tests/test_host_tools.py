@@ -58,6 +58,6 @@ def test_apply_rejects_unknown_knob():
 
         def set_path(self, p):
             pass
-    autotune.apply(Dummy(), {"wg_per_cu": 2, "path": _lib.PATH_STATE})
+    autotune.apply(Dummy(), {"wg_per_cu": 2, "chunk": 8, "path": _lib.PATH_STATE})
     with pytest.raises(ValueError):
         autotune.apply(Dummy(), {"frames_per_wg": 2})
