@@ -65,6 +65,15 @@ SIGNATURES = {
     "gf2l_code_destroy": (None, [ctypes.c_void_p]),
     "gf2l_qc_ira_shifts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.c_uint32, _u32p, _i32p]),
+    "gf2l_dvb_t2_params": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _u32p, _u32p, _u32p, _u32p,
+                                          _u32p]),
+    "gf2l_dvb_t2_table": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "gf2l_code_dvb_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                           ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                           ctypes.c_void_p, ctypes.c_void_p, _c_void_pp]),
+    "gf2l_code_dvb_t2": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_void_pp]),
+    "gf2l_code_gf2data": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t,
+                                         _c_void_pp]),
     "gf2l_decode_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_size_t, ctypes.c_uint32, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -108,6 +117,14 @@ SIGNATURES = {
                                                     ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
                                                     ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
                                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "gf2_ldpc_decoder_create": (ctypes.c_void_p, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+    "gf2_ldpc_decode": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                         ctypes.c_void_p]),
+    "gf2_ldpc_decode_frames": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                                ctypes.c_void_p, ctypes.c_void_p]),
+    "gf2_ldpc_decoder_info": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p]),
+    "gf2_ldpc_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "gf2_ldpc_decoder_destroy": (None, [ctypes.c_void_p]),
 }
 
 _lib = None

@@ -67,6 +67,31 @@ def random_frames(base: np.ndarray, z: int, frames: int, ebn0_db: float, seed: i
     return info, cw, llr
 
 
+def encode_dvb(table, n: int, k: int, q: int, info: np.ndarray, z: int = 360) -> np.ndarray:
+    """Systematic DVB-T2 / DVB-S2 LDPC encoding (ETSI EN 302 755 section
+    6.1.2.1): each information bit i*z + j is accumulated into the parity
+    bits (p + j*q) mod m for every address p of table row i, then
+    p_t ^= p_(t-1) for t = 1..m-1.  info [F, k] -> codewords [F, n]."""
+    info = np.asarray(info, dtype=np.uint8).reshape(-1, k)
+    m = n - k
+    par = np.zeros((info.shape[0], m), dtype=np.uint8)
+    lanes = np.arange(z, dtype=np.int64) * q
+    for i, row in enumerate(table):
+        u = info[:, i * z:(i + 1) * z]
+        for p in row:
+            par[:, (int(p) + lanes) % m] ^= u
+    par = np.bitwise_xor.accumulate(par, axis=1)
+    return np.concatenate([info, par], axis=1)
+
+
+def dvb_random_frames(table, n: int, k: int, q: int, frames: int, ebn0_db: float, seed: int, z: int = 360):
+    """(info bits [F,k], codewords [F,n], llr [F,n] float32) of a DVB code."""
+    rng = np.random.default_rng(seed)
+    info = rng.integers(0, 2, size=(frames, k), dtype=np.uint8)
+    cw = encode_dvb(table, n, k, q, info, z)
+    return info, cw, awgn_llr(cw, ebn0_db, k / n, rng)
+
+
 def unpack_words(words: np.ndarray, nbits: int) -> np.ndarray:
     """[F, W] LE u64 words -> [F, nbits] uint8."""
     w = np.ascontiguousarray(words, dtype=np.uint64)

@@ -110,3 +110,52 @@ def mixed_twelve(seed: int = DEFAULT_SEED) -> List[QcIraSpec]:
     """The twelve Z=360 codes: N in {16200, 64800} x six rates."""
     return [QcIraSpec(n, k, 360, seed + i)
             for i, (n, k) in enumerate(sorted(p for p in PROFILES if p[0] != 2304))]
+
+
+@dataclass(frozen=True)
+class DvbT2Spec:
+    """One DVB-T2 LDPC code: ETSI EN 302 755 parameters and address table, as
+    compiled into libgf2ldpc (DvbParams::for_code, crates/gf2-coding/src/ldpc/
+    dvb_t2/params.rs:56-86, and dvb_t2/dvb_t2_matrices.rs)."""
+    frame: int            # 0 short (16200), 1 normal (64800)
+    rate: int             # 0 = 1/2, 1 = 3/5, 2 = 2/3, 3 = 3/4, 4 = 4/5, 5 = 5/6
+    n: int
+    k: int
+    q: int
+    table: Tuple[Tuple[int, ...], ...]
+    z: int = 360
+
+    @property
+    def m(self) -> int:
+        return self.n - self.k
+
+    @property
+    def name(self) -> str:
+        return f"dvb-t2-{'normal' if self.frame else 'short'}-{DVB_RATE_NAMES[self.rate]}"
+
+
+DVB_RATE_NAMES = ["1/2", "3/5", "2/3", "3/4", "4/5", "5/6"]
+
+
+def dvb_t2(frame: int, rate: int) -> DvbT2Spec:
+    """Parameters and table of a DVB-T2 code, read from the library (no GPU
+    needed)."""
+    import ctypes
+
+    from . import _lib
+    lib = _lib.load()
+    n, k, q, rows, entries = (ctypes.c_uint32() for _ in range(5))
+    rc = lib.gf2l_dvb_t2_params(int(frame), int(rate), ctypes.byref(n), ctypes.byref(k), ctypes.byref(q),
+                                ctypes.byref(rows), ctypes.byref(entries))
+    if rc != 0:
+        raise ValueError("DVB-T2 frame size must be 0 (short) or 1 (normal) and rate 0..5")
+    lens = np.zeros(rows.value, dtype=np.uint32)
+    addr = np.zeros(entries.value, dtype=np.uint32)
+    lib.gf2l_dvb_t2_table(int(frame), int(rate), lens.ctypes.data, addr.ctypes.data)
+    ends = np.cumsum(lens)
+    table = tuple(tuple(int(a) for a in addr[e - l:e]) for l, e in zip(lens, ends))
+    return DvbT2Spec(int(frame), int(rate), n.value, k.value, q.value, table)
+
+
+def dvb_t2_all() -> List[DvbT2Spec]:
+    return [dvb_t2(f, r) for f in (0, 1) for r in range(6)]

@@ -9,6 +9,9 @@ Names, argument meaning and error behaviour follow crates/gf2-coding:
   LdpcDecoder.decode_soft    src/ldpc/core.rs:877-885
   DecoderResult              src/traits.rs:116-156
   BitVec (result words)      crates/gf2-core/src/bitvec.rs:40-63, 204
+  LdpcCode.dvb_t2_short / dvb_t2_normal
+                             src/ldpc/core.rs:197-255, dvb_t2/builder.rs:74-118
+  LdpcCode.from_gf2data      crates/gf2-core/src/io/sparse.rs:151-200, 397-449
 Where the reference panics, these raise ValueError with the reference's
 message.  Every decode runs on the GPU through libgf2ldpc; there is no CPU
 path.
@@ -16,6 +19,7 @@ path.
 from __future__ import annotations
 
 import ctypes
+import enum
 from dataclasses import dataclass
 from typing import Iterable, List, Optional, Sequence
 
@@ -204,12 +208,30 @@ class QuasiCyclicLdpc:
         return np.concatenate(out)
 
 
+class CodeRate(enum.IntEnum):
+    """DVB-T2 code rates in the order of the reference's CodeRate
+    (crates/gf2-coding/src/bch), as the C ABI numbers them."""
+    RATE_1_2 = 0
+    RATE_3_5 = 1
+    RATE_2_3 = 2
+    RATE_3_4 = 3
+    RATE_4_5 = 4
+    RATE_5_6 = 5
+
+
 class LdpcCode:
     """An LDPC code uploaded to one HIP device: quasi-cyclic (from a base
     matrix) or general sparse (from an edge list)."""
 
     def __init__(self, qc: Optional[QuasiCyclicLdpc], ctx: Optional[Context] = None,
-                 _edges: Optional[tuple] = None):
+                 _edges: Optional[tuple] = None, _build=None):
+        if _build is not None:
+            self._ctx = ctx or default_context()
+            self._qc = None
+            h = ctypes.c_void_p()
+            _lib.check(self._ctx.handle, _build(self._ctx, ctypes.byref(h)))
+            self._set_handle(h)
+            return
         if qc is not None:
             qc._check_shifts()
         else:
@@ -236,10 +258,13 @@ class LdpcCode:
             _lib.check(self._ctx.handle, lib.gf2l_code_edges(
                 self._ctx.handle, m, n, e32.ctypes.data if e32.size else None, e32.shape[0],
                 ctypes.byref(h)))
+        self._set_handle(h)
+
+    def _set_handle(self, h) -> None:
         self._h = h
         n, m, k = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
         nnz = ctypes.c_uint64()
-        lib.gf2l_code_info(h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(k), ctypes.byref(nnz))
+        _lib.load().gf2l_code_info(h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(k), ctypes.byref(nnz))
         self._n, self._m, self._k, self._nnz = n.value, m.value, k.value, nnz.value
 
     @classmethod
@@ -251,6 +276,42 @@ class LdpcCode:
         """LdpcCode::from_edges (core.rs:96-104): (check, variable) pairs,
         duplicates cancel in pairs as in SpBitMatrix::from_coo."""
         return cls(None, ctx, _edges=(int(m), int(n), edges))
+
+    @classmethod
+    def dvb_t2_short(cls, rate: "CodeRate", ctx: Optional[Context] = None) -> "LdpcCode":
+        """LdpcCode::dvb_t2_short (core.rs:197-218): N=16200, ETSI EN 302 755
+        Annex B table of `rate`; decoded on the CSR path."""
+        return cls(None, ctx, _build=lambda c, h: _lib.load().gf2l_code_dvb_t2(c.handle, 0, int(rate), h))
+
+    @classmethod
+    def dvb_t2_normal(cls, rate: "CodeRate", ctx: Optional[Context] = None) -> "LdpcCode":
+        """LdpcCode::dvb_t2_normal (core.rs:240-255): N=64800, Annex A table."""
+        return cls(None, ctx, _build=lambda c, h: _lib.load().gf2l_code_dvb_t2(c.handle, 1, int(rate), h))
+
+    @classmethod
+    def from_dvb_table(cls, n: int, k: int, q: int, table, z: int = 360,
+                       ctx: Optional[Context] = None) -> "LdpcCode":
+        """build_dvb_edges (dvb_t2/builder.rs:74-118) over a caller's address
+        table (one list of parity addresses per z-bit information block)."""
+        rows = [list(map(int, r)) for r in table]
+        lens = np.array([len(r) for r in rows], dtype=np.uint32)
+        addr = np.array([a for r in rows for a in r], dtype=np.uint32)
+        return cls(None, ctx, _build=lambda c, h: _lib.load().gf2l_code_dvb_table(
+            c.handle, int(n), int(k), int(q), int(z), len(rows), lens.ctypes.data if lens.size else None,
+            addr.ctypes.data if addr.size else None, h))
+
+    @classmethod
+    def from_gf2data(cls, source, ctx: Optional[Context] = None) -> "LdpcCode":
+        """Code from a GF2DATA file (path or bytes) holding the parity-check
+        matrix as SpBitMatrix (type 3) or SpBitMatrixDual (type 4)
+        (crates/gf2-core/src/io/sparse.rs:151-200, 397-449); rows are checks.
+        Equivalent to LdpcCode::from_edges over the loaded matrix's entries."""
+        if isinstance(source, (bytes, bytearray, memoryview)):
+            data = bytes(source)
+        else:
+            with open(source, "rb") as f:
+                data = f.read()
+        return cls(None, ctx, _build=lambda c, h: _lib.load().gf2l_code_gf2data(c.handle, data, len(data), h))
 
     @property
     def handle(self):

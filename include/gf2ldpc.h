@@ -139,6 +139,46 @@ void gf2l_code_destroy(gf2l_code *code);
 int gf2l_qc_ira_shifts(uint64_t seed, uint32_t mb, uint32_t nb, uint32_t z,
                        const uint32_t *info_col_weight, int32_t *shifts_out);
 
+/* ---- code ingestion: DVB-T2 tables and GF2DATA files ------------------ */
+
+/* DVB-T2 code parameters and ETSI EN 302 755 address tables, as
+ * DvbParams::for_code (crates/gf2-coding/src/ldpc/dvb_t2/params.rs:56-86)
+ * and the tables of dvb_t2/dvb_t2_matrices.rs.  frame: 0 short (N=16200),
+ * 1 normal (N=64800); rate: 0 = 1/2, 1 = 3/5, 2 = 2/3, 3 = 3/4, 4 = 4/5,
+ * 5 = 5/6 (the order of CodeRate).  gf2l_dvb_t2_table writes `rows` row
+ * lengths and `entries` addresses (row after row); either pointer may be
+ * NULL.  GF2L_ERR_INVALID for an unknown frame or rate. */
+int gf2l_dvb_t2_params(int frame, int rate, uint32_t *n, uint32_t *k, uint32_t *q, uint32_t *rows,
+                       uint32_t *entries);
+int gf2l_dvb_t2_table(int frame, int rate, uint32_t *row_len, uint32_t *addresses);
+
+/* DVB-style IRA code from an address table: build_dvb_edges
+ * (dvb_t2/builder.rs:74-118) + LdpcCode::from_edges.  Information bit
+ * i*z + j of table row i and address p connects to check (p + j*q) mod m
+ * (m = n - k); check p connects to parity bits k+p and, for p > 0, k+p-1.
+ * Validation as validate_table (builder.rs:15-41): rows == k / z, every
+ * address < m, no single-entry placeholder table -- GF2L_ERR_INVALID with
+ * the reference's panic text otherwise.  Any table of this shape works
+ * (DVB-S2 uses the same construction); the code runs on the CSR path. */
+int gf2l_code_dvb_table(gf2l_ctx *ctx, uint32_t n, uint32_t k, uint32_t q, uint32_t z, uint32_t rows,
+                        const uint32_t *row_len, const uint32_t *addresses, gf2l_code **out);
+/* LdpcCode::dvb_t2_short / LdpcCode::dvb_t2_normal (crates/gf2-coding/src/
+ * ldpc/core.rs:197-255): frame and rate as gf2l_dvb_t2_params. */
+int gf2l_code_dvb_t2(gf2l_ctx *ctx, int frame, int rate, gf2l_code **out);
+
+/* Code from a GF2DATA file image (`bytes`, `len` bytes, e.g. a whole file
+ * read into memory): crates/gf2-core/src/io/mod.rs:6-98, header.rs:33-86.
+ * Type 3 (SpBitMatrix, COO pairs; entries kept once, sparse.rs:151-200,
+ * 232-259) or type 4 (SpBitMatrixDual, CSR + CSC; the CSR rows are used,
+ * sparse.rs:397-449); the code is LdpcCode::from_edges over the matrix
+ * entries (rows = checks, cols = variables).  Errors (GF2L_ERR_INVALID)
+ * carry the reference's IoError text: "Invalid magic bytes", "Unsupported
+ * format version: v", "Unknown type tag: t", data-length and metadata
+ * mismatches; a compressed payload (flag bit 0, which the reference does not
+ * implement either) returns GF2L_ERR_UNSUPPORTED.  A checksum trailer (flag
+ * bit 1) is ignored, as by the reference reader. */
+int gf2l_code_gf2data(gf2l_ctx *ctx, const void *bytes, size_t len, gf2l_code **out);
+
 /* ---- device memory (for callers without their own HIP allocator) ------ */
 
 /* hipMalloc / hipFree on the context's device; copies are synchronous on the
@@ -287,6 +327,33 @@ int gf2l_decode_mixed_nms_device(gf2l_ctx *ctx, gf2l_codeset *set, const gf2l_nm
                                  const uint16_t *code_ids, size_t frames, const float *d_llr,
                                  uint32_t fallback_max_it, uint64_t *d_msg_words, uint32_t *d_iterations,
                                  uint8_t *d_converged, uint8_t *d_syndrome_ok, uint8_t *d_fallback);
+
+/* ---- SDR-style decoder (GNU Radio blocks) ---------------------------- */
+
+/* The C FFI planned in crates/gf2-coding/docs/SDR_INTEGRATION.md:39-57 for a
+ * GNU Radio DVB-T2 LDPC decoder block.  A handle owns its own context on HIP
+ * device 0, one DVB-T2 code and its staging buffers; one handle per thread.
+ *   create   frame_size 0 = short (16200), 1 = normal (64800); code_rate
+ *            0 = 1/2, 1 = 3/5, 2 = 2/3, 3 = 3/4, 4 = 4/5, 5 = 5/6;
+ *            NULL on a bad argument or when no HIP device is usable.
+ *   decode   one frame: `len` must equal n; writes the k information bits,
+ *            one per byte (0 or 1, GNU Radio's unpacked-bit stream), and
+ *            returns the iteration count when the syndrome is satisfied,
+ *            -1 when it is not (the output still holds the hard decision),
+ *            -2 on an error (gf2_ldpc_last_error says which).
+ *   decode_frames  `frames` frames at once (the block's work() with several
+ *            items): outputs k bytes per frame, per-frame iteration counts
+ *            (-1 = not converged) when `iterations` is not NULL; returns the
+ *            number of converged frames or -2.
+ * Results equal gf2l_decode_batch on the same code. */
+typedef struct gf2_ldpc_decoder gf2_ldpc_decoder;
+gf2_ldpc_decoder *gf2_ldpc_decoder_create(uint32_t frame_size, uint32_t code_rate, uint32_t max_iterations);
+int32_t gf2_ldpc_decode(gf2_ldpc_decoder *decoder, const float *llrs, size_t len, uint8_t *output);
+int32_t gf2_ldpc_decode_frames(gf2_ldpc_decoder *decoder, const float *llrs, size_t frames, uint8_t *output,
+                               int32_t *iterations);
+int gf2_ldpc_decoder_info(const gf2_ldpc_decoder *decoder, uint32_t *n, uint32_t *k);
+const char *gf2_ldpc_last_error(const gf2_ldpc_decoder *decoder);
+void gf2_ldpc_decoder_destroy(gf2_ldpc_decoder *decoder);
 
 #ifdef __cplusplus
 }

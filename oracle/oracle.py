@@ -98,6 +98,23 @@ def qc_to_edges(base: np.ndarray, z: int) -> np.ndarray:
     return out[: cnt.value].copy()
 
 
+def dvb_edges(table, n: int, k: int, q: int, z: int = 360) -> np.ndarray:
+    """build_dvb_edges (crates/gf2-coding/src/ldpc/dvb_t2/builder.rs:74-118):
+    for table row i, address p and lane j in [0, z), the edge (check
+    (p + j*q) mod m, variable i*z + j); then check p -> variable k+p, and for
+    p > 0 check p -> variable k+p-1.  (check, variable) pairs in the
+    reference's push order."""
+    m = n - k
+    j = np.arange(z, dtype=np.int64)
+    parts = [np.stack([(int(p) + j * q) % m, i * z + j], axis=1) for i, row in enumerate(table) for p in row]
+    p = np.arange(m, dtype=np.int64)
+    stair = np.empty((2 * m - 1, 2), dtype=np.int64)
+    stair[0] = (0, k)
+    stair[1::2] = np.stack([p[1:], k + p[1:]], axis=1)
+    stair[2::2] = np.stack([p[1:], k + p[1:] - 1], axis=1)
+    return np.concatenate(parts + [stair]) if parts else stair
+
+
 class Code:
     """Oracle code: reference CSR/CSC built by from_coo."""
 

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_symbols():
     src = open(os.path.join(ROOT, "include", "gf2ldpc.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(gf2l_\w+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(gf2l_\w+|gf2_ldpc_\w+)\s*\(", src)))
 
 
 def test_library_exports_every_declared_symbol():
