@@ -133,6 +133,18 @@ class DvbT2Spec:
     def name(self) -> str:
         return f"dvb-t2-{'normal' if self.frame else 'short'}-{DVB_RATE_NAMES[self.rate]}"
 
+    def edges(self) -> np.ndarray:
+        """(check, variable) pairs of the code as build_dvb_edges lists them
+        (dvb_t2/builder.rs:74-118), e.g. for LdpcCode.from_edges (CSR path)
+        or a GF2DATA export."""
+        j = np.arange(self.z, dtype=np.int64)
+        parts = [np.stack([(p + j * self.q) % self.m, i * self.z + j], axis=1)
+                 for i, row in enumerate(self.table) for p in row]
+        t = np.arange(self.m, dtype=np.int64)
+        parts.append(np.stack([t, self.k + t], axis=1))
+        parts.append(np.stack([t[1:], self.k + t[1:] - 1], axis=1))
+        return np.concatenate(parts)
+
 
 DVB_RATE_NAMES = ["1/2", "3/5", "2/3", "3/4", "4/5", "5/6"]
 

@@ -21,6 +21,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_runtime.h>
+
 #include "dvb_t2_tables.h"
 #include "gf2ldpc.h"
 #include "internal.h"
@@ -132,6 +134,59 @@ extern "C" int gf2l_code_dvb_table(gf2l_ctx *ctx, uint32_t n, uint32_t k, uint32
             if (addresses[at] >= m)
                 return invalid(ctx, "Invalid parity index " + std::to_string(addresses[at]) + " in table row " +
                                         std::to_string(r) + " (must be < " + std::to_string(m) + ")");
+    // Quasi-cyclic form (the decoder's fast path): check c = a + q*b and
+    // parity bit k + a + q*b become lane b of block-row / parity block-column
+    // a.  Table address p of row i is then a circulant in block (p mod q, i)
+    // with shift (z - p div q) mod z; the staircase gives identity blocks
+    // (a, kb+a) and (a, kb+a-1), and block (0, kb+q-1) with shift z-1 that
+    // lacks its lane-0 edge (check 0 has one parity bit).  Needs m = q*z,
+    // k = rows*z, and no address repeated within a row (a repeat cancels in
+    // from_coo); otherwise the code runs on the CSR path.
+    bool qc = m == (uint64_t)q * z && k == (uint64_t)rows * z && q >= 2 && z >= 2;
+    for (uint32_t r = 0, at = 0; qc && r < rows; at += row_len[r], ++r) {
+        std::vector<uint32_t> v(addresses + at, addresses + at + row_len[r]);
+        std::sort(v.begin(), v.end());
+        qc = std::adjacent_find(v.begin(), v.end()) == v.end();
+    }
+    if (qc) {
+        const uint32_t kb = rows, nb = kb + q;
+        std::vector<std::vector<uint2>> per_row(q);
+        for (uint32_t r = 0, at = 0; r < rows; ++r)
+            for (uint32_t i = 0; i < row_len[r]; ++i, ++at) {
+                const uint32_t p = addresses[at];
+                per_row[p % q].push_back(make_uint2(r, (z - p / q) % z));
+            }
+        gf2l_code *c = new gf2l_code;
+        c->device = ctx->device;
+        c->h_row_ptr.assign(q + 1, 0);
+        uint32_t max_dc = 0;
+        for (uint32_t a = 0; a < q; ++a) {
+            auto &v = per_row[a];
+            std::stable_sort(v.begin(), v.end(), [](const uint2 &x, const uint2 &y) {
+                return x.x != y.x ? x.x < y.x : x.y < y.y;
+            });
+            if (a == 0) {
+                v.push_back(make_uint2(kb + q - 1, z - 1));
+                v.push_back(make_uint2(kb, 0));
+            } else {
+                v.push_back(make_uint2(kb + a - 1, 0));
+                v.push_back(make_uint2(kb + a, 0));
+            }
+            c->h_row_edge.insert(c->h_row_edge.end(), v.begin(), v.end());
+            c->h_row_ptr[a + 1] = (uint32_t)c->h_row_edge.size();
+            max_dc = std::max<uint32_t>(max_dc, (uint32_t)v.size());
+        }
+        if (max_dc <= gf2l::kMaxStateDegree) {
+            if (hipSetDevice(ctx->device) != hipSuccess) {
+                delete c;
+                return gf2l::fail(ctx, GF2L_ERR_HIP, "hipSetDevice failed");
+            }
+            const int rc = gf2l::build_qc_view(ctx, c, q, nb, z, q);
+            if (rc == GF2L_OK) *out = c;
+            return rc;
+        }
+        delete c;
+    }
     // build_dvb_edges (builder.rs:74-118): table edges, then the staircase
     std::vector<uint32_t> rc;
     rc.reserve(2 * (total * z + 2ull * m));

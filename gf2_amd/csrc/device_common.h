@@ -189,13 +189,55 @@ struct ItemStream {
 // u32 (min1, min2, argmin | output sign bits << 5), read and written as one
 // 12-byte access; its beliefs are [nb][zp] floats.
 
-// Check update of block-row br of one frame: threads map to the z lifted
-// checks; belief reads of a circulant are a rotated contiguous run.  fsi /
-// fso: the frame's state (read / written; the same array in the decoder).  In
-// the first iteration the inputs are the channel LLRs fb (stride z) with
-// c2v = 0; later fb is the frame's belief array (stride zp) and the previous
-// message is rebuilt from fsi (times its weight w_prev[e] for NMS).
-template <int MAXD, bool FIRST, bool NMS>
+// Output sign bits (indexed by table slot) of a DVB check with a -0.0
+// input, where the AVX2 lane / remainder rule needs each input's place in
+// the check's variable order at lane i.  The slots of `row` (d_tab of them)
+// are in the standard's variable order except inside a block: slots of one
+// block-column (circulants of one block) are adjacent and sorted by shift,
+// and at lane i their order is that list rotated (slots with i + shift >= z
+// wrapped and come first).  `skip` is a slot without an edge at this lane
+// (or d_tab).  Out of line: only reached when an input is -0.0.
+[[maybe_unused]] __device__ __noinline__ uint32_t dvb_signs_slow(const uint2 *row, uint32_t d_tab, uint32_t z, uint32_t i,
+                                                uint32_t skip, uint64_t sb, uint64_t lt) {
+    const uint32_t d = d_tab - (skip < d_tab ? 1u : 0u);
+    auto pos = [&](uint32_t p) -> uint32_t {
+        const uint32_t bc = row[p].x;
+        uint32_t g0 = p, g1 = p + 1;
+        while (g0 > 0 && row[g0 - 1].x == bc) --g0;
+        while (g1 < d_tab && row[g1].x == bc) ++g1;
+        const uint32_t g = g1 - g0, l = p - g0;
+        uint32_t r = 0;  // slots of the group that did not wrap
+        for (uint32_t q = g0; q < g1; ++q) r += row[q].y + i < z ? 1u : 0u;
+        uint32_t x = l + g - r;
+        if (x >= g) x -= g;
+        return g0 + x - (skip < p ? 1u : 0u);
+    };
+    CheckAcc t;
+    t.init();
+    for (uint32_t p = 0; p < d_tab; ++p) {
+        if (p == skip) continue;
+        const uint32_t ps = pos(p);
+        t.sb |= ((sb >> p) & 1ull) << ps;
+        t.lt |= ((lt >> p) & 1ull) << ps;
+    }
+    uint32_t out = 0;
+    for (uint32_t p = 0; p < d_tab; ++p)
+        if (p != skip) out |= t.neg(d, pos(p)) << p;
+    return out;
+}
+
+// Output sign bits (indexed by slot) of a DVB check: without a -0.0 input
+// the sign of each output is the parity of the other inputs' signs,
+// whatever their order.
+template <int D>
+__device__ __forceinline__ uint32_t dvb_signs(const CodeView &cv, const CheckAcc &acc, const uint2 *row,
+                                              uint32_t i, uint32_t skip) {
+    constexpr uint64_t all = D >= 64 ? ~0ull : (1ull << D) - 1;
+    if (acc.sb == acc.lt) return (uint32_t)(((__popcll(acc.sb) & 1u) ? all : 0ull) ^ acc.sb);
+    return dvb_signs_slow(row, D, cv.z, i, skip, acc.sb, acc.lt);
+}
+
+template <int MAXD, bool FIRST, bool NMS, bool DVB = false>
 __device__ __forceinline__ void cn_state_item(const CodeView &cv, uint32_t br, const uint32_t *fsi,
                                               uint32_t *fso, const float *__restrict__ fb,
                                               const float *__restrict__ w_prev) {
@@ -228,10 +270,21 @@ __device__ __forceinline__ void cn_state_item(const CodeView &cv, uint32_t br, c
                 m2o = __uint_as_float(o.y);
                 mo = o.z;
             }
+            // DVB: block-row 0 has no staircase edge at lane 0 (check 0 of
+            // the standard code); `skip` is that slot, D when every slot is live
+            uint32_t skip = D;
+            if constexpr (DVB) {
+                if (br == 0 && i == 0) {
+#pragma unroll
+                    for (int p = 0; p < D; ++p)
+                        if (re[p].x == cv.wrap_bc) skip = p;
+                }
+            }
             CheckAcc acc;
             acc.init();
 #pragma unroll
             for (int p = 0; p < D; ++p) {
+                if (DVB && (uint32_t)p == skip) continue;
                 float v = xv[p];
                 if constexpr (!FIRST) {
                     float c = recon(m1o, m2o, mo, p);
@@ -241,7 +294,11 @@ __device__ __forceinline__ void cn_state_item(const CodeView &cv, uint32_t br, c
                 acc.add(p, v);
             }
             float m1 = acc.min1, m2 = acc.min2;
-            uint32_t meta = acc.idx | (acc.signs<D>(D) << 5);
+            uint32_t meta;
+            if constexpr (DVB)
+                meta = acc.idx | (dvb_signs<D>(cv, acc, cv.row_edge + e0, i, skip) << 5);
+            else
+                meta = acc.idx | (acc.signs<D>(D) << 5);
             if constexpr (D == 1) {  // the lone message is +0.0
                 m1 = 0.0f;
                 m2 = 0.0f;
@@ -257,7 +314,7 @@ __device__ __forceinline__ void cn_state_item(const CodeView &cv, uint32_t br, c
 // order; hard decisions packed by wave ballot into 64-bit words (fh may be
 // null).  fs: frame's state, fl: the block-column's LLRs, fb: its beliefs.
 // Needs blockDim.x % 64 == 0.
-template <int MAXD, bool NMS>
+template <int MAXD, bool NMS, bool DVB = false>
 __device__ __forceinline__ void vn_state_item(const CodeView &cv, uint32_t bc, const uint32_t *__restrict__ fs,
                                               const float *__restrict__ fl, float *__restrict__ fb,
                                               uint64_t *__restrict__ fh, const float *__restrict__ w_cur) {
@@ -290,11 +347,45 @@ __device__ __forceinline__ void vn_state_item(const CodeView &cv, uint32_t bc, c
                     mt[q] = o.z;
                 }
                 bel = ldu(fl, j);
+                if constexpr (DVB) {
+                    // Slots are in ascending standard check order at lane 0
+                    // (check a + q*b of block-row a, lane b).  At lane j the
+                    // order is that list rotated: slots whose check lane
+                    // (j + s) mod z wrapped come first, s = z - shift.
+                    float c[DA];
+                    uint64_t live = 0;
+                    uint32_t r = 0;
 #pragma unroll
-                for (int q = 0; q < D; ++q) {
-                    float c = recon(m1[q], m2[q], mt[q], ci[q].y & 31u);
-                    if constexpr (NMS) c = wq[q] * c;
-                    bel = bel + c;
+                    for (int q = 0; q < D; ++q) {
+                        c[q] = recon(m1[q], m2[q], mt[q], ci[q].y & 31u);
+                        const uint32_t sh = ci[q].y >> 5, s = sh ? cv.z - sh : 0u;
+                        r += (s + j < cv.z) ? 1u : 0u;
+                        // the staircase has no edge from check 0 to the last parity bit
+                        const bool dead = bc == cv.wrap_bc && ci[q].x == 0 && j == cv.z - 1;
+                        live |= (uint64_t)(dead ? 0u : 1u) << q;
+                    }
+#pragma unroll
+                    for (int k = 1; k < D; k <<= 1) {
+                        const bool rot = (r & (uint32_t)k) != 0;
+                        float t[DA];
+#pragma unroll
+                        for (int q = 0; q < D; ++q) t[q] = rot ? c[(q + k) % DA] : c[q];
+#pragma unroll
+                        for (int q = 0; q < D; ++q) c[q] = t[q];
+                    }
+                    const uint64_t full = (D >= 64 ? ~0ull : (1ull << D) - 1);
+                    const uint32_t rr = r % (uint32_t)DA;
+                    live = ((live >> rr) | (live << ((DA - rr) % 64))) & full;
+#pragma unroll
+                    for (int q = 0; q < D; ++q)
+                        if ((live >> q) & 1ull) bel = bel + c[q];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < D; ++q) {
+                        float c = recon(m1[q], m2[q], mt[q], ci[q].y & 31u);
+                        if constexpr (NMS) c = wq[q] * c;
+                        bel = bel + c;
+                    }
                 }
                 stu(fb, j, bel);
             }
@@ -336,7 +427,9 @@ __device__ __forceinline__ int frame_unsat(const CodeView &cv, uint64_t *sh, con
             uint64_t acc = 0;
             for (uint32_t e = cv.row_ptr[br]; e < cv.row_ptr[br + 1]; ++e) {
                 const uint2 re = cv.row_edge[e];
-                acc ^= cyc64(sh + re.x * zw, z, zw, wrap_add(64u * w, re.y, z));
+                uint64_t v = cyc64(sh + re.x * zw, z, zw, wrap_add(64u * w, re.y, z));
+                if (br == 0 && w == 0 && re.x == cv.wrap_bc) v &= ~1ull;  // no edge at check 0
+                acc ^= v;
             }
             if (w == zw - 1 && (z & 63u)) acc &= (1ull << (z & 63u)) - 1;
             bad |= acc;
@@ -347,6 +440,7 @@ __device__ __forceinline__ int frame_unsat(const CodeView &cv, uint64_t *sh, con
             uint32_t par = 0;
             for (uint32_t e = cv.row_ptr[br]; e < cv.row_ptr[br + 1]; ++e) {
                 const uint2 re = cv.row_edge[e];
+                if (br == 0 && i == 0 && re.x == cv.wrap_bc) continue;
                 par ^= (uint32_t)(sh[re.x * zw] >> wrap_add(i, re.y, z)) & 1u;
             }
             bad |= par;

@@ -22,6 +22,7 @@ void free_workspace(gf2l::Workspace &ws) {
     hipFree(ws.hb);
     hipFree(ws.lists);
     hipFree(ws.counts);
+    hipFree(ws.llr_perm);
     if (ws.h_count) hipHostFree(ws.h_count);
     ws = gf2l::Workspace{};
 }
@@ -33,13 +34,17 @@ int ensure_workspace(gf2l_ctx *ctx, const gf2l::CodeView &cv, int path, size_t f
     const size_t need_state = path == gf2l::kState ? 3ull * cv.mb * cv.zp * frames : 0;
     const size_t need_bel = (path == gf2l::kState ? (size_t)cv.nb * cv.zp : (size_t)cv.n) * frames;
     const size_t need_hb = (size_t)cv.nb * cv.zw * frames;
+    const size_t need_llr = cv.dvb_q ? (size_t)cv.n * frames : 0;
     if (ws.lists && ws.cap_frames >= frames && ws.cap_c2v >= need_c2v &&
-        ws.cap_state >= need_state && ws.cap_bel >= need_bel && ws.cap_hb >= need_hb)
+        ws.cap_state >= need_state && ws.cap_bel >= need_bel && ws.cap_hb >= need_hb &&
+        ws.cap_llr >= need_llr)
         return GF2L_OK;
     const size_t f = std::max(frames, ws.cap_frames);
     const size_t c2v = std::max(need_c2v, ws.cap_c2v), st = std::max(need_state, ws.cap_state),
-                 bel = std::max(need_bel, ws.cap_bel), hb = std::max(need_hb, ws.cap_hb);
+                 bel = std::max(need_bel, ws.cap_bel), hb = std::max(need_hb, ws.cap_hb),
+                 lp = std::max(need_llr, ws.cap_llr);
     free_workspace(ws);
+    if (lp) GF2L_HIP(ctx, hipMalloc(&ws.llr_perm, lp * sizeof(float)));
     if (c2v) GF2L_HIP(ctx, hipMalloc(&ws.c2v, c2v * sizeof(float)));
     if (st) GF2L_HIP(ctx, hipMalloc(&ws.state, st * sizeof(uint32_t)));
     GF2L_HIP(ctx, hipMalloc(&ws.belief, bel * sizeof(float)));
@@ -52,12 +57,17 @@ int ensure_workspace(gf2l_ctx *ctx, const gf2l::CodeView &cv, int path, size_t f
     ws.cap_state = st;
     ws.cap_bel = bel;
     ws.cap_hb = hb;
+    ws.cap_llr = lp;
     return GF2L_OK;
 }
 
 // Resolves the decoder path for a code (GF2L_PATH_*), or -1 if the forced
 // path cannot run it.
 int resolve_path(const gf2l_ctx *ctx, const gf2l::CodeView &cv) {
+    if (cv.dvb_q)  // the DVB lane rules live in the state kernels only
+        return (ctx->path == gf2l::kAuto || ctx->path == gf2l::kState) && cv.max_dc <= gf2l::kMaxStateDegree
+                   ? gf2l::kState
+                   : -1;
     const size_t bytes = gf2l::lds_path_bytes(cv);
     switch (ctx->path) {
         case gf2l::kLds: return bytes <= ctx->lds_limit ? gf2l::kLds : -1;
@@ -105,6 +115,8 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
         if (d_fb) GF2L_HIP(ctx, hipMemsetAsync(d_fb, 0, F, s));
         return decode_csr(ctx, code, d_llr, F, max_it, d_msg, d_iters, d_conv, d_synd);
     }
+    if (cv.dvb_q && (d_w || h_trace_c2v))
+        return fail(ctx, GF2L_ERR_UNSUPPORTED, "weights and traces are not supported for DVB codes");
     const int path = resolve_path(ctx, cv);
     if (path < 0) return fail(ctx, GF2L_ERR_UNSUPPORTED, "the forced decoder path cannot run this code");
     if (path == gf2l::kLds) {
@@ -141,6 +153,10 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
         ~Free() { hipFree(a); hipFree(b); }
     } free_trace{d_tc, d_tb};
     GF2L_HIP(ctx, gf2l::launch_hbm_init(cv, ws, path, d_llr, F, s));
+    if (cv.dvb_q) {  // the kernels read LLRs in the code's quasi-cyclic order
+        GF2L_HIP(ctx, gf2l::launch_dvb_permute(cv, d_llr, ws.llr_perm, F, s));
+        d_llr = ws.llr_perm;
+    }
     uint32_t running = F;  // frames entering the current iteration
     auto collect = [&]() -> int {  // stream already synchronized
         for (int k = 0; k < 3; ++k) {
@@ -269,6 +285,94 @@ int decode_host(gf2l_ctx *ctx, const gf2l_code *code, const float *llr, size_t f
 
 }  // namespace
 
+namespace gf2l {
+
+int build_qc_view(gf2l_ctx *ctx, gf2l_code *c, uint32_t mb, uint32_t nb, uint32_t z, uint32_t dvb_q) {
+    uint32_t max_dc = 0, max_dv = 0;
+    for (uint32_t br = 0; br < mb; ++br) max_dc = std::max(max_dc, c->h_row_ptr[br + 1] - c->h_row_ptr[br]);
+    // column order: block-rows ascending (the CSC order of every lifted
+    // variable); DVB: standard check index br + q*((z - shift) mod z) of the
+    // column's lane 0 ascending
+    c->h_col_ptr.assign(nb + 1, 0);
+    std::vector<std::vector<std::pair<uint64_t, uint4>>> cols(nb);
+    for (uint32_t br = 0; br < mb; ++br)
+        for (uint32_t e = c->h_row_ptr[br]; e < c->h_row_ptr[br + 1]; ++e) {
+            const uint2 re = c->h_row_edge[e];
+            const uint64_t key = dvb_q ? br + (uint64_t)dvb_q * ((z - re.y) % z) : ((uint64_t)br << 32) | e;
+            cols[re.x].push_back({key, make_uint4(br, e - c->h_row_ptr[br], re.y, e)});
+        }
+    for (uint32_t bc = 0; bc < nb; ++bc) {
+        std::stable_sort(cols[bc].begin(), cols[bc].end(),
+                         [](const std::pair<uint64_t, uint4> &a, const std::pair<uint64_t, uint4> &b) {
+                             return a.first < b.first;
+                         });
+        for (const auto &kv : cols[bc]) {
+            c->h_col_edge.push_back(make_uint2(kv.second.w, kv.second.z));
+            c->h_col_info.push_back(kv.second);
+        }
+        c->h_col_ptr[bc + 1] = (uint32_t)c->h_col_edge.size();
+        max_dv = std::max(max_dv, c->h_col_ptr[bc + 1] - c->h_col_ptr[bc]);
+    }
+    if (max_dc > gf2l::kMaxCheckDegree) {
+        delete c;
+        return fail(ctx, GF2L_ERR_UNSUPPORTED, "check degree above 32");
+    }
+    const uint32_t nbe = (uint32_t)c->h_row_edge.size();
+    const uint32_t zp = ((z + 31) / 32) * 32;
+    std::vector<uint2> col_pack(nbe);  // {first state record of the block-row, shift << 5 | position}
+    for (uint32_t q = 0; q < nbe; ++q) {
+        const uint4 ci = c->h_col_info[q];
+        col_pack[q] = make_uint2(ci.x * zp, (ci.z << 5) | ci.y);
+    }
+    const size_t b_rp = (mb + 1) * 4, b_cp = (nb + 1) * 4, b_e = (size_t)std::max(nbe, 1u) * 8,
+                 b_ci = (size_t)std::max(nbe, 1u) * 16;
+    auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    const size_t o_re = al(b_rp), o_cp = al(o_re + b_e), o_ce = al(o_cp + b_cp), o_ci = al(o_ce + b_e),
+                 o_pk = al(o_ci + b_ci);
+    const size_t total = o_pk + b_e;
+    std::vector<char> host(total, 0);
+    std::memcpy(host.data(), c->h_row_ptr.data(), b_rp);
+    if (nbe) std::memcpy(host.data() + o_re, c->h_row_edge.data(), nbe * 8);
+    std::memcpy(host.data() + o_cp, c->h_col_ptr.data(), b_cp);
+    if (nbe) std::memcpy(host.data() + o_ce, c->h_col_edge.data(), nbe * 8);
+    if (nbe) std::memcpy(host.data() + o_ci, c->h_col_info.data(), nbe * 16);
+    if (nbe) std::memcpy(host.data() + o_pk, col_pack.data(), nbe * 8);
+    hipError_t e = hipMalloc(&c->d_tables, total);
+    if (e == hipSuccess) e = hipMemcpy(c->d_tables, host.data(), total, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(c->d_tables);
+        delete c;
+        return fail(ctx, GF2L_ERR_HIP, std::string("code upload: ") + hipGetErrorString(e));
+    }
+    char *d = static_cast<char *>(c->d_tables);
+    gf2l::CodeView &v = c->view;
+    v.mb = mb;
+    v.nb = nb;
+    v.z = z;
+    v.n = nb * z;
+    v.m = mb * z;
+    v.k = v.n > v.m ? v.n - v.m : 0;
+    v.nbe = nbe;
+    v.max_dc = max_dc;
+    v.max_dv = max_dv;
+    v.zw = (z + 63) / 64;
+    v.row_ptr = reinterpret_cast<const uint32_t *>(d);
+    v.row_edge = reinterpret_cast<const uint2 *>(d + o_re);
+    v.col_ptr = reinterpret_cast<const uint32_t *>(d + o_cp);
+    v.col_edge = reinterpret_cast<const uint2 *>(d + o_ce);
+    v.col_info = reinterpret_cast<const uint4 *>(d + o_ci);
+    v.col_pack = reinterpret_cast<const uint2 *>(d + o_pk);
+    v.zp = zp;
+    if (dvb_q) {
+        v.dvb_q = dvb_q;
+        v.dvb_kb = nb - mb;
+        v.wrap_bc = nb - 1;
+    }
+    return GF2L_OK;
+}
+
+}  // namespace gf2l
+
 extern "C" {
 
 int gf2l_create(int device, gf2l_ctx **out) {
@@ -385,7 +489,7 @@ int gf2l_copy_to_host(gf2l_ctx *ctx, void *dst, const void *src, size_t bytes) {
 // identity blocks exactly at block-rows j and j+1.
 static bool is_staircase_ira(const gf2l_code *c) {
     const uint32_t mb = c->view.mb, nb = c->view.nb;
-    if (nb <= mb) return false;
+    if (nb <= mb || c->shifts.size() != (size_t)mb * nb) return false;
     const uint32_t kb = nb - mb;
     for (uint32_t br = 0; br < mb; ++br)
         for (uint32_t j = 0; j < mb; ++j) {
@@ -460,76 +564,15 @@ int gf2l_code_qc(gf2l_ctx *ctx, uint32_t mb, uint32_t nb, uint32_t z, const int3
     c->device = ctx->device;
     c->shifts.assign(shifts, shifts + (size_t)mb * nb);
     c->h_row_ptr.assign(mb + 1, 0);
-    uint32_t max_dc = 0, max_dv = 0;
     for (uint32_t br = 0; br < mb; ++br) {
         for (uint32_t bc = 0; bc < nb; ++bc) {
             const int32_t s = shifts[(size_t)br * nb + bc];
             if (s >= 0) c->h_row_edge.push_back(make_uint2(bc, (uint32_t)s));
         }
         c->h_row_ptr[br + 1] = (uint32_t)c->h_row_edge.size();
-        max_dc = std::max(max_dc, c->h_row_ptr[br + 1] - c->h_row_ptr[br]);
     }
-    c->h_col_ptr.assign(nb + 1, 0);
-    for (uint32_t bc = 0; bc < nb; ++bc) {
-        for (uint32_t br = 0; br < mb; ++br)
-            for (uint32_t e = c->h_row_ptr[br]; e < c->h_row_ptr[br + 1]; ++e)
-                if (c->h_row_edge[e].x == bc) {
-                    c->h_col_edge.push_back(make_uint2(e, c->h_row_edge[e].y));
-                    c->h_col_info.push_back(make_uint4(br, e - c->h_row_ptr[br], c->h_row_edge[e].y, e));
-                }
-        c->h_col_ptr[bc + 1] = (uint32_t)c->h_col_edge.size();
-        max_dv = std::max(max_dv, c->h_col_ptr[bc + 1] - c->h_col_ptr[bc]);
-    }
-    if (max_dc > gf2l::kMaxCheckDegree) {
-        delete c;
-        return fail(ctx, GF2L_ERR_UNSUPPORTED, "check degree above 32");
-    }
-    const uint32_t nbe = (uint32_t)c->h_row_edge.size();
-    const uint32_t zp = ((z + 31) / 32) * 32;
-    std::vector<uint2> col_pack(nbe);  // {first state record of the block-row, shift << 5 | position}
-    for (uint32_t q = 0; q < nbe; ++q) {
-        const uint4 ci = c->h_col_info[q];
-        col_pack[q] = make_uint2(ci.x * zp, (ci.z << 5) | ci.y);
-    }
-    const size_t b_rp = (mb + 1) * 4, b_cp = (nb + 1) * 4, b_e = (size_t)std::max(nbe, 1u) * 8,
-                 b_ci = (size_t)std::max(nbe, 1u) * 16;
-    auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
-    const size_t o_re = al(b_rp), o_cp = al(o_re + b_e), o_ce = al(o_cp + b_cp), o_ci = al(o_ce + b_e),
-                 o_pk = al(o_ci + b_ci);
-    const size_t total = o_pk + b_e;
-    std::vector<char> host(total, 0);
-    std::memcpy(host.data(), c->h_row_ptr.data(), b_rp);
-    if (nbe) std::memcpy(host.data() + o_re, c->h_row_edge.data(), nbe * 8);
-    std::memcpy(host.data() + o_cp, c->h_col_ptr.data(), b_cp);
-    if (nbe) std::memcpy(host.data() + o_ce, c->h_col_edge.data(), nbe * 8);
-    if (nbe) std::memcpy(host.data() + o_ci, c->h_col_info.data(), nbe * 16);
-    if (nbe) std::memcpy(host.data() + o_pk, col_pack.data(), nbe * 8);
-    hipError_t e = hipMalloc(&c->d_tables, total);
-    if (e == hipSuccess) e = hipMemcpy(c->d_tables, host.data(), total, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        hipFree(c->d_tables);
-        delete c;
-        return fail(ctx, GF2L_ERR_HIP, std::string("code upload: ") + hipGetErrorString(e));
-    }
-    char *d = static_cast<char *>(c->d_tables);
-    gf2l::CodeView &v = c->view;
-    v.mb = mb;
-    v.nb = nb;
-    v.z = z;
-    v.n = nb * z;
-    v.m = mb * z;
-    v.k = v.n > v.m ? v.n - v.m : 0;
-    v.nbe = nbe;
-    v.max_dc = max_dc;
-    v.max_dv = max_dv;
-    v.zw = (z + 63) / 64;
-    v.row_ptr = reinterpret_cast<const uint32_t *>(d);
-    v.row_edge = reinterpret_cast<const uint2 *>(d + o_re);
-    v.col_ptr = reinterpret_cast<const uint32_t *>(d + o_cp);
-    v.col_edge = reinterpret_cast<const uint2 *>(d + o_ce);
-    v.col_info = reinterpret_cast<const uint4 *>(d + o_ci);
-    v.col_pack = reinterpret_cast<const uint2 *>(d + o_pk);
-    v.zp = zp;
+    const int rc = gf2l::build_qc_view(ctx, c, mb, nb, z, 0);
+    if (rc != GF2L_OK) return rc;
     *out = c;
     return GF2L_OK;
 }
@@ -539,7 +582,9 @@ int gf2l_code_info(const gf2l_code *code, uint32_t *n, uint32_t *m, uint32_t *k,
     if (n) *n = code->view.n;
     if (m) *m = code->view.m;
     if (k) *k = code->view.k;
-    if (nnz) *nnz = code->is_csr ? (uint64_t)code->csr.nnz : (uint64_t)code->view.nbe * code->view.z;
+    if (nnz)
+        *nnz = code->is_csr ? (uint64_t)code->csr.nnz
+                            : (uint64_t)code->view.nbe * code->view.z - (code->view.dvb_q ? 1u : 0u);
     return GF2L_OK;
 }
 

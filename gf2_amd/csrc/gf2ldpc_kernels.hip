@@ -13,6 +13,7 @@
 //           edge (any check degree up to 32).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "device_common.h"
@@ -61,7 +62,7 @@ struct XcdSplit {
 // sched + (kind * 8 + xcd) * kCtrStride (kind 0: check, 1: variable kernel);
 // otherwise a fixed stride.  Each kernel clears the other kind's counters
 // after its items, for the kernel that follows it.
-template <int MAXD, bool FIRST, bool NMS, bool DYN>
+template <int MAXD, bool FIRST, bool NMS, bool DYN, bool DVB = false>
 __global__ void __launch_bounds__(512)
 cn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
                 const uint32_t *state_in, uint32_t *state_out, const float *__restrict__ belief,
@@ -73,7 +74,7 @@ cn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *
         const uint32_t f = list[xs.list_index(j, cv.mb)];
         const uint32_t br = j % cv.mb;
         const float *fb = FIRST ? llr + (size_t)f * cv.n : belief + (size_t)f * cv.nb * cv.zp;
-        cn_state_item<MAXD, FIRST, NMS>(cv, br, state_in + f * fstride, state_out + f * fstride, fb, w_prev);
+        cn_state_item<MAXD, FIRST, NMS, DVB>(cv, br, state_in + f * fstride, state_out + f * fstride, fb, w_prev);
     };
     if constexpr (DYN) {
         __shared__ uint32_t s_next[2];
@@ -90,7 +91,7 @@ cn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *
 }
 
 // Variable update of one block-column of one frame per item.
-template <int MAXD, bool NMS, bool DYN>
+template <int MAXD, bool NMS, bool DYN, bool DVB = false>
 __global__ void __launch_bounds__(512)
 vn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
                 const uint32_t *__restrict__ state, float *__restrict__ belief,
@@ -102,7 +103,7 @@ vn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *
     auto item = [&](uint32_t jj) {
         const uint32_t f = list[xs.list_index(jj, cv.nb)];
         const uint32_t bc = jj % cv.nb;
-        vn_state_item<MAXD, NMS>(cv, bc, state + f * fstride, llr + (size_t)f * cv.n + (size_t)bc * cv.z,
+        vn_state_item<MAXD, NMS, DVB>(cv, bc, state + f * fstride, llr + (size_t)f * cv.n + (size_t)bc * cv.z,
                                  belief + ((size_t)f * cv.nb + bc) * cv.zp,
                                  hb ? hb + ((size_t)f * cv.nb + bc) * cv.zw : nullptr, w_cur);
     };
@@ -487,7 +488,32 @@ hipError_t launch_lds_t(const CodeView &cv, const float *llr, uint32_t frames, u
     return hipGetLastError();
 }
 
+// LLRs in the standard's bit order -> quasi-cyclic order of a DVB code:
+// parity bit k + a + q*b moves to k + a*z + b (a < q, b < z).
+__global__ void dvb_permute_kernel(CodeView cv, const float *__restrict__ llr, float *__restrict__ out,
+                                   uint32_t frames) {
+    const size_t total = (size_t)frames * cv.n;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t f = t / cv.n;
+        const uint32_t u = (uint32_t)(t - f * cv.n);
+        uint32_t v = u;
+        if (u >= cv.k) {
+            const uint32_t w = u - cv.k, a = w / cv.z, b = w - a * cv.z;
+            v = cv.k + a + cv.dvb_q * b;
+        }
+        out[t] = llr[f * cv.n + v];
+    }
+}
+
 }  // namespace
+
+hipError_t launch_dvb_permute(const CodeView &cv, const float *llr, float *out, uint32_t frames, hipStream_t s) {
+    if (frames == 0) return hipSuccess;
+    const size_t total = (size_t)frames * cv.n;
+    const uint32_t grid = (uint32_t)std::min<size_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(dvb_permute_kernel, dim3(grid), dim3(256), 0, s, cv, llr, out, frames);
+    return hipGetLastError();
+}
 
 size_t lds_path_bytes(const CodeView &cv) {
     return ((size_t)cv.nbe * cv.z + 2 * (size_t)cv.n) * sizeof(float);
@@ -549,7 +575,28 @@ hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, con
     if (ev) hipEventRecord(ev[0], s);
     const bool nms_prev = a.w_prev != nullptr, nms_cur = a.w_cur != nullptr;
     const bool wide_c = cv.max_dc > 16, wide_v = cv.max_dv > 16;
-    if (path == kState) {
+    if (path == kState && cv.dvb_q) {  // DVB codes: plain min-sum, lane rules of the standard's order
+#define GF2L_CN_DYN(MD, FI, UNUSED, DY)                                                            \
+    hipLaunchKernelGGL((cn_state_kernel<MD, FI, false, DY, true>), dim3(g_cn), dim3(block), 0, s, cv, list, \
+                       count, ws.state, ws.state, ws.belief, a.llr, nullptr, xcd, sched, chunk)
+#define GF2L_CN(MD, FI, NM) \
+    do { if (dyn) GF2L_CN_DYN(MD, FI, NM, true); else GF2L_CN_DYN(MD, FI, NM, false); } while (0)
+        if (first)
+            GF2L_STATE_CLASS(cv.max_dc, GF2L_CN, true, false);
+        else
+            GF2L_STATE_CLASS(cv.max_dc, GF2L_CN, false, false);
+#undef GF2L_CN
+#undef GF2L_CN_DYN
+        if (ev) hipEventRecord(ev[1], s);
+#define GF2L_VN_DYN(MD, UNUSED, DY)                                                                \
+    hipLaunchKernelGGL((vn_state_kernel<MD, false, DY, true>), dim3(g_vn), dim3(block), 0, s, cv, list, count, \
+                       ws.state, ws.belief, a.llr, ws.hb, next_count, nullptr, xcd, sched, chunk)
+#define GF2L_VN(MD, NM, UNUSED) \
+    do { if (dyn) GF2L_VN_DYN(MD, NM, true); else GF2L_VN_DYN(MD, NM, false); } while (0)
+        GF2L_VAR_CLASS(cv.max_dv, GF2L_VN, false, 0);
+#undef GF2L_VN
+#undef GF2L_VN_DYN
+    } else if (path == kState) {
 #define GF2L_CN_DYN(MD, FI, NM, DY)                                                                \
     hipLaunchKernelGGL((cn_state_kernel<MD, FI, NM, DY>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
                        ws.state, ws.state, ws.belief, a.llr, a.w_prev, xcd, sched, chunk)

@@ -36,6 +36,11 @@ struct CodeView {
     const uint2 *col_edge;       // nbe: {row-order edge id, shift}
     const uint4 *col_info;       // nbe: {block-row, position in row, shift, row-order edge id}
     const uint2 *col_pack;       // nbe: {block-row * zp, shift << 5 | position} (z < 2^27)
+    // DVB-T2/S2 codes in quasi-cyclic order (check c -> (c mod q, c div q),
+    // parity bit k+t likewise): the staircase block of block-row 0 that has no
+    // edge at lane 0 (~0u for every other code), kb = k/z, q.
+    uint32_t wrap_bc = 0xffffffffu;
+    uint32_t dvb_kb = 0, dvb_q = 0;
 };
 
 // Device view of a general sparse code (LdpcCode::from_edges): CSR rows with
@@ -63,6 +68,8 @@ struct Workspace {
     uint32_t *lists = nullptr;   // [2][frames] running frame lists
     uint32_t *counts = nullptr;  // [0..1] running counts; item counters from kSchedOffset
     uint32_t *h_count = nullptr; // pinned host mirror of one count
+    float *llr_perm = nullptr;   // DVB codes: [frames][n] LLRs in quasi-cyclic order
+    size_t cap_llr = 0;
 };
 
 // Accumulated per-kernel timings of the HBM paths (HIP events recorded
@@ -160,6 +167,14 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
                   uint32_t max_it, const float *d_w, uint32_t fb_it, uint64_t *d_msg,
                   uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb,
                   float *h_trace_c2v, float *h_trace_bel, uint32_t trace_iters);
+
+// Uploads the tables of a quasi-cyclic code whose block-row lists
+// (h_row_ptr, h_row_edge) are filled and sets its view.  dvb_q > 0: DVB
+// order (each block-column's edges by ascending standard check index of its
+// lane 0; block-column kb+q-1's edge in block-row 0 has no lane-0 edge).
+int build_qc_view(gf2l_ctx *ctx, gf2l_code *c, uint32_t mb, uint32_t nb, uint32_t z, uint32_t dvb_q);
+// LLRs [frames][n] in the standard's order -> quasi-cyclic order of a DVB code.
+hipError_t launch_dvb_permute(const CodeView &cv, const float *llr, float *out, uint32_t frames, hipStream_t s);
 
 // Weights [iterations][nbe] (device), weighted iterations and code of a
 // learned decoder (nms.hip).

@@ -738,7 +738,8 @@ int gf2l_codeset_create(gf2l_ctx *ctx, const gf2l_code *const *codes, uint32_t c
     for (uint32_t c = 0; c < count; ++c) {
         const gf2l_code *code = codes[c];
         if (!code) return fail(ctx, GF2L_ERR_INVALID, "null code in a code set");
-        if (code->is_csr) return fail(ctx, GF2L_ERR_UNSUPPORTED, "code sets hold quasi-cyclic codes");
+        if (code->is_csr || code->view.dvb_q)
+            return fail(ctx, GF2L_ERR_UNSUPPORTED, "code sets hold quasi-cyclic codes (not DVB)");
         if (code->device != ctx->device) return fail(ctx, GF2L_ERR_INVALID, "code built on another device");
         if (code->view.max_dc > gf2l::kMaxStateDegree)
             return fail(ctx, GF2L_ERR_UNSUPPORTED, "check degree above 27 in a code set");

@@ -368,7 +368,8 @@ int gf2l_nms_create(gf2l_ctx *ctx, const gf2l_code *code, uint32_t iterations,
     if (!ctx || !out) return GF2L_ERR_INVALID;
     *out = nullptr;
     if (!code || iterations == 0) return fail(ctx, GF2L_ERR_INVALID, "need a code and iterations >= 1");
-    if (code->is_csr) return fail(ctx, GF2L_ERR_UNSUPPORTED, "learned weights need a quasi-cyclic code");
+    if (code->is_csr || code->view.dvb_q)
+        return fail(ctx, GF2L_ERR_UNSUPPORTED, "learned weights need a quasi-cyclic code (not DVB)");
     GF2L_HIP(ctx, hipSetDevice(ctx->device));
     gf2l_nms *q = new gf2l_nms;
     q->code = code;

@@ -1,4 +1,4 @@
-"""DVB-T2 decode throughput on the GPU (CSR path, bit-exact min-sum).
+"""DVB-T2 decode throughput on the GPU (bit-exact min-sum, quasi-cyclic kernels).
 
 Two workloads per code:
   clean   the reference's own benchmark input (crates/gf2-coding/benches/
@@ -91,6 +91,7 @@ def main(argv=None) -> int:
     ap.add_argument("--ebn0", type=float, default=None)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--csr", action="store_true", help="also time the CSR path on the awgn frames")
     a = ap.parse_args(argv)
     ctx = Context(0)
     for frame, rate in parse_codes(a.codes):
@@ -105,6 +106,13 @@ def main(argv=None) -> int:
         noisy = awgn_llr(cw, ebn0, spec.k / spec.n, rng)
         del cw
         line = {"code": spec.name, "n": spec.n, "k": spec.k, "nnz": code.nnz(), "frames": F}
+        if a.csr:  # the same code on the general sparse path, for comparison
+            d = DeviceBuffer.from_array(ctx, noisy)
+            csr = LdpcCode.from_edges(spec.m, spec.n, spec.edges(), ctx)
+            dt, mit, bad = time_device(ctx, csr, d, F, 50, a.reps)
+            line["awgn_csr_path_mbps"] = round(spec.k * F / dt / 1e6, 1)
+            d.free()
+            del csr
         for name, llr, max_it in (("clean", clean, 50), ("awgn", noisy, 50)):
             d = DeviceBuffer.from_array(ctx, llr)
             dt, mit, bad = time_device(ctx, code, d, F, max_it, a.reps)

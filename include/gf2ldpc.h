@@ -159,7 +159,11 @@ int gf2l_dvb_t2_table(int frame, int rate, uint32_t *row_len, uint32_t *addresse
  * Validation as validate_table (builder.rs:15-41): rows == k / z, every
  * address < m, no single-entry placeholder table -- GF2L_ERR_INVALID with
  * the reference's panic text otherwise.  Any table of this shape works
- * (DVB-S2 uses the same construction); the code runs on the CSR path. */
+ * (DVB-S2 uses the same construction).  When m = q*z (and no address
+ * repeats within a row) the code is decoded by the state kernels in a
+ * renumbered quasi-cyclic form (check a + q*b -> lane b of block-row a,
+ * parity bit likewise), bit-exact with the standard numbering; otherwise by
+ * the CSR path.  DVB codes take plain min-sum only. */
 int gf2l_code_dvb_table(gf2l_ctx *ctx, uint32_t n, uint32_t k, uint32_t q, uint32_t z, uint32_t rows,
                         const uint32_t *row_len, const uint32_t *addresses, gf2l_code **out);
 /* LdpcCode::dvb_t2_short / LdpcCode::dvb_t2_normal (crates/gf2-coding/src/

@@ -44,14 +44,57 @@ def test_dvb_short_codes_match_oracle(ctx, rate):
     same(decode_batch_arrays(code, llr, 50), oc.decode_batch(llr, 50, threads=8), spec.name)
 
 
-@pytest.mark.parametrize("rate", [0, 1])
+@pytest.mark.parametrize("rate", range(6))
 def test_dvb_normal_codes_match_oracle(ctx, rate):
     spec = codes.dvb_t2(1, rate)
     print(f"[dvb] {spec.name}", flush=True)
     code = LdpcCode.dvb_t2_normal(CodeRate(rate), ctx)
     oc = oracle_code(spec)
-    _, _, llr = dvb_random_frames(spec.table, spec.n, spec.k, spec.q, 24, [1.0, 1.5][rate], seed=7)
+    assert code.nnz() == oc.nnz
+    # around each code's min-sum waterfall: converging and failing frames
+    ebn0 = [1.55, 2.2, 2.0, 2.45, 2.85, 3.15][rate]
+    _, _, llr = dvb_random_frames(spec.table, spec.n, spec.k, spec.q, 24, ebn0, seed=7 + rate)
     same(decode_batch_arrays(code, llr, 30), oc.decode_batch(llr, 30, threads=8), spec.name)
+
+
+@pytest.mark.parametrize("frame,rate", [(0, 1), (0, 3), (0, 4), (0, 5), (1, 1), (1, 5)])
+def test_dvb_signed_zero_and_infinite_llrs(ctx, frame, rate):
+    """-0.0 inputs make the min-sum sign rule depend on each input's place in
+    the check's variable order, which inside a block of several circulants
+    changes from lane to lane; +-inf inputs saturate."""
+    spec = codes.dvb_t2(frame, rate)
+    print(f"[dvb +-0] {spec.name}", flush=True)
+    code = (LdpcCode.dvb_t2_normal if frame else LdpcCode.dvb_t2_short)(CodeRate(rate), ctx)
+    rng = np.random.default_rng(100 + 10 * frame + rate)
+    _, _, llr = dvb_random_frames(spec.table, spec.n, spec.k, spec.q, 16, 3.5, seed=60 + rate)
+    u = rng.random(llr.shape)
+    llr[u < 0.10] = -0.0
+    llr[(u >= 0.10) & (u < 0.15)] = 0.0
+    llr[(u >= 0.15) & (u < 0.16)] = np.inf
+    llr[(u >= 0.16) & (u < 0.17)] = -np.inf
+    assert np.signbit(llr[llr == 0]).sum() > 0
+    same(decode_batch_arrays(code, llr, 20), oracle_code(spec).decode_batch(llr, 20, threads=8), spec.name)
+
+
+def test_dvb_quasi_cyclic_form_limits(ctx):
+    from gf2_amd.ldpc import decode_trace_arrays
+    from gf2_amd.mixed import CodeSet
+    from gf2_amd.nms import NmsDecoder
+    code = LdpcCode.dvb_t2_short(CodeRate.RATE_2_3, ctx)
+    llr = np.full((2, code.n()), 4.0, dtype=np.float32)
+    with pytest.raises(_lib.Gf2lError, match="not DVB"):
+        NmsDecoder(code, 10)
+    with pytest.raises(_lib.Gf2lError, match="not DVB"):
+        CodeSet([code])
+    with pytest.raises(_lib.Gf2lError, match="DVB"):
+        decode_trace_arrays(code, llr, 5, 2)
+    try:
+        ctx.set_path(_lib.PATH_STREAM)
+        with pytest.raises(_lib.Gf2lError, match="forced decoder path"):
+            decode_batch_arrays(code, llr, 5)
+    finally:
+        ctx.set_path(_lib.PATH_AUTO)
+    assert decode_batch_arrays(code, llr, 5)[2].all()
 
 
 def test_dvb_reference_kats(ctx):
