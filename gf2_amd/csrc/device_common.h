@@ -37,6 +37,8 @@ struct CheckAcc {
         sb = 0;
         lt = 0;
     }
+    // a < min1 moves min1 to min2; otherwise a < min2 replaces min2 (strict
+    // comparisons, so the first of equal magnitudes keeps idx).
     __device__ void add(uint32_t p, float x) {
         float a = fabsf(x);
         if (a < min1) {
@@ -46,6 +48,17 @@ struct CheckAcc {
         } else if (a < min2) {
             min2 = a;
         }
+        sb |= (uint64_t)(__float_as_uint(x) >> 31) << p;
+        lt |= (uint64_t)(x < 0.0f) << p;
+    }
+    // The same update as selects instead of branches: cheaper to issue for
+    // small degrees, but it keeps more values live for large ones.
+    __device__ void add_sel(uint32_t p, float x) {
+        const float a = fabsf(x);
+        const bool lt1 = a < min1, lt2 = a < min2;
+        min2 = lt1 ? min1 : (lt2 ? a : min2);
+        min1 = lt1 ? a : min1;
+        idx = lt1 ? p : idx;
         sb |= (uint64_t)(__float_as_uint(x) >> 31) << p;
         lt |= (uint64_t)(x < 0.0f) << p;
     }
@@ -83,6 +96,13 @@ struct CheckAcc {
 __device__ __forceinline__ float recon(float m1, float m2, uint32_t meta, uint32_t p) {
     const float mag = (p == (meta & 31u)) ? m2 : m1;
     return ((meta >> (5 + p)) & 1u) ? -mag : mag;
+}
+// The same value with the sign bit or-ed in: the magnitudes carry no sign
+// bit (they are |x| minima), so output sign bit 5 + p of meta moves to bit
+// 31 (p < 27).  One instruction fewer; used where it costs no registers.
+__device__ __forceinline__ float recon_or(float m1, float m2, uint32_t meta, uint32_t p) {
+    const float mag = (p == (meta & 31u)) ? m2 : m1;
+    return __uint_as_float(__float_as_uint(mag) | ((meta << (26u - p)) & 0x80000000u));
 }
 
 __device__ __forceinline__ uint32_t wrap_add(uint32_t a, uint32_t b, uint32_t z) {
@@ -122,6 +142,11 @@ __device__ __forceinline__ T ldu(const T *base, uint32_t idx) {
 template <typename T>
 __device__ __forceinline__ void stu(T *base, uint32_t idx, T v) {
     *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(T)) = v;
+}
+// Same with the byte offset given (scaling folded into wave-uniform terms).
+template <typename T>
+__device__ __forceinline__ T ldb(const T *base, uint32_t off) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
 }
 
 // Calls f(std::integral_constant<int, d>) for a wave-uniform degree d in
@@ -258,18 +283,40 @@ __device__ __forceinline__ void cn_state_item(const CodeView &cv, uint32_t br, c
             re[p] = make_uint2(uni(r.x), uni(r.y));
             if constexpr (NMS && !FIRST) wp[p] = __uint_as_float(uni(__float_as_uint(ldc(w_prev + e0 + p))));
         }
-        for (uint32_t i = threadIdx.x; i < cv.z; i += blockDim.x) {
-            float xv[DA];
+        // small degrees: byte offset of belief (i + s) mod z of block-column c
+        // precomputed as 4*(c*stride + s) + 4i, less 4z once i >= z - s (two
+        // more scalar registers per edge, which larger degrees cannot spare)
+        constexpr bool kPre = MAXD <= 8;
+        uint32_t boff[kPre ? DA : 1], bthr[kPre ? DA : 1];
+        if constexpr (kPre) {
 #pragma unroll
-            for (int p = 0; p < D; ++p) xv[p] = ldu(fb, re[p].x * bstride + wrap_add(i, re[p].y, cv.z));
+            for (int p = 0; p < D; ++p) {
+                boff[p] = 4u * (re[p].x * bstride + re[p].y);
+                bthr[p] = cv.z - re[p].y;
+            }
+        }
+        const uint32_t z4 = 4u * cv.z;
+        for (uint32_t i = threadIdx.x; i < cv.z; i += blockDim.x) {
+            const uint32_t i4 = 4u * i;
             float m1o = 0.0f, m2o = 0.0f;
             uint32_t mo = 0;
-            if constexpr (!FIRST) {
+            if constexpr (!FIRST) {  // first: the old state is needed by every edge
                 const uint3 o = ldu(si, i);
                 m1o = __uint_as_float(o.x);
                 m2o = __uint_as_float(o.y);
                 mo = o.z;
             }
+            float xv[DA];
+#pragma unroll
+            for (int p = 0; p < D; ++p) {  // belief (i + shift) mod z of the slot's block-column
+                if constexpr (kPre)
+                    xv[p] = ldb(fb, i4 + boff[p] - (i >= bthr[p] ? z4 : 0u));
+                else
+                    xv[p] = ldu(fb, re[p].x * bstride + wrap_add(i, re[p].y, cv.z));
+            }
+            // small degrees: every load of the item issued before the first use
+            // (larger ones keep the compiler's interleaving, which needs fewer registers)
+            if constexpr (MAXD <= 8) __builtin_amdgcn_sched_barrier(0);
             // DVB: block-row 0 has no staircase edge at lane 0 (check 0 of
             // the standard code); `skip` is that slot, D when every slot is live
             uint32_t skip = D;
@@ -287,11 +334,14 @@ __device__ __forceinline__ void cn_state_item(const CodeView &cv, uint32_t br, c
                 if (DVB && (uint32_t)p == skip) continue;
                 float v = xv[p];
                 if constexpr (!FIRST) {
-                    float c = recon(m1o, m2o, mo, p);
+                    float c = recon_or(m1o, m2o, mo, p);
                     if constexpr (NMS) c = wp[p] * c;
                     v = v - c;
                 }
-                acc.add(p, v);
+                if constexpr (kPre)
+                    acc.add_sel(p, v);
+                else
+                    acc.add(p, v);
             }
             float m1 = acc.min1, m2 = acc.min2;
             uint32_t meta;
