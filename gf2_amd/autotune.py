@@ -42,7 +42,7 @@ from .device import DeviceBuffer, generate_frames
 from .ldpc import Context, LdpcCode
 
 CANDIDATES: Dict[str, List[int]] = {
-    "wg_per_cu": [2, 3, 4, 6],
+    "wg_per_cu": [2, 3, 4, 5, 6],
     "xcd_map": [1, 0],
     "graph_iters": [2, 4, 8],
     "chunk": [0, 4, 8, 16],

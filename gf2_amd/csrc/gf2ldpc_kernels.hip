@@ -90,9 +90,11 @@ cn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *
     }
 }
 
-// Variable update of one block-column of one frame per item.
+// Variable update of one block-column of one frame per item.  The smallest
+// degree class is compiled for 8 waves per SIMD (it fits in 78 scalar and
+// 56 vector registers; the default allocation of scalars capped it at 7).
 template <int MAXD, bool NMS, bool DYN, bool DVB = false>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MAXD <= 8 ? 8 : 1)))
 vn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
                 const uint32_t *__restrict__ state, float *__restrict__ belief,
                 const float *__restrict__ llr, uint64_t *__restrict__ hb,
@@ -564,7 +566,9 @@ hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, con
     const bool dyn = ch > 0;
     uint32_t *sched = ws.counts + kSchedOffset;
     const uint32_t chunk = dyn ? (uint32_t)ch : 1u;
-    const int wg = tune.wg_per_cu ? tune.wg_per_cu : (dyn ? 4 : 3);
+    // counter-fed items go with the smallest degree class, whose variable kernel
+    // is built for 8 waves per SIMD: 5 workgroups of 6 waves fit a CU
+    const int wg = tune.wg_per_cu ? tune.wg_per_cu : (dyn ? 5 : 3);
     const uint32_t block = round_block(cv.z);
     const uint32_t g_cn = hbm_grid(a.frames * cv.mb, num_cus, wg);
     const uint32_t g_vn = hbm_grid(a.frames * cv.nb, num_cus, wg);

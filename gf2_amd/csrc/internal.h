@@ -85,7 +85,7 @@ struct Profile {
 // Launch-shape tunables of the HBM paths (set by gf2l_set_tuning or the
 // autotuner).
 struct Tuning {
-    int wg_per_cu = 0;   // persistent workgroups per compute unit (0: 4 with counter-fed items, else 3)
+    int wg_per_cu = 0;   // persistent workgroups per compute unit (0: 5 with counter-fed items, else 3)
     int xcd_map = 1;     // 1: frames pinned to XCDs (L2 reuse), 0: frame-major over the grid
     int graph_iters = 4; // iterations per graph replay of the mixed-code decoder
     int chunk = -1;      // items per counter fetch of the check-state kernels (0: fixed stride, -1: automatic)

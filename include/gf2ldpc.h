@@ -87,7 +87,7 @@ int gf2l_set_path(gf2l_ctx *ctx, int path);
 
 /* Launch-shape tunables of the HBM paths (results never depend on them).
  *   GF2L_TUNE_WG_PER_CU  persistent workgroups per compute unit (1..32); 0 (default):
- *                        4 with counter-fed items (GF2L_TUNE_CHUNK), else 3
+ *                        5 with counter-fed items (GF2L_TUNE_CHUNK), else 3
  *   GF2L_TUNE_XCD_MAP    1 (default): each frame's block-rows / block-columns
  *                        run on one XCD so its beliefs and check states are
  *                        re-read from that XCD's L2; 0: frame-major over the grid
