@@ -53,6 +53,9 @@ SIGNATURES = {
     "gf2l_frames_generate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                             ctypes.c_uint64, ctypes.c_size_t, ctypes.c_float,
                                             ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]),
+    "gf2l_frames_generate_zero": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                                 ctypes.c_uint64, ctypes.c_size_t, ctypes.c_float,
+                                                 ctypes.c_float, ctypes.c_void_p]),
     "gf2l_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "gf2l_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "gf2l_profile_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,

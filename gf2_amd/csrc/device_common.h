@@ -149,6 +149,16 @@ __device__ __forceinline__ T ldb(const T *base, uint32_t off) {
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
 }
 
+// Waves per SIMD the state kernels of a degree class are compiled for
+// (amdgpu_waves_per_eu; 1 = no constraint).  The compiler's default
+// allocation gave the class-8 variable kernel 106 scalar registers, which
+// caps it at 7 waves; compiled for 8 it fits in 78 scalar and 56 vector
+// registers without spills.  Class 16 also fits 8 (check) and 7 (variable)
+// waves, but measured slower on the mixed-code stream, so it keeps the
+// compiler's choice.
+template <int MAXD> constexpr int kCheckWaves = MAXD <= 8 ? 8 : 1;
+template <int MAXD> constexpr int kVarWaves = MAXD <= 8 ? 8 : 1;
+
 // Calls f(std::integral_constant<int, d>) for a wave-uniform degree d in
 // [0, MAXD], so per-edge loops are fully unrolled with no guards and every
 // load of an item is issued before the first wait.  Degree 0 (a variable or

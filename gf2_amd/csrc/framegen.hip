@@ -145,6 +145,34 @@ framegen_kernel(CodeView cv, uint32_t seed_lo, uint32_t seed_hi, uint64_t first_
     }
 }
 
+// All-zero codeword frames of any code (the reference's FER example sends
+// all-zero codewords, crates/gf2-coding/examples/ldpc_awgn.rs): the same
+// Eb/N0 and noise streams as framegen_kernel with every bit 0, so x = +1.
+__global__ void __launch_bounds__(256)
+zero_framegen_kernel(uint32_t n, double rate, uint32_t seed_lo, uint32_t seed_hi, uint64_t first_frame,
+                     uint32_t frames, float ebn0_lo, float ebn0_hi, float *__restrict__ llr) {
+    for (uint32_t f = blockIdx.x; f < frames; f += gridDim.x) {
+        const uint64_t g = first_frame + f;
+        const uint32_t g_lo = (uint32_t)g, g_hi = (uint32_t)(g >> 32);
+        uint32_t o[4];
+        philox(0u, g_lo, g_hi, 2u, seed_lo, seed_hi, o);
+        const double u = ((double)o[0] + 0.5) * (1.0 / 4294967296.0);
+        const double ebn0 = (double)ebn0_lo + ((double)ebn0_hi - (double)ebn0_lo) * u;
+        const double s2 = 1.0 / (2.0 * rate * pow(10.0, ebn0 / 10.0));
+        const double sigma = sqrt(s2);
+        for (uint32_t q = threadIdx.x; 2 * q < n; q += blockDim.x) {
+            philox(q, g_lo, g_hi, 1u, seed_lo, seed_hi, o);
+            const double u1 = ((double)o[0] + 0.5) * (1.0 / 4294967296.0);
+            const double u2 = ((double)o[1] + 0.5) * (1.0 / 4294967296.0);
+            const double r = sqrt(-2.0 * log(u1));
+            const double th = 6.283185307179586476925286766559 * u2;
+            const uint32_t v0 = 2 * q;
+            llr[(size_t)f * n + v0] = (float)(2.0 * (1.0 + sigma * (r * cos(th))) / s2);
+            if (v0 + 1 < n) llr[(size_t)f * n + v0 + 1] = (float)(2.0 * (1.0 + sigma * (r * sin(th))) / s2);
+        }
+    }
+}
+
 }  // namespace
 
 hipError_t launch_framegen(const CodeView &cv, uint64_t seed, uint64_t first_frame, uint32_t frames,
@@ -155,6 +183,15 @@ hipError_t launch_framegen(const CodeView &cv, uint64_t seed, uint64_t first_fra
     const size_t sh = (size_t)cv.nb * cv.zw * sizeof(uint64_t);
     hipLaunchKernelGGL(framegen_kernel, dim3(grid), dim3(256), sh, s, cv, (uint32_t)seed,
                        (uint32_t)(seed >> 32), first_frame, frames, ebn0_lo, ebn0_hi, llr, cw_words);
+    return hipGetLastError();
+}
+
+hipError_t launch_framegen_zero(uint32_t n, uint32_t k, uint64_t seed, uint64_t first_frame, uint32_t frames,
+                                float ebn0_lo, float ebn0_hi, float *llr, int num_cus, hipStream_t s) {
+    if (frames == 0) return hipSuccess;
+    const uint32_t grid = frames < (uint32_t)num_cus * 8u ? frames : (uint32_t)num_cus * 8u;
+    hipLaunchKernelGGL(zero_framegen_kernel, dim3(grid), dim3(256), 0, s, n, (double)k / (double)n,
+                       (uint32_t)seed, (uint32_t)(seed >> 32), first_frame, frames, ebn0_lo, ebn0_hi, llr);
     return hipGetLastError();
 }
 

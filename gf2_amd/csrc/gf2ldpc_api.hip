@@ -516,6 +516,19 @@ int gf2l_frames_generate(gf2l_ctx *ctx, const gf2l_code *code, uint64_t seed, ui
     return GF2L_OK;
 }
 
+int gf2l_frames_generate_zero(gf2l_ctx *ctx, const gf2l_code *code, uint64_t seed, uint64_t first_frame,
+                              size_t frames, float ebn0_lo_db, float ebn0_hi_db, float *d_llr) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    if (!code || (!d_llr && frames)) return fail(ctx, GF2L_ERR_INVALID, "null argument");
+    if (frames > 0xffffffffull) return fail(ctx, GF2L_ERR_INVALID, "too many frames");
+    const uint32_t n = code->view.n, k = code->view.k;
+    if (n == 0 || k == 0) return fail(ctx, GF2L_ERR_INVALID, "code rate must be positive");
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    GF2L_HIP(ctx, gf2l::launch_framegen_zero(n, k, seed, first_frame, (uint32_t)frames, ebn0_lo_db, ebn0_hi_db,
+                                             d_llr, ctx->num_cus, ctx->stream));
+    return GF2L_OK;
+}
+
 int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value) {
     if (!ctx) return GF2L_ERR_INVALID;
     switch (key) {

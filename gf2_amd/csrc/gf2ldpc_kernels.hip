@@ -63,7 +63,7 @@ struct XcdSplit {
 // otherwise a fixed stride.  Each kernel clears the other kind's counters
 // after its items, for the kernel that follows it.
 template <int MAXD, bool FIRST, bool NMS, bool DYN, bool DVB = false>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kCheckWaves<MAXD>)))
 cn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
                 const uint32_t *state_in, uint32_t *state_out, const float *__restrict__ belief,
                 const float *__restrict__ llr, const float *__restrict__ w_prev, int xcd,
@@ -90,11 +90,9 @@ cn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *
     }
 }
 
-// Variable update of one block-column of one frame per item.  The smallest
-// degree class is compiled for 8 waves per SIMD (it fits in 78 scalar and
-// 56 vector registers; the default allocation of scalars capped it at 7).
+// Variable update of one block-column of one frame per item.
 template <int MAXD, bool NMS, bool DYN, bool DVB = false>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MAXD <= 8 ? 8 : 1)))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kVarWaves<MAXD>)))
 vn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
                 const uint32_t *__restrict__ state, float *__restrict__ belief,
                 const float *__restrict__ llr, uint64_t *__restrict__ hb,

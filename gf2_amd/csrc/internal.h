@@ -207,6 +207,9 @@ hipError_t launch_state_forward(const CodeView &cv, const uint32_t *list, const 
 hipError_t launch_framegen(const CodeView &cv, uint64_t seed, uint64_t first_frame, uint32_t frames,
                            float ebn0_lo, float ebn0_hi, float *llr, uint64_t *cw_words, int num_cus,
                            hipStream_t s);
+// All-zero codeword frames of an n-bit, rate k/n code (framegen.hip).
+hipError_t launch_framegen_zero(uint32_t n, uint32_t k, uint64_t seed, uint64_t first_frame, uint32_t frames,
+                                float ebn0_lo, float ebn0_hi, float *llr, int num_cus, hipStream_t s);
 // Expands the state path's compressed checks and padded beliefs into the
 // explicit trace layout ([f][e*z+i] and [f][n]) for all `frames` frames.
 hipError_t launch_state_expand(const CodeView &cv, const Workspace &ws, uint32_t frames,

@@ -209,7 +209,7 @@ __device__ uint32_t build_prefix(const MixShared &m, uint32_t mask, const uint32
 // channel LLRs, no previous messages.  NMS: the previous messages were
 // weighted by iteration t-1's weights (t = this iteration, 1-based).
 template <int MAXD, bool FIRST, bool NMS, bool DYN>
-__global__ void __launch_bounds__(512) cn_mix_kernel(MixShared m, uint32_t mask, int xcd, uint32_t t, uint32_t slot,
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kCheckWaves<MAXD>))) cn_mix_kernel(MixShared m, uint32_t mask, int xcd, uint32_t t, uint32_t slot,
                                                      uint32_t chunk) {
     __shared__ uint32_t s_start[kMaxCodes + 1];
     const uint32_t cur = ldc(m.misc);
@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(512) cn_mix_kernel(MixShared m, uint32_t mask,
 
 // Variable update; NMS: messages weighted by iteration t's weights.
 template <int MAXD, bool NMS, bool DYN>
-__global__ void __launch_bounds__(512) vn_mix_kernel(MixShared m, uint32_t mask, int xcd, uint32_t t, uint32_t slot,
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kVarWaves<MAXD>))) vn_mix_kernel(MixShared m, uint32_t mask, int xcd, uint32_t t, uint32_t slot,
                                                      uint32_t chunk) {
     __shared__ uint32_t s_start[kMaxCodes + 1];
     const uint32_t cur = ldc(m.misc);

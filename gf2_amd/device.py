@@ -68,3 +68,17 @@ def generate_frames(code: LdpcCode, seed: int, first_frame: int, frames: int, eb
         cw.ptr if cw else None))
     ctx.synchronize()
     return llr, cw
+
+
+def generate_zero_frames(code: LdpcCode, seed: int, first_frame: int, frames: int, ebn0_lo: float,
+                         ebn0_hi: float = None) -> DeviceBuffer:
+    """All-zero codeword frames of any code on the device
+    (gf2l_frames_generate_zero).  Returns the LLR DeviceBuffer [frames, n]."""
+    ctx = code.context
+    if ebn0_hi is None:
+        ebn0_hi = ebn0_lo
+    llr = DeviceBuffer(ctx, frames * code.n() * 4)
+    _lib.check(ctx.handle, _lib.load().gf2l_frames_generate_zero(
+        ctx.handle, code.handle, seed, first_frame, frames, ebn0_lo, ebn0_hi, llr.ptr))
+    ctx.synchronize()
+    return llr

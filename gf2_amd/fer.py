@@ -13,6 +13,7 @@ LLRs), so a sweep never leaves the GPU except for the per-batch error count.
     python -m gf2_amd.fer --code n64800 --ebn0 1.5:2.5:0.25 --frames 4096 \\
         --max-frames 65536 --min-errors 50 --json out.json
     python -m gf2_amd.fer --code n16200-k9720 --target-fer 1e-2
+    python -m gf2_amd.fer --code dvb-normal-3/5 --ebn0 2.0:2.6:0.1     (all-zero codewords)
 
 A decoder is any callable (d_llr, frames, outs) -> None that fills the
 device outputs [msg words, iterations, converged, syndrome_ok]; plain BP and
@@ -30,7 +31,7 @@ from typing import Callable, List, Optional
 import numpy as np
 
 from . import _lib, codes
-from .device import DeviceBuffer, generate_frames
+from .device import DeviceBuffer, generate_frames, generate_zero_frames
 from .ldpc import Context, LdpcCode
 
 
@@ -78,9 +79,21 @@ def count_errors(msg: np.ndarray, cw: np.ndarray, conv: np.ndarray, synd: np.nda
 
 def sweep(code: LdpcCode, ebn0_list: List[float], frames_per_batch: int, max_frames: int,
           min_errors: int, decoder: Optional[Callable] = None, seed: int = 1, name: str = "",
-          log=sys.stderr) -> List[FerPoint]:
-    """FER/BER per Eb/N0: batches until max_frames or min_errors."""
+          log=sys.stderr, zero_codeword: Optional[bool] = None) -> List[FerPoint]:
+    """FER/BER per Eb/N0: batches until max_frames or min_errors.
+    zero_codeword: send all-zero codewords (the reference example's method,
+    examples/ldpc_awgn.rs:121-175; valid for any code) instead of encoded
+    random messages; None = encoded where the device encoder supports the
+    code (staircase QC-IRA), all-zero otherwise."""
     ctx = code.context
+    if zero_codeword is None:
+        try:
+            generate_frames(code, 0, 0, 1, 1.0, 1.0, with_codewords=False)
+            zero_codeword = False
+        except _lib.Gf2lError as e:
+            if e.code != _lib.GF2L_ERR_UNSUPPORTED:
+                raise
+            zero_codeword = True
     decoder = decoder or plain_decoder(code)
     n, k = code.n(), code.k()
     F = frames_per_batch
@@ -93,8 +106,11 @@ def sweep(code: LdpcCode, ebn0_list: List[float], frames_per_batch: int, max_fra
         t_dec = 0.0
         b = 0
         while nf < max_frames and fe < min_errors:
-            d_llr, d_cw = generate_frames(code, seed * 1000003 + int(round(ebn0 * 1000)), b * F, F,
-                                          ebn0, ebn0)
+            fseed = seed * 1000003 + int(round(ebn0 * 1000))
+            if zero_codeword:
+                d_llr, d_cw = generate_zero_frames(code, fseed, b * F, F, ebn0, ebn0), None
+            else:
+                d_llr, d_cw = generate_frames(code, fseed, b * F, F, ebn0, ebn0)
             t0 = time.perf_counter()
             decoder(d_llr, F, outs)
             ctx.synchronize()
@@ -103,7 +119,7 @@ def sweep(code: LdpcCode, ebn0_list: List[float], frames_per_batch: int, max_fra
             it = outs[1].download(np.uint32, F)
             conv = outs[2].download(np.uint8, F).astype(bool)
             synd = outs[3].download(np.uint8, F).astype(bool)
-            cw = d_cw.download(np.uint64, (F, nw))
+            cw = d_cw.download(np.uint64, (F, nw)) if d_cw is not None else np.zeros((F, nw), dtype=np.uint64)
             e, w = count_errors(msg, cw, conv, synd, k)
             nf += F
             fe += e
@@ -150,6 +166,21 @@ def spec_by_name(name: str) -> codes.QcIraSpec:
     raise ValueError(f"unknown code {name!r}")
 
 
+def code_by_name(name: str, ctx: Context):
+    """(label, LdpcCode): a QC-IRA name as spec_by_name, dvb-normal-<rate> /
+    dvb-short-<rate> (rate 1/2, 3/5, 2/3, 3/4, 4/5, 5/6) for a DVB-T2 code,
+    or the path of a GF2DATA parity-check matrix file."""
+    if name.startswith("dvb-"):
+        _, frame, rate = name.split("-", 2)
+        r = codes.DVB_RATE_NAMES.index(rate)
+        code = (LdpcCode.dvb_t2_normal if frame == "normal" else LdpcCode.dvb_t2_short)(r, ctx)
+        return codes.dvb_t2(1 if frame == "normal" else 0, r).name, code
+    if name.endswith(".gf2") or "/" in name:
+        return name, LdpcCode.from_gf2data(name, ctx)
+    spec = spec_by_name(name)
+    return spec.name, LdpcCode(spec.qc(), ctx)
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--code", default="n64800")
@@ -162,10 +193,11 @@ def main(argv=None) -> int:
     ap.add_argument("--target-fer", type=float, default=None,
                     help="find the Eb/N0 of this FER by bisection over the --ebn0 range ends")
     ap.add_argument("--json", default=None)
+    ap.add_argument("--zero", action="store_true",
+                    help="all-zero codewords (automatic for codes the device encoder does not cover)")
     a = ap.parse_args(argv)
-    spec = spec_by_name(a.code)
     ctx = Context(0)
-    code = LdpcCode(spec.qc(), ctx)
+    label, code = code_by_name(a.code, ctx)
     if a.weights:
         from .nms import NmsDecoder
         decoder = learned_decoder(NmsDecoder.load(code, a.weights), a.max_it)
@@ -174,12 +206,13 @@ def main(argv=None) -> int:
     if a.target_fer is not None:
         r = _parse_range(a.ebn0)
         e = calibrate(code, a.target_fer, min(r), max(r), a.frames, decoder=decoder)
-        print(json.dumps({"code": spec.name, "target_fer": a.target_fer, "ebn0_db": e}))
+        print(json.dumps({"code": label, "target_fer": a.target_fer, "ebn0_db": e}))
         return 0
-    pts = sweep(code, _parse_range(a.ebn0), a.frames, a.max_frames, a.min_errors, decoder, name=spec.name)
+    pts = sweep(code, _parse_range(a.ebn0), a.frames, a.max_frames, a.min_errors, decoder, name=label,
+                zero_codeword=True if a.zero else None)
     if a.json:
         with open(a.json, "w") as f:
-            json.dump({"code": spec.name, "points": [asdict(p) for p in pts]}, f, indent=1)
+            json.dump({"code": label, "points": [asdict(p) for p in pts]}, f, indent=1)
     return 0
 
 

@@ -209,6 +209,16 @@ int gf2l_frames_generate(gf2l_ctx *ctx, const gf2l_code *code, uint64_t seed, ui
                          size_t frames, float ebn0_lo_db, float ebn0_hi_db, float *d_llr,
                          uint64_t *d_codeword_words);
 
+/* All-zero codeword frames of ANY code (QC, DVB-T2, general sparse): the
+ * reference FER example's method (crates/gf2-coding/examples/
+ * ldpc_awgn.rs:121-175 sends all-zero codewords; a linear code over a
+ * symmetric channel with this decoder has the same FER for every codeword).
+ * Same Eb/N0 and noise streams as gf2l_frames_generate with every bit 0:
+ * LLR = 2(1 + sigma n)/sigma^2, sigma^2 = 1/(2 (k/n) 10^(EbN0/10)).  Writes
+ * frames*n floats to d_llr, queued on the context's stream. */
+int gf2l_frames_generate_zero(gf2l_ctx *ctx, const gf2l_code *code, uint64_t seed, uint64_t first_frame,
+                              size_t frames, float ebn0_lo_db, float ebn0_hi_db, float *d_llr);
+
 /* ---- decoding --------------------------------------------------------- */
 
 /* LdpcDecoder::decode_batch (core.rs:729-756) on host buffers.

@@ -491,3 +491,29 @@ void or_framegen(const int32_t *shifts, uint32_t mb, uint32_t nb, uint32_t z, ui
     }
     free(bit);
 }
+
+/* All-zero codeword frame g of an n-bit, rate k/n code: the Eb/N0 and noise
+ * streams of or_framegen with every bit 0 (x = +1), as the reference FER
+ * example transmits (crates/gf2-coding/examples/ldpc_awgn.rs:121-175). */
+void or_framegen_zero(uint32_t n, uint32_t k, uint64_t seed, uint64_t g, float ebn0_lo, float ebn0_hi,
+                      float *llr) {
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    const uint32_t c2[4] = {0, (uint32_t)g, (uint32_t)(g >> 32), 2};
+    or_philox4x32(c2, key, o);
+    const double rate = (double)k / (double)n;
+    const double u = ((double)o[0] + 0.5) * (1.0 / 4294967296.0);
+    const double ebn0 = (double)ebn0_lo + ((double)ebn0_hi - (double)ebn0_lo) * u;
+    const double s2 = 1.0 / (2.0 * rate * pow(10.0, ebn0 / 10.0));
+    const double sigma = sqrt(s2);
+    for (uint32_t v = 0; v < n; ++v) {
+        const uint32_t c1[4] = {v / 2, (uint32_t)g, (uint32_t)(g >> 32), 1};
+        or_philox4x32(c1, key, o);
+        const double u1 = ((double)o[0] + 0.5) * (1.0 / 4294967296.0);
+        const double u2 = ((double)o[1] + 0.5) * (1.0 / 4294967296.0);
+        const double r = sqrt(-2.0 * log(u1));
+        const double th = 6.283185307179586476925286766559 * u2;
+        const double nz = (v % 2 == 0) ? r * cos(th) : r * sin(th);
+        llr[v] = (float)(2.0 * (1.0 + sigma * nz) / s2);
+    }
+}

@@ -71,6 +71,9 @@ def load():
         lib.or_framegen.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                     ctypes.c_uint64, ctypes.c_uint64, ctypes.c_float,
                                     ctypes.c_float, vp, vp]
+        lib.or_framegen_zero.restype = None
+        lib.or_framegen_zero.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64,
+                                         ctypes.c_float, ctypes.c_float, vp]
         lib.port_code_new.restype = vp
         lib.port_code_new.argtypes = [ctypes.c_uint32, ctypes.c_uint32, vp, ctypes.c_size_t]
         lib.port_code_free.restype = None
@@ -212,6 +215,13 @@ def framegen(base: np.ndarray, z: int, seed: int, frame_id: int, ebn0_lo: float,
     cw = np.zeros((n + 63) // 64, dtype=np.uint64)
     load().or_framegen(_p(base), mb, nb, z, seed, frame_id, ebn0_lo, ebn0_hi, _p(llr), _p(cw))
     return llr, cw
+
+
+def framegen_zero(n: int, k: int, seed: int, frame_id: int, ebn0_lo: float, ebn0_hi: float) -> np.ndarray:
+    """LLRs [n] float32 of all-zero codeword frame frame_id."""
+    llr = np.zeros(n, dtype=np.float32)
+    load().or_framegen_zero(n, k, seed, frame_id, ebn0_lo, ebn0_hi, _p(llr))
+    return llr
 
 
 def minsum_avx2(x) -> np.float32:

@@ -180,3 +180,18 @@ def test_host_mirror_exposes_the_constructors():
     for name in ("dvb_t2_short", "dvb_t2_normal", "from_dvb_table", "from_gf2data", "from_edges"):
         assert callable(getattr(LdpcCode, name))
     assert [int(r) for r in CodeRate] == list(range(6))
+
+
+def test_oracle_zero_codeword_frames_share_the_noise_stream():
+    """All-zero codeword frames use framegen's Eb/N0 and noise streams: where
+    the encoded frame's bit is 0 the LLRs are identical, where it is 1 they
+    differ by exactly the +-1 of BPSK (2*2/sigma^2 before rounding)."""
+    spec = codes.n2304_r12()
+    base = spec.base_matrix()
+    for g in (0, 7, 123456789012):
+        llr, cw = orc.framegen(base, spec.z, 99, g, 1.0, 3.0)
+        zl = orc.framegen_zero(spec.n, spec.k, 99, g, 1.0, 3.0)
+        bits = np.unpackbits(cw.view(np.uint8), bitorder="little")[:spec.n].astype(bool)
+        assert np.array_equal(zl[~bits], llr[~bits])
+        d = (zl[bits].astype(np.float64) - llr[bits].astype(np.float64))
+        assert bits.any() and np.allclose(d, d[0], rtol=0, atol=1e-4 * abs(d[0]))
