@@ -126,28 +126,14 @@ int decode_csr(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, uint32_
     const CsrView &cv = code->csr;
     hipStream_t s = ctx->stream;
     const size_t nw = (cv.n + 63) / 64;
-    float *c2v = nullptr, *bel = nullptr;
-    uint64_t *hb = nullptr;
-    uint32_t *lists = nullptr, *counts = nullptr, *h_count = nullptr;
-    struct Free {
-        float **a, **b;
-        uint64_t **c;
-        uint32_t **d, **e, **hc;
-        ~Free() {
-            hipFree(*a);
-            hipFree(*b);
-            hipFree(*c);
-            hipFree(*d);
-            hipFree(*e);
-            if (*hc) hipHostFree(*hc);
-        }
-    } guard{&c2v, &bel, &hb, &lists, &counts, &h_count};
-    GF2L_HIP(ctx, hipMalloc(&c2v, (size_t)F * std::max<uint64_t>(cv.nnz, 1) * sizeof(float)));
-    GF2L_HIP(ctx, hipMalloc(&bel, (size_t)F * cv.n * sizeof(float)));
-    GF2L_HIP(ctx, hipMalloc(&hb, (size_t)F * nw * sizeof(uint64_t)));
-    GF2L_HIP(ctx, hipMalloc(&lists, 2ull * F * sizeof(uint32_t)));
-    GF2L_HIP(ctx, hipMalloc(&counts, 2 * sizeof(uint32_t)));
-    GF2L_HIP(ctx, hipHostMalloc((void **)&h_count, sizeof(uint32_t), hipHostMallocDefault));
+    // the context's reusable workspace: c2v [F][nnz], beliefs [F][n], hard bits [F][nw]
+    const int rc = ensure_buffers(ctx, F, (size_t)F * std::max<uint64_t>(cv.nnz, 1), 0, (size_t)F * cv.n,
+                                  (size_t)F * nw, 0);
+    if (rc != GF2L_OK) return rc;
+    Workspace &ws = ctx->ws;
+    float *c2v = ws.c2v, *bel = ws.belief;
+    uint64_t *hb = ws.hb;
+    uint32_t *lists = ws.lists, *counts = ws.counts, *h_count = ws.h_count;
     GF2L_HIP(ctx, hipMemcpyAsync(bel, d_llr, (size_t)F * cv.n * sizeof(float), hipMemcpyDeviceToDevice, s));
     GF2L_HIP(ctx, hipMemsetAsync(c2v, 0, (size_t)F * cv.nnz * sizeof(float), s));
     hipLaunchKernelGGL(csr_init_kernel, dim3((F + 255) / 256), dim3(256), 0, s, lists, counts, F);

@@ -27,38 +27,53 @@ void free_workspace(gf2l::Workspace &ws) {
     ws = gf2l::Workspace{};
 }
 
-// Grows the HBM-path workspace to hold `frames` frames of `code` on `path`.
-int ensure_workspace(gf2l_ctx *ctx, const gf2l::CodeView &cv, int path, size_t frames) {
+}  // namespace
+
+namespace gf2l {
+
+// Grows the context's workspace so every buffer holds at least the given
+// element counts (never shrinks; one reallocation of all buffers).
+int ensure_buffers(gf2l_ctx *ctx, size_t frames, size_t c2v, size_t state, size_t bel, size_t hb, size_t llr) {
     gf2l::Workspace &ws = ctx->ws;
-    const size_t need_c2v = path == gf2l::kStream ? (size_t)cv.nbe * cv.z * frames : 0;
-    const size_t need_state = path == gf2l::kState ? 3ull * cv.mb * cv.zp * frames : 0;
-    const size_t need_bel = (path == gf2l::kState ? (size_t)cv.nb * cv.zp : (size_t)cv.n) * frames;
-    const size_t need_hb = (size_t)cv.nb * cv.zw * frames;
-    const size_t need_llr = cv.dvb_q ? (size_t)cv.n * frames : 0;
-    if (ws.lists && ws.cap_frames >= frames && ws.cap_c2v >= need_c2v &&
-        ws.cap_state >= need_state && ws.cap_bel >= need_bel && ws.cap_hb >= need_hb &&
-        ws.cap_llr >= need_llr)
+    if (ws.lists && ws.cap_frames >= frames && ws.cap_c2v >= c2v && ws.cap_state >= state && ws.cap_bel >= bel &&
+        ws.cap_hb >= hb && ws.cap_llr >= llr)
         return GF2L_OK;
     const size_t f = std::max(frames, ws.cap_frames);
-    const size_t c2v = std::max(need_c2v, ws.cap_c2v), st = std::max(need_state, ws.cap_state),
-                 bel = std::max(need_bel, ws.cap_bel), hb = std::max(need_hb, ws.cap_hb),
-                 lp = std::max(need_llr, ws.cap_llr);
+    c2v = std::max(c2v, ws.cap_c2v);
+    state = std::max(state, ws.cap_state);
+    bel = std::max(bel, ws.cap_bel);
+    hb = std::max(hb, ws.cap_hb);
+    llr = std::max(llr, ws.cap_llr);
     free_workspace(ws);
-    if (lp) GF2L_HIP(ctx, hipMalloc(&ws.llr_perm, lp * sizeof(float)));
+    if (llr) GF2L_HIP(ctx, hipMalloc(&ws.llr_perm, llr * sizeof(float)));
     if (c2v) GF2L_HIP(ctx, hipMalloc(&ws.c2v, c2v * sizeof(float)));
-    if (st) GF2L_HIP(ctx, hipMalloc(&ws.state, st * sizeof(uint32_t)));
-    GF2L_HIP(ctx, hipMalloc(&ws.belief, bel * sizeof(float)));
-    GF2L_HIP(ctx, hipMalloc(&ws.hb, hb * sizeof(uint64_t)));
-    GF2L_HIP(ctx, hipMalloc(&ws.lists, 2 * f * sizeof(uint32_t)));
+    if (state) GF2L_HIP(ctx, hipMalloc(&ws.state, state * sizeof(uint32_t)));
+    GF2L_HIP(ctx, hipMalloc(&ws.belief, std::max<size_t>(bel, 1) * sizeof(float)));
+    GF2L_HIP(ctx, hipMalloc(&ws.hb, std::max<size_t>(hb, 1) * sizeof(uint64_t)));
+    GF2L_HIP(ctx, hipMalloc(&ws.lists, 2 * std::max<size_t>(f, 1) * sizeof(uint32_t)));
     GF2L_HIP(ctx, hipMalloc(&ws.counts, gf2l::kCountsWords * sizeof(uint32_t)));  // counts + item counters
     GF2L_HIP(ctx, hipHostMalloc((void **)&ws.h_count, sizeof(uint32_t), hipHostMallocDefault));
     ws.cap_frames = f;
     ws.cap_c2v = c2v;
-    ws.cap_state = st;
+    ws.cap_state = state;
     ws.cap_bel = bel;
     ws.cap_hb = hb;
-    ws.cap_llr = lp;
+    ws.cap_llr = llr;
     return GF2L_OK;
+}
+
+}  // namespace gf2l
+
+namespace {
+
+// Grows the HBM-path workspace to hold `frames` frames of `code` on `path`.
+int ensure_workspace(gf2l_ctx *ctx, const gf2l::CodeView &cv, int path, size_t frames) {
+    const size_t c2v = path == gf2l::kStream ? (size_t)cv.nbe * cv.z * frames : 0;
+    const size_t state = path == gf2l::kState ? 3ull * cv.mb * cv.zp * frames : 0;
+    const size_t bel = (path == gf2l::kState ? (size_t)cv.nb * cv.zp : (size_t)cv.n) * frames;
+    const size_t hb = (size_t)cv.nb * cv.zw * frames;
+    const size_t llr = cv.dvb_q ? (size_t)cv.n * frames : 0;
+    return gf2l::ensure_buffers(ctx, frames, c2v, state, bel, hb, llr);
 }
 
 // Resolves the decoder path for a code (GF2L_PATH_*), or -1 if the forced

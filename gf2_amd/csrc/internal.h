@@ -176,6 +176,11 @@ int build_qc_view(gf2l_ctx *ctx, gf2l_code *c, uint32_t mb, uint32_t nb, uint32_
 // LLRs [frames][n] in the standard's order -> quasi-cyclic order of a DVB code.
 hipError_t launch_dvb_permute(const CodeView &cv, const float *llr, float *out, uint32_t frames, hipStream_t s);
 
+// Grows the context's workspace to at least these element counts per buffer
+// (c2v floats, state words, beliefs, hard-decision words, permuted LLRs) and
+// `frames` running-list entries (gf2ldpc_api.hip).
+int ensure_buffers(gf2l_ctx *ctx, size_t frames, size_t c2v, size_t state, size_t bel, size_t hb, size_t llr);
+
 // Weights [iterations][nbe] (device), weighted iterations and code of a
 // learned decoder (nms.hip).
 const float *nms_weights(const gf2l_nms *q, uint32_t *iterations, const gf2l_code **code);
