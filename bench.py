@@ -100,7 +100,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(frames shard across GPUs with no exchange, see DESIGN.md)"}))
+                                   "(frames would shard across GPUs with no exchange between them)"}))
         return 2
     from gf2_amd import _lib, channel, codes
     from gf2_amd.device import DeviceBuffer, generate_frames

@@ -2,7 +2,7 @@
 
 Bar: bit-exact.  Decoded words, iteration counts, converged and syndrome
 flags must be identical to the oracle's, and so must every per-iteration
-check message and belief (the north-star floor of 1e-5 relative is
+check message and belief (a 1e-5 relative tolerance on the messages is
 tightened to equality because the decoder's arithmetic is exact or uses the
 same IEEE operations in the same order).
 """
