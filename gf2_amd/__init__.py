@@ -5,11 +5,11 @@ A drop-in for the reference's `LdpcDecoder::decode_batch` path
 hand-written HIP kernels in libgf2ldpc.so, reached through the C ABI in
 include/gf2ldpc.h; this package mirrors the reference API on top of it.
 """
-from .ldpc import (BitVec, Context, DecoderResult, LdpcCode, LdpcDecoder, QuasiCyclicLdpc,
+from .ldpc import (BitVec, CodeRate, Context, DecoderResult, LdpcCode, LdpcDecoder, QuasiCyclicLdpc,
                    decode_batch_arrays, decode_trace_arrays, default_context,
                    qc_ira_base_matrix)
 
 __all__ = [
-    "BitVec", "Context", "DecoderResult", "LdpcCode", "LdpcDecoder", "QuasiCyclicLdpc",
+    "BitVec", "CodeRate", "Context", "DecoderResult", "LdpcCode", "LdpcDecoder", "QuasiCyclicLdpc",
     "decode_batch_arrays", "decode_trace_arrays", "default_context", "qc_ira_base_matrix",
 ]
