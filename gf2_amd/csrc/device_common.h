@@ -392,6 +392,18 @@ __device__ __forceinline__ void vn_state_item(const CodeView &cv, uint32_t bc, c
             ci[q] = make_uint2(uni(r.x), uni(r.y));
             if constexpr (NMS) wq[q] = __uint_as_float(uni(__float_as_uint(ldc(w_cur + ldc(&cv.col_info[c0 + q].w)))));
         }
+        // DVB: per slot the lane below which its check lane does not wrap
+        // (its shift, or z for shift 0); whether every shift is 0
+        uint32_t thr[DVB ? DA : 1];
+        bool uni_all_zero_shift = true;
+        if constexpr (DVB) {
+#pragma unroll
+            for (int q = 0; q < D; ++q) {
+                const uint32_t sh = ci[q].y >> 5;
+                thr[q] = sh ? sh : cv.z;
+                uni_all_zero_shift = uni_all_zero_shift && sh == 0;
+            }
+        }
         for (uint32_t j0 = 0; j0 < cv.z; j0 += blockDim.x) {
             const uint32_t j = j0 + threadIdx.x;
             const bool valid = j < cv.z;
@@ -411,34 +423,47 @@ __device__ __forceinline__ void vn_state_item(const CodeView &cv, uint32_t bc, c
                     // Slots are in ascending standard check order at lane 0
                     // (check a + q*b of block-row a, lane b).  At lane j the
                     // order is that list rotated: slots whose check lane
-                    // (j + s) mod z wrapped come first, s = z - shift.
+                    // (j + s) mod z wrapped come first (s = z - shift, so a
+                    // slot stays in place while j < shift, or always for
+                    // shift 0).  Parity columns other than the last have
+                    // only shift-0 slots and keep their order (uniform test).
                     float c[DA];
-                    uint64_t live = 0;
-                    uint32_t r = 0;
 #pragma unroll
-                    for (int q = 0; q < D; ++q) {
-                        c[q] = recon(m1[q], m2[q], mt[q], ci[q].y & 31u);
-                        const uint32_t sh = ci[q].y >> 5, s = sh ? cv.z - sh : 0u;
-                        r += (s + j < cv.z) ? 1u : 0u;
-                        // the staircase has no edge from check 0 to the last parity bit
-                        const bool dead = bc == cv.wrap_bc && ci[q].x == 0 && j == cv.z - 1;
-                        live |= (uint64_t)(dead ? 0u : 1u) << q;
+                    for (int q = 0; q < D; ++q) c[q] = recon(m1[q], m2[q], mt[q], ci[q].y & 31u);
+                    if (uni_all_zero_shift) {
+#pragma unroll
+                        for (int q = 0; q < D; ++q) bel = bel + c[q];
+                    } else {
+                        uint32_t r = 0;
+#pragma unroll
+                        for (int q = 0; q < D; ++q) r += (j < thr[q]) ? 1u : 0u;
+#pragma unroll
+                        for (int k = 1; k < D; k <<= 1) {
+                            const bool rot = (r & (uint32_t)k) != 0;
+                            float t[DA];
+#pragma unroll
+                            for (int q = 0; q < D; ++q) t[q] = rot ? c[(q + k) % DA] : c[q];
+#pragma unroll
+                            for (int q = 0; q < D; ++q) c[q] = t[q];
+                        }
+                        if (bc == cv.wrap_bc) {
+                            // the staircase has no edge from check 0 to the last
+                            // parity bit (lane z-1, the block-row-0 slot)
+                            uint64_t live = 0;
+#pragma unroll
+                            for (int q = 0; q < D; ++q)
+                                live |= (uint64_t)((ci[q].x == 0 && j == cv.z - 1) ? 0u : 1u) << q;
+                            const uint64_t full = (D >= 64 ? ~0ull : (1ull << D) - 1);
+                            const uint32_t rr = r % (uint32_t)DA;
+                            live = ((live >> rr) | (live << ((DA - rr) % 64))) & full;
+#pragma unroll
+                            for (int q = 0; q < D; ++q)
+                                if ((live >> q) & 1ull) bel = bel + c[q];
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < D; ++q) bel = bel + c[q];
+                        }
                     }
-#pragma unroll
-                    for (int k = 1; k < D; k <<= 1) {
-                        const bool rot = (r & (uint32_t)k) != 0;
-                        float t[DA];
-#pragma unroll
-                        for (int q = 0; q < D; ++q) t[q] = rot ? c[(q + k) % DA] : c[q];
-#pragma unroll
-                        for (int q = 0; q < D; ++q) c[q] = t[q];
-                    }
-                    const uint64_t full = (D >= 64 ? ~0ull : (1ull << D) - 1);
-                    const uint32_t rr = r % (uint32_t)DA;
-                    live = ((live >> rr) | (live << ((DA - rr) % 64))) & full;
-#pragma unroll
-                    for (int q = 0; q < D; ++q)
-                        if ((live >> q) & 1ull) bel = bel + c[q];
                 } else {
 #pragma unroll
                     for (int q = 0; q < D; ++q) {
