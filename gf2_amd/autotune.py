@@ -38,7 +38,7 @@ from typing import Dict, Iterable, List, Optional, Sequence
 import numpy as np
 
 from . import _lib
-from .device import DeviceBuffer, generate_frames
+from .device import DeviceBuffer, generate_frames, generate_zero_frames
 from .ldpc import Context, LdpcCode
 
 CANDIDATES: Dict[str, List[int]] = {
@@ -60,8 +60,8 @@ def default_cache_path() -> str:
 def code_key(code: LdpcCode) -> str:
     """Shape signature of a code: sizes, lift and degree bounds."""
     qc = code.qc
-    if qc is None:  # general sparse code
-        return f"csr-n{code.n()}-m{code.m()}-e{code.nnz()}"
+    if qc is None:  # DVB-T2 or general sparse code
+        return f"sparse-n{code.n()}-m{code.m()}-e{code.nnz()}"
     base = qc.base_matrix
     deg_c = int((base >= 0).sum(axis=1).max())
     deg_v = int((base >= 0).sum(axis=0).max())
@@ -139,7 +139,12 @@ def tune_code(code: LdpcCode, frames: int, ebn0_db: float, max_it: int = 50, rep
     lib = _lib.load()
     F = frames
     kw = (code.k() + 63) // 64
-    d_llr, _ = generate_frames(code, seed, 0, F, ebn0_db, ebn0_db, with_codewords=False)
+    try:
+        d_llr, _ = generate_frames(code, seed, 0, F, ebn0_db, ebn0_db, with_codewords=False)
+    except _lib.Gf2lError as e:  # no device encoder for this code: all-zero codewords
+        if e.code != _lib.GF2L_ERR_UNSUPPORTED:
+            raise
+        d_llr = generate_zero_frames(code, seed, 0, F, ebn0_db, ebn0_db)
     outs = [DeviceBuffer(ctx, F * kw * 8), DeviceBuffer(ctx, F * 4), DeviceBuffer(ctx, F), DeviceBuffer(ctx, F)]
 
     def run():

@@ -49,3 +49,17 @@ def test_tune_code_set(ctx, tmp_path):
                               knobs=("wg_per_cu", "graph_iters"), cache=cache)
     assert len(r["results"]) == len(autotune.CANDIDATES["wg_per_cu"]) * len(autotune.CANDIDATES["graph_iters"])
     assert r["key"].startswith("mixed:") and cache.get(r["key"]) == r["config"]
+
+
+def test_tune_dvb_code(ctx, tmp_path):
+    """A DVB-T2 code has no device encoder (all-zero frames) and runs on the
+    state kernels only: the forced STREAM/LDS shapes are skipped, the rest
+    agree bit for bit."""
+    from gf2_amd.ldpc import CodeRate
+    code = LdpcCode.dvb_t2_short(CodeRate.RATE_2_3, ctx)
+    cache = autotune.TuningCache(str(tmp_path / "d.json"))
+    r = autotune.tune_code(code, 256, 2.3, max_it=20, repeats=1, knobs=("path", "chunk"), cache=cache)
+    paths = {cfg["path"] for cfg, _ in r["results"]}
+    assert paths == {_lib.PATH_AUTO, _lib.PATH_STATE}
+    assert len(r["results"]) == 2 * len(autotune.CANDIDATES["chunk"])
+    assert r["key"].startswith("single:sparse-n16200")
