@@ -4,6 +4,7 @@
     python -m gf2_amd.train_nms --code n64800  [--out gf2_amd/weights/nms_n64800_t10.npz]
     python -m gf2_amd.train_nms --code n16200-k9720   (any of the twelve Z=360 codes)
     python -m gf2_amd.train_nms --all-mixed   (one checkpoint per Z=360 code)
+    python -m gf2_amd.train_nms --code n64800 --iterations 50 --steps 200 --init 0.9
 
 Recipes (defaults per code):
   n2304   N=2304 K=1152 Z=96 QC-IRA: 1024 steps x 4096 frames = 2^22 frames,
@@ -54,19 +55,21 @@ def default_path(code: str, iterations: int = 10) -> str:
     return os.path.join(HERE, "weights", f"nms_{code}_t{iterations}.npz")
 
 
-def train_one(name: str, iterations: int, ctx: Context, seed: int, out: str = None, **override) -> str:
+def train_one(name: str, iterations: int, ctx: Context, seed: int, out: str = None, init: float = 1.0,
+              **override) -> str:
     r = recipe(name)
     r.update({k: v for k, v in override.items() if v is not None})
     spec = r["spec"]()
     code = LdpcCode(spec.qc(), ctx)
-    dec = NmsDecoder(code, iterations, init=1.0)
+    dec = NmsDecoder(code, iterations, init=init)
     t0 = time.perf_counter()
     losses = train(dec, r["steps"], r["batch"], r["lr"], r["lo"], r["hi"], seed=seed,
                    log_every=max(1, r["steps"] // 4))
     dt = time.perf_counter() - t0
     out = out or default_path(name, iterations)
     os.makedirs(os.path.dirname(out), exist_ok=True)
-    dec.save(out, meta={"code_name": spec.name, "recipe": {k: v for k, v in r.items() if k != "spec"},
+    dec.save(out, meta={"code_name": spec.name, "init": init,
+                        "recipe": {k: v for k, v in r.items() if k != "spec"},
                         "seed": seed, "final_loss": float(losses[-10:].mean()), "train_seconds": dt})
     print(f"[nms-train] {spec.name}: {r['steps']} steps x {r['batch']} frames in {dt:.1f}s, "
           f"loss {losses[:10].mean():.4f} -> {losses[-10:].mean():.4f}; saved {out}", flush=True)
@@ -82,6 +85,7 @@ def main(argv=None) -> int:
     ap.add_argument("--batch", type=int)
     ap.add_argument("--lr", type=float)
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--init", type=float, default=1.0, help="initial value of every weight")
     ap.add_argument("--out")
     ap.add_argument("--out-dir", help="directory for the checkpoints (default gf2_amd/weights)")
     a = ap.parse_args(argv)
@@ -91,7 +95,7 @@ def main(argv=None) -> int:
         out = a.out if not a.all_mixed else None
         if a.out_dir:
             out = os.path.join(a.out_dir, os.path.basename(default_path(name, a.iterations)))
-        train_one(name, a.iterations, ctx, a.seed, out, steps=a.steps, batch=a.batch, lr=a.lr)
+        train_one(name, a.iterations, ctx, a.seed, out, init=a.init, steps=a.steps, batch=a.batch, lr=a.lr)
     return 0
 
 
