@@ -13,6 +13,8 @@ Two decoders are timed on the same batch:
            failing the syndrome re-decoded by plain 50-iteration BP  -> value
   plain    flooding min-sum BP, 50 iterations, bit-exact with the reference
            decoder                                                  -> config
+and, when its checkpoint exists, a 50-iteration weighted network with the
+same fallback (config.deeper_learned), which converges most frames itself.
 The JSON line carries
   roofline      achieved HBM bandwidth of the dominant kernel of the learned
                 run, from HIP events recorded by the library around every
@@ -59,6 +61,8 @@ def parse():
     ap.add_argument("--ebn0", type=float, default=DEFAULT_EBN0)
     ap.add_argument("--max-it", type=int, default=50)
     ap.add_argument("--weights", default=os.path.join(ROOT, "gf2_amd", "weights", "nms_n64800_t10.npz"))
+    ap.add_argument("--deep-weights", default=os.path.join(ROOT, "gf2_amd", "weights", "nms_n64800_t50.npz"),
+                    help="a deeper learned decoder timed beside the headline one ('' to skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=7)
@@ -163,6 +167,22 @@ def main():
         return dt, it, fe, fb, prof
 
     dt_p, it_p, fe_p, _, _ = run(plain_step, False)
+    deep = None
+    if a.deep_weights and os.path.exists(a.deep_weights):
+        # a deeper weighted network with the same fallback, reported beside
+        # the specified 10-iteration one (not the headline value)
+        deep_dec = NmsDecoder.load(code, a.deep_weights)
+
+        def deep_step():
+            deep_dec.decode_device(d_llr, F, a.max_it, *outs)
+            ctx.synchronize()
+
+        dt_d, it_d, fe_d, fb_d, _ = run(deep_step, False)
+        deep = {"weights": os.path.relpath(a.deep_weights, ROOT), "weighted_iterations": deep_dec.iterations,
+                "gbps": round(a.steps * F * spec.k / dt_d / 1e9, 4), "fer": fe_d / F,
+                "mean_iterations": round(float(it_d.mean()), 3),
+                "fallback_fraction": round(float(fb_d.mean()), 4),
+                "over_plain": round(dt_p / dt_d, 4)}
     dt_l, it_l, fe_l, fb_l, (ms, launches, frame_iters) = run(learned_step, True)
     gbps_p = a.steps * F * spec.k / dt_p / 1e9
     gbps_l = a.steps * F * spec.k / dt_l / 1e9
@@ -206,6 +226,7 @@ def main():
             "plain_bp_ms_per_step": round(dt_p / a.steps * 1e3, 3),
             "learned_over_plain": round(gbps_l / gbps_p, 4),
             "frames": F, "ebn0_db": a.ebn0, "max_iterations": a.max_it,
+            "deeper_learned": deep,
         },
         "roofline": {
             "bound": "hbm", "kernel": kernels[dom],
