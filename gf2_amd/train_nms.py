@@ -4,7 +4,8 @@
     python -m gf2_amd.train_nms --code n64800  [--out gf2_amd/weights/nms_n64800_t10.npz]
     python -m gf2_amd.train_nms --code n16200-k9720   (any of the twelve Z=360 codes)
     python -m gf2_amd.train_nms --all-mixed   (one checkpoint per Z=360 code)
-    python -m gf2_amd.train_nms --code n64800 --iterations 50 --steps 200 --init 0.9
+    python -m gf2_amd.train_nms --code n64800 --iterations 50 --steps 200 --init 0.9 \
+        --lr 3e-3 --lo 1.4 --hi 1.8
 
 Recipes (defaults per code):
   n2304   N=2304 K=1152 Z=96 QC-IRA: 1024 steps x 4096 frames = 2^22 frames,
@@ -84,6 +85,8 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int)
     ap.add_argument("--batch", type=int)
     ap.add_argument("--lr", type=float)
+    ap.add_argument("--lo", type=float, help="lowest training Eb/N0 (dB)")
+    ap.add_argument("--hi", type=float, help="highest training Eb/N0 (dB)")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--init", type=float, default=1.0, help="initial value of every weight")
     ap.add_argument("--out")
@@ -95,7 +98,8 @@ def main(argv=None) -> int:
         out = a.out if not a.all_mixed else None
         if a.out_dir:
             out = os.path.join(a.out_dir, os.path.basename(default_path(name, a.iterations)))
-        train_one(name, a.iterations, ctx, a.seed, out, init=a.init, steps=a.steps, batch=a.batch, lr=a.lr)
+        train_one(name, a.iterations, ctx, a.seed, out, init=a.init, steps=a.steps, batch=a.batch, lr=a.lr,
+                  lo=a.lo, hi=a.hi)
     return 0
 
 
