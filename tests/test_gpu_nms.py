@@ -197,3 +197,32 @@ def test_training_gain_over_untrained(ctx, n2304):
     say("fer", fer)
     # measured with the full 2^22-frame recipe: 0.152 vs 0.207 at 2.5 dB
     assert fer["trained"] < 0.85 * fer["untrained"], fer
+
+
+def test_shipped_n64800_t50_beats_plain_bp(ctx):
+    """The shipped 50-iteration N=64800 checkpoint with plain-BP fallback
+    converges in fewer iterations than plain min-sum (all weights 1 for 50
+    iterations) on the same frames at 1.55 dB, with no more frame errors."""
+    from gf2_amd.device import DeviceBuffer
+    from gf2_amd.fer import count_errors
+    spec = codes.n64800_r12()
+    code = LdpcCode(spec.qc(), ctx)
+    path = os.path.join(os.path.dirname(_lib.__file__), "weights", "nms_n64800_t50.npz")
+    F, kw, nw = 1024, (spec.k + 63) // 64, (spec.n + 63) // 64
+    d_llr, d_cw = generate_frames(code, 7, 0, F, 1.55, 1.55)
+    cw = d_cw.download(np.uint64, (F, nw))
+    res = {}
+    for name, dec, fb_it in [("learned", NmsDecoder.load(code, path), 50),
+                             ("plain", NmsDecoder(code, 50, init=1.0), 0)]:
+        say("n64800 t50 vs plain:", name)
+        outs = [DeviceBuffer(ctx, F * kw * 8), DeviceBuffer(ctx, F * 4), DeviceBuffer(ctx, F),
+                DeviceBuffer(ctx, F), DeviceBuffer(ctx, F)]
+        dec.decode_device(d_llr, F, fb_it, *outs)
+        conv = outs[2].download(np.uint8, F).astype(bool)
+        synd = outs[3].download(np.uint8, F).astype(bool)
+        fe, _ = count_errors(outs[0].download(np.uint64, (F, kw)), cw, conv, synd, spec.k)
+        res[name] = (fe, float(outs[1].download(np.uint32, F).mean()))
+    say("frame errors, mean iterations", res)
+    # measured on 4096 frames: 0 vs 161 errors, 18.2 vs 30.8 iterations
+    assert res["learned"][0] <= res["plain"][0], res
+    assert res["learned"][1] < 0.75 * res["plain"][1], res
