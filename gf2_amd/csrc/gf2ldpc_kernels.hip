@@ -216,9 +216,14 @@ vn_stream_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t 
 }
 
 // ---------------------------------------------------------------------------
-// Syndrome + retirement, shared by both HBM paths: one wave per frame.
+// Syndrome + retirement, shared by both HBM paths: one workgroup of
+// kSynThreads per frame (the check-row words are independent, so four waves
+// share a frame's rows and the pass is not latency-bound on one wave).
 
-__global__ void __launch_bounds__(64)
+constexpr uint32_t kSynThreads = 256;
+constexpr uint32_t kRetired = 0xffffffffu;  // next_list slot of a frame that stopped
+
+__global__ void __launch_bounds__(kSynThreads)
 syn_words_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
                  uint32_t *__restrict__ next_list, uint32_t *__restrict__ next_count,
                  const uint64_t *__restrict__ hb, uint32_t it_report, int last, int fb_phase,
@@ -239,11 +244,44 @@ syn_words_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t 
                 synd[f] = (uint8_t)!unsat;
                 if (fb) fb[f] = (uint8_t)fb_phase;
             }
-        } else if (threadIdx.x == 0) {
-            next_list[atomicAdd(next_count, 1u)] = f;
         }
+        // the slot is kept in place (retired frames leave a hole) so that
+        // list_compact_kernel keeps the survivors in list order
+        if (threadIdx.x == 0) next_list[item] = (!unsat || last) ? kRetired : f;
         __syncthreads();
     }
+}
+
+// Removes the holes the syndrome pass left in next_list, in place and in
+// order (every write lands at or before the slot it came from, and a chunk's
+// writes start only after the chunk has been read), and stores the survivor
+// count.  One workgroup: the list holds at most one slot per frame.
+__global__ void __launch_bounds__(1024)
+list_compact_kernel(const uint32_t *__restrict__ count_ptr, uint32_t *__restrict__ next_list,
+                    uint32_t *__restrict__ next_count) {
+    __shared__ uint32_t wave_kept[16];
+    const uint32_t count = *count_ptr;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t waves = blockDim.x >> 6;
+    uint32_t base = 0;
+    for (uint32_t c0 = 0; c0 < count; c0 += blockDim.x) {
+        const uint32_t i = c0 + threadIdx.x;
+        const uint32_t v = i < count ? next_list[i] : kRetired;
+        const bool keep = v != kRetired;
+        const uint64_t kept = __ballot(keep);
+        if (lane == 0) wave_kept[wave] = (uint32_t)__popcll(kept);
+        __syncthreads();
+        uint32_t off = base, total = 0;
+        for (uint32_t w = 0; w < waves; ++w) {
+            const uint32_t n = wave_kept[w];
+            if (w < wave) off += n;
+            total += n;
+        }
+        if (keep) next_list[off + (uint32_t)__popcll(kept & ((1ull << lane) - 1ull))] = v;
+        base += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *next_count = base;
 }
 
 // counts: [0..1] running-list counts; from kSchedOffset the check-kernel
@@ -646,9 +684,10 @@ hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, con
     }
     if (ev) hipEventRecord(ev[2], s);
     const size_t sh = (size_t)cv.nb * cv.zw * sizeof(uint64_t);
-    hipLaunchKernelGGL(syn_words_kernel, dim3(g_syn), dim3(64), sh, s, cv, list, count, next_list,
+    hipLaunchKernelGGL(syn_words_kernel, dim3(g_syn), dim3(kSynThreads), sh, s, cv, list, count, next_list,
                        next_count, ws.hb, a.it_report, a.last, a.fb_phase, a.msg, a.iters, a.conv,
                        a.synd, a.fb);
+    hipLaunchKernelGGL(list_compact_kernel, dim3(1), dim3(1024), 0, s, count, next_list, next_count);
     if (ev) hipEventRecord(ev[3], s);
     return hipGetLastError();
 }
