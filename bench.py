@@ -46,7 +46,7 @@ HBM_PEAK_GBPS = 8000.0
 # dispatches, all 1024 frames running) and scaled by the counts of a plain
 # copy of known size (bytes read = 2 x FETCH_SIZE, written = 1 x WRITE_SIZE):
 # profiles/n64800_state_kernels_bench.md.
-MEASURED_TRAFFIC_PER_FI = {"check": 828_172 + 389_200, "variable": 1_172_202 + 271_595}
+MEASURED_TRAFFIC_PER_FI = {"check": 827_374 + 389_200, "variable": 1_173_078 + 271_543}
 # Plain min-sum FER on this code is about 1e-3 here: the gf2_amd.fer sweep
 # measured 2.1e-3 at 1.60 dB (16384 frames) and 0 errors in 16384 at 1.65 dB.
 DEFAULT_EBN0 = 1.62
