@@ -488,6 +488,40 @@ __device__ __forceinline__ void vn_state_item(const CodeView &cv, uint32_t bc, c
 
 // Bits pos .. pos+63 (cyclically mod z) of a z-bit vector stored in zw
 // words whose bits at z and above are zero.  Needs z >= 64.
+// Slot value the syndrome pass leaves in the next running list for a frame
+// that stopped.
+constexpr uint32_t kRetiredSlot = 0xffffffffu;
+
+// Removes the kRetiredSlot holes from list[0, count) in place, keeping the
+// other slots in order, and returns how many remain.  Called by a whole
+// workgroup (blockDim.x a multiple of 64, at most 1024; wave_kept holds
+// blockDim.x / 64 entries): every write lands at or before the slot it came
+// from, and a chunk's writes start only after the whole chunk has been read.
+__device__ __forceinline__ uint32_t compact_list(uint32_t *__restrict__ list, uint32_t count,
+                                                 uint32_t *wave_kept) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t waves = blockDim.x >> 6;
+    uint32_t base = 0;
+    for (uint32_t c0 = 0; c0 < count; c0 += blockDim.x) {
+        const uint32_t i = c0 + threadIdx.x;
+        const uint32_t v = i < count ? list[i] : kRetiredSlot;
+        const bool keep = v != kRetiredSlot;
+        const uint64_t kept = __ballot(keep);
+        if (lane == 0) wave_kept[wave] = (uint32_t)__popcll(kept);
+        __syncthreads();
+        uint32_t off = base, total = 0;
+        for (uint32_t w = 0; w < waves; ++w) {
+            const uint32_t n = wave_kept[w];
+            if (w < wave) off += n;
+            total += n;
+        }
+        if (keep) list[off + (uint32_t)__popcll(kept & ((1ull << lane) - 1ull))] = v;
+        base += total;
+        __syncthreads();
+    }
+    return base;
+}
+
 __device__ __forceinline__ uint64_t cyc64(const uint64_t *v, uint32_t z, uint32_t zw, uint32_t pos) {
     const uint32_t wi = pos >> 6, sh = pos & 63u;
     uint64_t r = v[wi] >> sh;

@@ -221,7 +221,6 @@ vn_stream_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t 
 // share a frame's rows and the pass is not latency-bound on one wave).
 
 constexpr uint32_t kSynThreads = 256;
-constexpr uint32_t kRetired = 0xffffffffu;  // next_list slot of a frame that stopped
 
 __global__ void __launch_bounds__(kSynThreads)
 syn_words_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
@@ -247,41 +246,20 @@ syn_words_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t 
         }
         // the slot is kept in place (retired frames leave a hole) so that
         // list_compact_kernel keeps the survivors in list order
-        if (threadIdx.x == 0) next_list[item] = (!unsat || last) ? kRetired : f;
+        if (threadIdx.x == 0) next_list[item] = (!unsat || last) ? kRetiredSlot : f;
         __syncthreads();
     }
 }
 
-// Removes the holes the syndrome pass left in next_list, in place and in
-// order (every write lands at or before the slot it came from, and a chunk's
-// writes start only after the chunk has been read), and stores the survivor
-// count.  One workgroup: the list holds at most one slot per frame.
+// Closes the holes the syndrome pass left in next_list (survivors stay in
+// list order) and stores the survivor count.  One workgroup: the list holds
+// at most one slot per frame.
 __global__ void __launch_bounds__(1024)
 list_compact_kernel(const uint32_t *__restrict__ count_ptr, uint32_t *__restrict__ next_list,
                     uint32_t *__restrict__ next_count) {
     __shared__ uint32_t wave_kept[16];
-    const uint32_t count = *count_ptr;
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t waves = blockDim.x >> 6;
-    uint32_t base = 0;
-    for (uint32_t c0 = 0; c0 < count; c0 += blockDim.x) {
-        const uint32_t i = c0 + threadIdx.x;
-        const uint32_t v = i < count ? next_list[i] : kRetired;
-        const bool keep = v != kRetired;
-        const uint64_t kept = __ballot(keep);
-        if (lane == 0) wave_kept[wave] = (uint32_t)__popcll(kept);
-        __syncthreads();
-        uint32_t off = base, total = 0;
-        for (uint32_t w = 0; w < waves; ++w) {
-            const uint32_t n = wave_kept[w];
-            if (w < wave) off += n;
-            total += n;
-        }
-        if (keep) next_list[off + (uint32_t)__popcll(kept & ((1ull << lane) - 1ull))] = v;
-        base += total;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *next_count = base;
+    const uint32_t kept = compact_list(next_list, *count_ptr, wave_kept);
+    if (threadIdx.x == 0) *next_count = kept;
 }
 
 // counts: [0..1] running-list counts; from kSchedOffset the check-kernel

@@ -315,7 +315,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kVarWa
 }
 
 // One 64-lane workgroup per running frame: syndrome, then retire the frame
-// (converged, or at the iteration cap) or append it to its code's next list.
+// (converged, or at the iteration cap) or keep its slot in its code's next list.
 // A frame failing at the end of the weighted iterations (it == T < cap)
 // stays on the list: the next iteration restarts it from its LLRs.
 // The last workgroup to finish advances the iteration counter and flips the
@@ -327,7 +327,6 @@ __global__ void __launch_bounds__(64) syn_mix_kernel(MixShared m) {
     const uint32_t it = ldc(m.misc + 2) + 1;
     const uint32_t *cnt = m.counts + cur * kMaxCodes;
     const uint32_t *list = m.lists + (size_t)cur * m.list_cap;
-    uint32_t *next_cnt = m.counts + (cur ^ 1u) * kMaxCodes;
     uint32_t *next_list = m.lists + (size_t)(cur ^ 1u) * m.list_cap;
     const uint32_t max_total = ldc(&m.args->max_total);
     const uint32_t weighted = ldc(&m.args->weighted);
@@ -353,9 +352,11 @@ __global__ void __launch_bounds__(64) syn_mix_kernel(MixShared m) {
                     a->synd[si.frame] = (uint8_t)!unsat;
                     if (a->fb) a->fb[si.frame] = (uint8_t)(weighted && it > weighted);
                 }
-            } else if (threadIdx.x == 0) {
-                next_list[seg.list_off + atomicAdd(next_cnt + c, 1u)] = slot;
             }
+            // same position in the next list; mix_compact_kernel closes the
+            // holes of retired frames in list order
+            if (threadIdx.x == 0)
+                next_list[seg.list_off + item - start] = (!unsat || it >= max_total) ? kRetiredSlot : slot;
             __syncthreads();
         }
     }
@@ -368,6 +369,19 @@ __global__ void __launch_bounds__(64) syn_mix_kernel(MixShared m) {
             for (uint32_t k = 0; k < kSchedCounters; ++k) m.misc[kSchedBase + k * kCtrStride] = 0;
         }
     }
+}
+
+// After the syndrome pass (which has flipped the list parity): one workgroup
+// per code closes the holes of its segment of the list just written, in list
+// order, and stores the code's running count.
+__global__ void __launch_bounds__(1024) mix_compact_kernel(MixShared m) {
+    __shared__ uint32_t wave_kept[16];
+    const uint32_t c = blockIdx.x;
+    const uint32_t cur = m.misc[0];
+    const uint32_t n = m.counts[(cur ^ 1u) * kMaxCodes + c];
+    uint32_t *list = m.lists + (size_t)cur * m.list_cap + m.args->seg[c].list_off;
+    const uint32_t kept = compact_list(list, n, wave_kept);
+    if (threadIdx.x == 0) m.counts[cur * kMaxCodes + c] = kept;
 }
 
 // Start of a call: every slot on its code's list, lists and counters reset.
@@ -526,6 +540,7 @@ void enqueue_iteration(const gf2l_codeset *cs, const Shape &sh, hipStream_t s, u
     launch_vn<16, NMS>(m, cs->vn_mask[1], sh, s, t, 4u);
     launch_vn<32, NMS>(m, cs->vn_mask[2], sh, s, t, 5u);
     hipLaunchKernelGGL(syn_mix_kernel, dim3(sh.g_syn), dim3(64), sh.sh, s, m);
+    hipLaunchKernelGGL(mix_compact_kernel, dim3(m.ncodes), dim3(1024), 0, s, m);
 }
 
 // Captures `unroll` non-first iterations as a graph over the code set's
