@@ -314,13 +314,13 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kVarWa
     if (blockIdx.x == 0 && threadIdx.x < kMaxCodes) m.counts[(cur ^ 1u) * kMaxCodes + threadIdx.x] = 0;
 }
 
-// One 64-lane workgroup per running frame: syndrome, then retire the frame
+// One 256-thread workgroup per running frame: syndrome, then retire the frame
 // (converged, or at the iteration cap) or keep its slot in its code's next list.
 // A frame failing at the end of the weighted iterations (it == T < cap)
 // stays on the list: the next iteration restarts it from its LLRs.
 // The last workgroup to finish advances the iteration counter and flips the
 // list parity.
-__global__ void __launch_bounds__(64) syn_mix_kernel(MixShared m) {
+__global__ void __launch_bounds__(256) syn_mix_kernel(MixShared m) {
     extern __shared__ uint64_t sh[];
     __shared__ uint32_t s_start[kMaxCodes + 1];
     const uint32_t cur = ldc(m.misc);
@@ -539,7 +539,7 @@ void enqueue_iteration(const gf2l_codeset *cs, const Shape &sh, hipStream_t s, u
     launch_vn<8, NMS>(m, cs->vn_mask[0], sh, s, t, 3u);
     launch_vn<16, NMS>(m, cs->vn_mask[1], sh, s, t, 4u);
     launch_vn<32, NMS>(m, cs->vn_mask[2], sh, s, t, 5u);
-    hipLaunchKernelGGL(syn_mix_kernel, dim3(sh.g_syn), dim3(64), sh.sh, s, m);
+    hipLaunchKernelGGL(syn_mix_kernel, dim3(sh.g_syn), dim3(256), sh.sh, s, m);
     hipLaunchKernelGGL(mix_compact_kernel, dim3(m.ncodes), dim3(1024), 0, s, m);
 }
 
