@@ -484,7 +484,7 @@ __device__ __forceinline__ void vn_state_item(const CodeView &cv, uint32_t bc, c
 }
 
 // ---------------------------------------------------------------------------
-// Syndrome and message extraction of one frame by one 64-lane workgroup.
+// Syndrome and message extraction of one frame by one workgroup (any size).
 
 // Bits pos .. pos+63 (cyclically mod z) of a z-bit vector stored in zw
 // words whose bits at z and above are zero.  Needs z >= 64.

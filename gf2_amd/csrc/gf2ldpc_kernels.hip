@@ -118,7 +118,8 @@ vn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *
     } else {
         for (uint32_t jj = xs.q; jj < xs.items; jj += xs.Q) item(jj);
     }
-    // stored last: the running count the syndrome pass appends to
+    // stored last: the next running count starts at zero (the syndrome step
+    // stores the survivor count after compacting the list)
     if (next_count && blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
 }
 

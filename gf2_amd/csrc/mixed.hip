@@ -307,9 +307,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kVarWa
             for (uint32_t j = first_item(start, sp.q, sp.Q); j < end; j += sp.Q) item(cv, seg, w_cur, j - start);
         }
     }
-    // The syndrome pass of this iteration appends to the other list: every
-    // variable launch of the iteration clears its counts (the syndrome pass
-    // follows them all).  Stored last: a global store ahead of the loop would
+    // The syndrome pass of this iteration writes the other list and
+    // mix_compact_kernel stores its counts: every variable launch of the
+    // iteration clears them first (the syndrome pass follows them all).  Stored last: a global store ahead of the loop would
     // keep the table loads above from being scalar loads.
     if (blockIdx.x == 0 && threadIdx.x < kMaxCodes) m.counts[(cur ^ 1u) * kMaxCodes + threadIdx.x] = 0;
 }
