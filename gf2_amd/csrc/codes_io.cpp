@@ -7,8 +7,11 @@
 //            LdpcCode::from_edges.  Info bit i*360 + j of table row i and
 //            address p connects to check (p + j*q) mod m; the parity part is
 //            the bit-level staircase (check p -> parity bits p and p-1, no
-//            wrap at p = 0).  That staircase is not circulant, so these codes
-//            run on the CSR path.
+//            wrap at p = 0).  Renumbered (check a + q*b -> lane b of
+//            block-row a) these codes are quasi-cyclic and run on the state
+//            kernels; a table outside that form runs on the CSR path.
+//   Sizes and indices from the caller or the file are validated by the
+//   host-only code of host_codes.cpp before anything is allocated.
 //   GF2DATA  crates/gf2-core/src/io/mod.rs:6-98 (32-byte header, JSON
 //            metadata, little-endian payload), header.rs:33-86,
 //            sparse.rs:151-200 (type 3, COO pairs, set semantics on load)
@@ -25,6 +28,7 @@
 
 #include "dvb_t2_tables.h"
 #include "gf2ldpc.h"
+#include "host_codes.h"
 #include "internal.h"
 
 namespace {
@@ -44,45 +48,6 @@ constexpr DvbShape kDvbShapes[2][6] = {
 };
 
 bool dvb_index_ok(int frame, int rate) { return frame >= 0 && frame < 2 && rate >= 0 && rate < 6; }
-
-uint32_t rd32(const uint8_t *p) { return p[0] | (p[1] << 8) | (p[2] << 16) | ((uint32_t)p[3] << 24); }
-uint64_t rd64(const uint8_t *p) { return rd32(p) | ((uint64_t)rd32(p + 4) << 32); }
-
-// Minimal reader of the flat JSON object the writer emits: one number or
-// string value per key.
-struct Meta {
-    std::string text;
-    bool find(const char *key, size_t &pos) const {
-        const std::string k = std::string("\"") + key + "\"";
-        size_t at = text.find(k);
-        if (at == std::string::npos) return false;
-        at += k.size();
-        while (at < text.size() && (text[at] == ' ' || text[at] == '\t' || text[at] == '\n' || text[at] == '\r')) ++at;
-        if (at >= text.size() || text[at] != ':') return false;
-        ++at;
-        while (at < text.size() && (text[at] == ' ' || text[at] == '\t' || text[at] == '\n' || text[at] == '\r')) ++at;
-        pos = at;
-        return at < text.size();
-    }
-    bool number(const char *key, uint64_t &v) const {
-        size_t p;
-        if (!find(key, p) || text[p] < '0' || text[p] > '9') return false;
-        v = 0;
-        for (; p < text.size() && text[p] >= '0' && text[p] <= '9'; ++p) {
-            if (v > (UINT64_MAX - 9) / 10) return false;
-            v = v * 10 + (uint64_t)(text[p] - '0');
-        }
-        return true;
-    }
-    bool string(const char *key, std::string &v) const {
-        size_t p;
-        if (!find(key, p) || text[p] != '"') return false;
-        const size_t e = text.find('"', p + 1);
-        if (e == std::string::npos) return false;
-        v = text.substr(p + 1, e - p - 1);
-        return true;
-    }
-};
 
 int invalid(gf2l_ctx *ctx, const std::string &msg) { return gf2l::fail(ctx, GF2L_ERR_INVALID, msg); }
 
@@ -119,189 +84,114 @@ extern "C" int gf2l_code_dvb_table(gf2l_ctx *ctx, uint32_t n, uint32_t k, uint32
                                    const uint32_t *row_len, const uint32_t *addresses, gf2l_code **out) {
     if (!ctx || !out) return GF2L_ERR_INVALID;
     *out = nullptr;
-    if (!row_len || !addresses || rows == 0) return invalid(ctx, "null DVB table");
-    if (z == 0 || k > n || n - k == 0) return invalid(ctx, "DVB parameters need 0 < k < n and z > 0");
-    const uint32_t m = n - k;
-    // validate_table (builder.rs:15-41)
-    if (!(rows > 1 || row_len[0] > 1)) return invalid(ctx, "DVB-T2 table not yet implemented (placeholder detected)");
-    if (rows != k / z)
-        return invalid(ctx, "Table must have " + std::to_string(k / z) + " rows (info blocks), found " +
-                                std::to_string(rows));
-    size_t total = 0;
-    for (uint32_t r = 0; r < rows; ++r) total += row_len[r];
-    for (uint32_t r = 0, at = 0; r < rows; ++r)
-        for (uint32_t i = 0; i < row_len[r]; ++i, ++at)
-            if (addresses[at] >= m)
-                return invalid(ctx, "Invalid parity index " + std::to_string(addresses[at]) + " in table row " +
-                                        std::to_string(r) + " (must be < " + std::to_string(m) + ")");
-    // Quasi-cyclic form (the decoder's fast path): check c = a + q*b and
-    // parity bit k + a + q*b become lane b of block-row / parity block-column
-    // a.  Table address p of row i is then a circulant in block (p mod q, i)
-    // with shift (z - p div q) mod z; the staircase gives identity blocks
-    // (a, kb+a) and (a, kb+a-1), and block (0, kb+q-1) with shift z-1 that
-    // lacks its lane-0 edge (check 0 has one parity bit).  Needs m = q*z,
-    // k = rows*z, and no address repeated within a row (a repeat cancels in
-    // from_coo); otherwise the code runs on the CSR path.
-    bool qc = m == (uint64_t)q * z && k == (uint64_t)rows * z && q >= 2 && z >= 2;
-    for (uint32_t r = 0, at = 0; qc && r < rows; at += row_len[r], ++r) {
-        std::vector<uint32_t> v(addresses + at, addresses + at + row_len[r]);
-        std::sort(v.begin(), v.end());
-        qc = std::adjacent_find(v.begin(), v.end()) == v.end();
-    }
-    if (qc) {
-        const uint32_t kb = rows, nb = kb + q;
-        std::vector<std::vector<uint2>> per_row(q);
+    return gf2l::guard(ctx, [&]() -> int {
+        // validate_table (builder.rs:15-41) and the size limits, before any
+        // allocation sized from the table
+        size_t total = 0;
+        std::string err;
+        const int vrc = gf2l::host::check_dvb_table(n, k, z, rows, row_len, addresses, total, err);
+        if (vrc != GF2L_OK) return gf2l::fail(ctx, vrc, err);
+        const uint32_t m = n - k;
+        // Quasi-cyclic form (the decoder's fast path): check c = a + q*b and
+        // parity bit k + a + q*b become lane b of block-row / parity block-column
+        // a.  Table address p of row i is then a circulant in block (p mod q, i)
+        // with shift (z - p div q) mod z; the staircase gives identity blocks
+        // (a, kb+a) and (a, kb+a-1), and block (0, kb+q-1) with shift z-1 that
+        // lacks its lane-0 edge (check 0 has one parity bit).  Needs m = q*z,
+        // k = rows*z, and no address repeated within a row (a repeat cancels in
+        // from_coo); otherwise the code runs on the CSR path.
+        bool qc = m == (uint64_t)q * z && k == (uint64_t)rows * z && q >= 2 && z >= 2;
+        for (uint32_t r = 0, at = 0; qc && r < rows; at += row_len[r], ++r) {
+            std::vector<uint32_t> v(addresses + at, addresses + at + row_len[r]);
+            std::sort(v.begin(), v.end());
+            qc = std::adjacent_find(v.begin(), v.end()) == v.end();
+        }
+        if (qc) {
+            const uint32_t kb = rows, nb = kb + q;
+            std::vector<std::vector<uint2>> per_row(q);
+            for (uint32_t r = 0, at = 0; r < rows; ++r)
+                for (uint32_t i = 0; i < row_len[r]; ++i, ++at) {
+                    const uint32_t p = addresses[at];
+                    per_row[p % q].push_back(make_uint2(r, (z - p / q) % z));
+                }
+            gf2l_code *c = new gf2l_code;
+            c->device = ctx->device;
+            c->h_row_ptr.assign(q + 1, 0);
+            uint32_t max_dc = 0;
+            for (uint32_t a = 0; a < q; ++a) {
+                auto &v = per_row[a];
+                std::stable_sort(v.begin(), v.end(), [](const uint2 &x, const uint2 &y) {
+                    return x.x != y.x ? x.x < y.x : x.y < y.y;
+                });
+                if (a == 0) {
+                    v.push_back(make_uint2(kb + q - 1, z - 1));
+                    v.push_back(make_uint2(kb, 0));
+                } else {
+                    v.push_back(make_uint2(kb + a - 1, 0));
+                    v.push_back(make_uint2(kb + a, 0));
+                }
+                c->h_row_edge.insert(c->h_row_edge.end(), v.begin(), v.end());
+                c->h_row_ptr[a + 1] = (uint32_t)c->h_row_edge.size();
+                max_dc = std::max<uint32_t>(max_dc, (uint32_t)v.size());
+            }
+            if (max_dc <= gf2l::kMaxStateDegree) {
+                if (hipSetDevice(ctx->device) != hipSuccess) {
+                    delete c;
+                    return gf2l::fail(ctx, GF2L_ERR_HIP, "hipSetDevice failed");
+                }
+                const int rc = gf2l::build_qc_view(ctx, c, q, nb, z, q);
+                if (rc == GF2L_OK) *out = c;
+                return rc;
+            }
+            delete c;
+        }
+        // build_dvb_edges (builder.rs:74-118): table edges, then the staircase
+        std::vector<uint32_t> rc;
+        rc.reserve(2 * (total * z + 2ull * m));
         for (uint32_t r = 0, at = 0; r < rows; ++r)
-            for (uint32_t i = 0; i < row_len[r]; ++i, ++at) {
-                const uint32_t p = addresses[at];
-                per_row[p % q].push_back(make_uint2(r, (z - p / q) % z));
-            }
-        gf2l_code *c = new gf2l_code;
-        c->device = ctx->device;
-        c->h_row_ptr.assign(q + 1, 0);
-        uint32_t max_dc = 0;
-        for (uint32_t a = 0; a < q; ++a) {
-            auto &v = per_row[a];
-            std::stable_sort(v.begin(), v.end(), [](const uint2 &x, const uint2 &y) {
-                return x.x != y.x ? x.x < y.x : x.y < y.y;
-            });
-            if (a == 0) {
-                v.push_back(make_uint2(kb + q - 1, z - 1));
-                v.push_back(make_uint2(kb, 0));
-            } else {
-                v.push_back(make_uint2(kb + a - 1, 0));
-                v.push_back(make_uint2(kb + a, 0));
-            }
-            c->h_row_edge.insert(c->h_row_edge.end(), v.begin(), v.end());
-            c->h_row_ptr[a + 1] = (uint32_t)c->h_row_edge.size();
-            max_dc = std::max<uint32_t>(max_dc, (uint32_t)v.size());
-        }
-        if (max_dc <= gf2l::kMaxStateDegree) {
-            if (hipSetDevice(ctx->device) != hipSuccess) {
-                delete c;
-                return gf2l::fail(ctx, GF2L_ERR_HIP, "hipSetDevice failed");
-            }
-            const int rc = gf2l::build_qc_view(ctx, c, q, nb, z, q);
-            if (rc == GF2L_OK) *out = c;
-            return rc;
-        }
-        delete c;
-    }
-    // build_dvb_edges (builder.rs:74-118): table edges, then the staircase
-    std::vector<uint32_t> rc;
-    rc.reserve(2 * (total * z + 2ull * m));
-    for (uint32_t r = 0, at = 0; r < rows; ++r)
-        for (uint32_t i = 0; i < row_len[r]; ++i, ++at)
-            for (uint32_t j = 0; j < z; ++j) {
-                rc.push_back((uint32_t)((addresses[at] + (uint64_t)j * q) % m));
-                rc.push_back(r * z + j);
-            }
-    for (uint32_t p = 0; p < m; ++p) {
-        rc.push_back(p);
-        rc.push_back(k + p);
-        if (p > 0) {
+            for (uint32_t i = 0; i < row_len[r]; ++i, ++at)
+                for (uint32_t j = 0; j < z; ++j) {
+                    rc.push_back((uint32_t)((addresses[at] + (uint64_t)j * q) % m));
+                    rc.push_back(r * z + j);
+                }
+        for (uint32_t p = 0; p < m; ++p) {
             rc.push_back(p);
-            rc.push_back(k + p - 1);
+            rc.push_back(k + p);
+            if (p > 0) {
+                rc.push_back(p);
+                rc.push_back(k + p - 1);
+            }
         }
-    }
-    return gf2l_code_edges(ctx, m, n, rc.data(), rc.size() / 2, out);
+        return gf2l_code_edges(ctx, m, n, rc.data(), rc.size() / 2, out);
+    });
 }
 
 extern "C" int gf2l_code_dvb_t2(gf2l_ctx *ctx, int frame, int rate, gf2l_code **out) {
     if (!ctx || !out) return GF2L_ERR_INVALID;
     *out = nullptr;
-    if (!dvb_index_ok(frame, rate))
-        return invalid(ctx, "DVB-T2 frame size must be 0 (short) or 1 (normal) and rate 0..5");
-    const DvbShape &s = kDvbShapes[frame][rate];
-    const gf2l::dvb_tables::Table &t = gf2l::dvb_tables::kTables[frame][rate];
-    std::vector<uint32_t> len(t.rows), addr;
-    for (uint32_t r = 0, at = 0; r < t.rows; ++r) {
-        len[r] = t.row_len[r];
-        for (uint32_t i = 0; i < len[r]; ++i, ++at) addr.push_back(t.addr[at]);
-    }
-    return gf2l_code_dvb_table(ctx, s.n, s.k, s.q, kDvbZ, t.rows, len.data(), addr.data(), out);
+    return gf2l::guard(ctx, [&]() -> int {
+        if (!dvb_index_ok(frame, rate))
+            return invalid(ctx, "DVB-T2 frame size must be 0 (short) or 1 (normal) and rate 0..5");
+        const DvbShape &s = kDvbShapes[frame][rate];
+        const gf2l::dvb_tables::Table &t = gf2l::dvb_tables::kTables[frame][rate];
+        std::vector<uint32_t> len(t.rows), addr;
+        for (uint32_t r = 0, at = 0; r < t.rows; ++r) {
+            len[r] = t.row_len[r];
+            for (uint32_t i = 0; i < len[r]; ++i, ++at) addr.push_back(t.addr[at]);
+        }
+        return gf2l_code_dvb_table(ctx, s.n, s.k, s.q, kDvbZ, t.rows, len.data(), addr.data(), out);
+    });
 }
 
 extern "C" int gf2l_code_gf2data(gf2l_ctx *ctx, const void *bytes, size_t len, gf2l_code **out) {
     if (!ctx || !out) return GF2L_ERR_INVALID;
     *out = nullptr;
-    if (!bytes) return invalid(ctx, "null GF2DATA buffer");
-    const uint8_t *b = static_cast<const uint8_t *>(bytes);
-    // Header::read_from (header.rs:33-86)
-    if (len < 32) return invalid(ctx, "I/O error: file shorter than the 32-byte header");
-    if (std::memcmp(b, "GF2DATA\0", 8) != 0) return invalid(ctx, "Invalid magic bytes");
-    const uint32_t version = b[8] | (b[9] << 8);
-    if (version != 1) return invalid(ctx, "Unsupported format version: " + std::to_string(version));
-    const uint8_t type = b[10], flags = b[11];
-    if (type < 1 || type > 4) return invalid(ctx, "Unknown type tag: " + std::to_string(type));
-    if (type != 3 && type != 4)
-        return invalid(ctx, "Invalid data: a code needs a sparse matrix (type 3 or 4), got type " +
-                                std::to_string(type));
-    if (flags & 0x01) return gf2l::fail(ctx, GF2L_ERR_UNSUPPORTED, "Invalid data: compressed payloads are not supported");
-    const uint32_t meta_len = rd32(b + 20);
-    const uint64_t data_len = rd64(b + 24);
-    if ((uint64_t)len - 32 < meta_len) return invalid(ctx, "I/O error: metadata runs past the end of the buffer");
-    Meta meta{std::string(reinterpret_cast<const char *>(b + 32), meta_len)};
-    std::string tname;
-    uint64_t rows = 0, cols = 0, nnz = 0;
-    if (!meta.string("type", tname) || !meta.number("rows", rows) || !meta.number("cols", cols) ||
-        !meta.number("nnz", nnz))
-        return invalid(ctx, "Invalid data: Failed to parse metadata");
-    const char *want = type == 3 ? "SpBitMatrix" : "SpBitMatrixDual";
-    if (tname != want) return invalid(ctx, std::string("Invalid data: Expected ") + want + " type, got " + tname);
-    if (rows > 0xFFFFFFFFull || cols > 0xFFFFFFFFull || nnz > 0xFFFFFFFFull)
-        return gf2l::fail(ctx, GF2L_ERR_UNSUPPORTED, "matrix dimensions above 2^32");
-    const uint64_t expected = type == 3 ? nnz * 8 : (rows + 1 + nnz + cols + 1 + nnz) * 4;
-    if (type == 3) {
-        std::string fmt;
-        if (!meta.string("format", fmt) || fmt != "coo")
-            return invalid(ctx, "Invalid data: Expected COO format, got " + fmt);
-    }
-    if (data_len != expected)
-        return invalid(ctx, "Invalid data: Data length mismatch: header says " + std::to_string(data_len) +
-                                " bytes, expected " + std::to_string(expected));
-    if ((uint64_t)len - 32 - meta_len < data_len) return invalid(ctx, "I/O error: payload runs past the end of the buffer");
-    const uint8_t *pl = b + 32 + meta_len;
-    std::vector<uint32_t> rc;
-    if (type == 3) {
-        // SpBitMatrix::from_coo_deduplicated (sparse.rs:232-259): set
-        // semantics, so each distinct entry is kept once.
-        std::vector<std::vector<uint32_t>> per_row(rows);
-        for (uint64_t e = 0; e < nnz; ++e) {
-            const uint32_t r = rd32(pl + 8 * e), c = rd32(pl + 8 * e + 4);
-            if (r >= rows)
-                return invalid(ctx, "row index " + std::to_string(r) + " out of bounds (rows=" + std::to_string(rows) + ")");
-            if (c >= cols)
-                return invalid(ctx, "col index " + std::to_string(c) + " out of bounds (cols=" + std::to_string(cols) + ")");
-            per_row[r].push_back(c);
-        }
-        for (uint32_t r = 0; r < rows; ++r) {
-            auto &v = per_row[r];
-            std::sort(v.begin(), v.end());
-            v.erase(std::unique(v.begin(), v.end()), v.end());
-            for (uint32_t c : v) {
-                rc.push_back(r);
-                rc.push_back(c);
-            }
-        }
-    } else {
-        // SpBitMatrixDual::from_csr_csc keeps the CSR rows as stored; the
-        // code is built from those (row, col) entries.  The CSC half only
-        // has to be present.
-        std::vector<uint32_t> ro(rows + 1);
-        for (uint64_t r = 0; r <= rows; ++r) ro[r] = rd32(pl + 4 * r);
-        if (ro[0] != 0 || ro[rows] != nnz) return invalid(ctx, "Invalid data: row offsets do not span nnz entries");
-        const uint8_t *ri = pl + 4 * (rows + 1);
-        for (uint32_t r = 0; r < rows; ++r) {
-            if (ro[r + 1] < ro[r]) return invalid(ctx, "Invalid data: row offsets decrease");
-            for (uint32_t e = ro[r]; e < ro[r + 1]; ++e) {
-                const uint32_t c = rd32(ri + 4ull * e);
-                if (c >= cols)
-                    return invalid(ctx, "col index " + std::to_string(c) + " out of bounds (cols=" + std::to_string(cols) + ")");
-                rc.push_back(r);
-                rc.push_back(c);
-            }
-        }
-    }
-    return gf2l_code_edges(ctx, (uint32_t)rows, (uint32_t)cols, rc.empty() ? nullptr : rc.data(), rc.size() / 2, out);
+    return gf2l::guard(ctx, [&]() -> int {
+        uint32_t rows = 0, cols = 0;
+        std::vector<uint32_t> rc;
+        std::string err;
+        const int prc = gf2l::host::parse_gf2data(bytes, len, rows, cols, rc, err);
+        if (prc != GF2L_OK) return gf2l::fail(ctx, prc, err);
+        return gf2l_code_edges(ctx, rows, cols, rc.empty() ? nullptr : rc.data(), rc.size() / 2, out);
+    });
 }

@@ -19,6 +19,7 @@
 
 #include "device_common.h"
 #include "gf2ldpc.h"
+#include "host_codes.h"
 
 namespace gf2l {
 namespace {
@@ -166,79 +167,54 @@ extern "C" int gf2l_code_edges(gf2l_ctx *ctx, uint32_t m, uint32_t n, const uint
                                size_t nnz, gf2l_code **out) {
     if (!ctx || !out) return GF2L_ERR_INVALID;
     *out = nullptr;
-    if (nnz && !rc_pairs) return fail(ctx, GF2L_ERR_INVALID, "null edge list");
-    // SpBitMatrix::from_coo (sparse.rs:166-204): bounds checks, per-row sort,
-    // duplicates cancel in pairs.
-    std::vector<std::vector<uint32_t>> rows(m);
-    for (size_t e = 0; e < nnz; ++e) {
-        const uint32_t r = rc_pairs[2 * e], c = rc_pairs[2 * e + 1];
-        if (r >= m)
-            return fail(ctx, GF2L_ERR_INVALID, "row index " + std::to_string(r) + " out of bounds (rows=" +
-                                                   std::to_string(m) + ")");
-        if (c >= n)
-            return fail(ctx, GF2L_ERR_INVALID, "col index " + std::to_string(c) + " out of bounds (cols=" +
-                                                   std::to_string(n) + ")");
-        rows[r].push_back(c);
-    }
-    std::vector<uint32_t> row_ptr(m + 1, 0), col_idx;
-    uint32_t max_dc = 0;
-    for (uint32_t r = 0; r < m; ++r) {
-        auto &v = rows[r];
-        std::sort(v.begin(), v.end());
-        size_t i = 0;
-        while (i < v.size()) {
-            size_t k = 1;
-            while (i + k < v.size() && v[i + k] == v[i]) ++k;
-            if (k % 2 == 1) col_idx.push_back(v[i]);
-            i += k;
+    return gf2l::guard(ctx, [&]() -> int {
+        // SpBitMatrix::from_coo (sparse.rs:166-204): bounds checks, per-row sort,
+        // duplicates cancel in pairs; then the stable transpose
+        gf2l::host::Csr h;
+        std::string err;
+        const int hrc = gf2l::host::build_csr(m, n, rc_pairs, nnz, h, err);
+        if (hrc != GF2L_OK) return fail(ctx, hrc, err);
+        if (h.max_dc > gf2l::kMaxCheckDegree) return fail(ctx, GF2L_ERR_UNSUPPORTED, "check degree above 32");
+        const std::vector<uint32_t> &row_ptr = h.row_ptr, &col_idx = h.col_idx, &col_ptr = h.col_ptr,
+                                    &csc_edge = h.csc_edge;
+        const uint32_t max_dc = h.max_dc;
+        const uint32_t E = (uint32_t)col_idx.size();
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        const size_t b_rp = ((size_t)m + 1) * 4, b_ci = (size_t)std::max(E, 1u) * 4, b_cp = ((size_t)n + 1) * 4;
+        auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
+        const size_t o_ci = al(b_rp), o_cp = al(o_ci + b_ci), o_ce = al(o_cp + b_cp), total = o_ce + b_ci;
+        std::vector<char> host(total, 0);
+        std::memcpy(host.data(), row_ptr.data(), b_rp);
+        if (E) std::memcpy(host.data() + o_ci, col_idx.data(), (size_t)E * 4);
+        std::memcpy(host.data() + o_cp, col_ptr.data(), b_cp);
+        if (E) std::memcpy(host.data() + o_ce, csc_edge.data(), (size_t)E * 4);
+        gf2l_code *c = new gf2l_code;
+        c->device = ctx->device;
+        c->is_csr = true;
+        hipError_t he = hipMalloc(&c->d_tables, total);
+        if (he == hipSuccess) he = hipMemcpy(c->d_tables, host.data(), total, hipMemcpyHostToDevice);
+        if (he != hipSuccess) {
+            hipFree(c->d_tables);
+            delete c;
+            return fail(ctx, GF2L_ERR_HIP, std::string("code upload: ") + hipGetErrorString(he));
         }
-        row_ptr[r + 1] = (uint32_t)col_idx.size();
-        max_dc = std::max(max_dc, row_ptr[r + 1] - row_ptr[r]);
-    }
-    if (max_dc > gf2l::kMaxCheckDegree) return fail(ctx, GF2L_ERR_UNSUPPORTED, "check degree above 32");
-    const uint32_t E = (uint32_t)col_idx.size();
-    // stable transpose (sparse.rs:297-331): each column's checks ascending
-    std::vector<uint32_t> col_ptr(n + 1, 0), csc_edge(E);
-    for (uint32_t e = 0; e < E; ++e) col_ptr[col_idx[e] + 1]++;
-    for (uint32_t v = 0; v < n; ++v) col_ptr[v + 1] += col_ptr[v];
-    std::vector<uint32_t> next(col_ptr.begin(), col_ptr.end() - 1);
-    for (uint32_t r = 0; r < m; ++r)
-        for (uint32_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) csc_edge[next[col_idx[e]]++] = e;
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    gf2l_code *c = new gf2l_code;
-    c->device = ctx->device;
-    c->is_csr = true;
-    const size_t b_rp = (size_t)(m + 1) * 4, b_ci = (size_t)std::max(E, 1u) * 4, b_cp = (size_t)(n + 1) * 4;
-    auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
-    const size_t o_ci = al(b_rp), o_cp = al(o_ci + b_ci), o_ce = al(o_cp + b_cp), total = o_ce + b_ci;
-    std::vector<char> host(total, 0);
-    std::memcpy(host.data(), row_ptr.data(), b_rp);
-    if (E) std::memcpy(host.data() + o_ci, col_idx.data(), (size_t)E * 4);
-    std::memcpy(host.data() + o_cp, col_ptr.data(), b_cp);
-    if (E) std::memcpy(host.data() + o_ce, csc_edge.data(), (size_t)E * 4);
-    hipError_t err = hipMalloc(&c->d_tables, total);
-    if (err == hipSuccess) err = hipMemcpy(c->d_tables, host.data(), total, hipMemcpyHostToDevice);
-    if (err != hipSuccess) {
-        hipFree(c->d_tables);
-        delete c;
-        return fail(ctx, GF2L_ERR_HIP, std::string("code upload: ") + hipGetErrorString(err));
-    }
-    char *d = static_cast<char *>(c->d_tables);
-    gf2l::CsrView &v = c->csr;
-    v.m = m;
-    v.n = n;
-    v.k = n > m ? n - m : 0;
-    v.nnz = E;
-    v.row_ptr = reinterpret_cast<const uint32_t *>(d);
-    v.col_idx = reinterpret_cast<const uint32_t *>(d + o_ci);
-    v.col_ptr = reinterpret_cast<const uint32_t *>(d + o_cp);
-    v.csc_edge = reinterpret_cast<const uint32_t *>(d + o_ce);
-    // the QC view stays empty except the sizes used by the generic helpers
-    c->view = gf2l::CodeView{};
-    c->view.n = n;
-    c->view.m = m;
-    c->view.k = v.k;
-    c->view.max_dc = max_dc;
-    *out = c;
-    return GF2L_OK;
+        char *d = static_cast<char *>(c->d_tables);
+        gf2l::CsrView &v = c->csr;
+        v.m = m;
+        v.n = n;
+        v.k = n > m ? n - m : 0;
+        v.nnz = E;
+        v.row_ptr = reinterpret_cast<const uint32_t *>(d);
+        v.col_idx = reinterpret_cast<const uint32_t *>(d + o_ci);
+        v.col_ptr = reinterpret_cast<const uint32_t *>(d + o_cp);
+        v.csc_edge = reinterpret_cast<const uint32_t *>(d + o_ce);
+        // the QC view stays empty except the sizes used by the generic helpers
+        c->view = gf2l::CodeView{};
+        c->view.n = n;
+        c->view.m = m;
+        c->view.k = v.k;
+        c->view.max_dc = max_dc;
+        *out = c;
+        return GF2L_OK;
+    });
 }

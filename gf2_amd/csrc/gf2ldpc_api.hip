@@ -52,7 +52,7 @@ int ensure_buffers(gf2l_ctx *ctx, size_t frames, size_t c2v, size_t state, size_
     GF2L_HIP(ctx, hipMalloc(&ws.hb, std::max<size_t>(hb, 1) * sizeof(uint64_t)));
     GF2L_HIP(ctx, hipMalloc(&ws.lists, 2 * std::max<size_t>(f, 1) * sizeof(uint32_t)));
     GF2L_HIP(ctx, hipMalloc(&ws.counts, gf2l::kCountsWords * sizeof(uint32_t)));  // counts + item counters
-    GF2L_HIP(ctx, hipHostMalloc((void **)&ws.h_count, sizeof(uint32_t), hipHostMallocDefault));
+    GF2L_HIP(ctx, hipHostMalloc((void **)&ws.h_count, gf2l::kProfileSlots * sizeof(uint32_t), hipHostMallocDefault));
     ws.cap_frames = f;
     ws.cap_c2v = c2v;
     ws.cap_state = state;
@@ -76,6 +76,10 @@ int ensure_workspace(gf2l_ctx *ctx, const gf2l::CodeView &cv, int path, size_t f
     return gf2l::ensure_buffers(ctx, frames, c2v, state, bel, hb, llr);
 }
 
+}  // namespace
+
+namespace gf2l {
+
 // Resolves the decoder path for a code (GF2L_PATH_*), or -1 if the forced
 // path cannot run it.
 int resolve_path(const gf2l_ctx *ctx, const gf2l::CodeView &cv) {
@@ -94,10 +98,6 @@ int resolve_path(const gf2l_ctx *ctx, const gf2l::CodeView &cv) {
             return cv.max_dc <= gf2l::kMaxStateDegree ? gf2l::kState : gf2l::kStream;
     }
 }
-
-}  // namespace
-
-namespace gf2l {
 
 // Runs the decoder on device buffers.  Without weights: plain min-sum for up
 // to max_it iterations.  With weights d_w ([max_it][nbe]): max_it weighted
@@ -157,7 +157,6 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
     int rc = ensure_workspace(ctx, cv, path, F);
     if (rc != GF2L_OK) return rc;
     gf2l::Workspace &ws = ctx->ws;
-    gf2l::Profile &pf = ctx->prof;
     float *d_tc = nullptr, *d_tb = nullptr;  // explicit-layout trace staging (state path)
     if (h_trace_c2v && trace_iters && path == gf2l::kState) {
         GF2L_HIP(ctx, hipMalloc(&d_tc, (size_t)F * E * sizeof(float)));
@@ -167,26 +166,47 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
         float *a, *b;
         ~Free() { hipFree(a); hipFree(b); }
     } free_trace{d_tc, d_tb};
+    return run_hbm(ctx, cv, path, ws, d_llr, F, max_it, d_w, fb_it, d_msg, d_iters, d_conv, d_synd, d_fb,
+                   h_trace_c2v, h_trace_bel, trace_iters, d_tc, d_tb);
+}
+
+// The iteration loop of the HBM paths over a prepared workspace: LLR
+// permutation of DVB codes, then phase 0 (max_it iterations, weighted by d_w
+// when given) and, with fb_it > 0, phase 1 (fb_it plain iterations restarted
+// from the LLRs for frames still failing).  Iterations are enqueued in
+// batches of GF2L_TUNE_GRAPH_ITERS; the host reads the running count once per
+// batch (every iteration when tracing) and stops once no frame runs.
+// Iterations enqueued after the last frame retired find an empty list and
+// do nothing.  Allocates nothing: ws.h_count may point at pageable memory.
+int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const float *d_llr, uint32_t F,
+            uint32_t max_it, const float *d_w, uint32_t fb_it, uint64_t *d_msg, uint32_t *d_iters,
+            uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb, float *h_trace_c2v, float *h_trace_bel,
+            uint32_t trace_iters, float *d_tc, float *d_tb) {
+    hipStream_t s = ctx->stream;
+    const size_t E = (size_t)cv.nbe * cv.z;
+    gf2l::Profile &pf = ctx->prof;
     GF2L_HIP(ctx, gf2l::launch_hbm_init(cv, ws, path, d_llr, F, s));
     if (cv.dvb_q) {  // the kernels read LLRs in the code's quasi-cyclic order
         GF2L_HIP(ctx, gf2l::launch_dvb_permute(cv, d_llr, ws.llr_perm, F, s));
         d_llr = ws.llr_perm;
     }
-    uint32_t running = F;  // frames entering the current iteration
-    auto collect = [&]() -> int {  // stream already synchronized
-        for (int k = 0; k < 3; ++k) {
-            float ms = 0.0f;
-            GF2L_HIP(ctx, hipEventElapsedTime(&ms, pf.ev[k], pf.ev[k + 1]));
-            pf.ms[k] += ms;
-            pf.launches[k] += 1;
-        }
-        pf.frame_iters += running;
+    const uint32_t batch = h_trace_c2v ? 1u : (uint32_t)std::max(1, std::min(ctx->tune.graph_iters, kProfileSlots));
+    uint32_t running = F;  // frames entering the first iteration of the current batch
+    uint32_t queued = 0;   // iterations enqueued since the last readback
+    // per-kernel times of the iterations of one batch (stream synchronized)
+    auto collect = [&](uint32_t iters) -> int {
+        for (uint32_t b = 0; b < iters; ++b)
+            for (int k = 0; k < 3; ++k) {
+                float ms = 0.0f;
+                GF2L_HIP(ctx, hipEventElapsedTime(&ms, pf.ev[4 * b + k], pf.ev[4 * b + k + 1]));
+                pf.ms[k] += ms;
+                pf.launches[k] += 1;
+            }
         return GF2L_OK;
     };
     const uint32_t n_phase = fb_it ? 2 : 1;
     uint32_t g = 0;  // iterations launched so far: parity of the running list
-    bool done = false;
-    for (uint32_t phase = 0; phase < n_phase && !done; ++phase) {
+    for (uint32_t phase = 0; phase < n_phase; ++phase) {
         const uint32_t n_it = phase == 0 ? max_it : fb_it;
         const float *w = phase == 0 ? d_w : nullptr;
         if (phase == 1 && path == gf2l::kStream) {  // restart the survivors from their LLRs
@@ -211,7 +231,7 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
             a.synd = d_synd;
             a.fb = d_fb;
             GF2L_HIP(ctx, gf2l::launch_hbm_iteration(cv, ws, path, a, ctx->num_cus, ctx->tune, s,
-                                                     pf.on ? pf.ev : nullptr));
+                                                     pf.on ? pf.ev + 4 * queued : nullptr));
             if (phase == 0 && h_trace_c2v && it <= trace_iters) {
                 const float *src_c = ws.c2v, *src_b = ws.belief;
                 if (path == gf2l::kState) {
@@ -224,12 +244,25 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
                 GF2L_HIP(ctx, hipMemcpyAsync(h_trace_bel + (size_t)(it - 1) * F * cv.n, src_b,
                                              (size_t)F * cv.n * sizeof(float), hipMemcpyDeviceToHost, s));
             }
-            // count of frames still running, read back to stop launching early
-            GF2L_HIP(ctx, hipMemcpyAsync(ws.h_count, ws.counts + ((g + 1) & 1u), sizeof(uint32_t),
-                                         hipMemcpyDeviceToHost, s));
+            // count of frames still running after this iteration: every
+            // iteration's when profiling (frames processed per launch), else
+            // the batch's last one, read back to stop launching early
+            const bool readback = queued + 1 >= batch || (it == n_it && phase + 1 == n_phase);
+            if (pf.on || readback)
+                GF2L_HIP(ctx, hipMemcpyAsync(ws.h_count + (pf.on ? queued : 0), ws.counts + ((g + 1) & 1u),
+                                             sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            ++queued;
+            if (!readback) continue;
             GF2L_HIP(ctx, hipStreamSynchronize(s));
-            if (pf.on && (rc = collect()) != GF2L_OK) return rc;
-            running = *ws.h_count;
+            if (pf.on) {
+                const int rc = collect(queued);
+                if (rc != GF2L_OK) return rc;
+                // frames entering each iteration of the batch
+                pf.frame_iters += running;
+                for (uint32_t b = 0; b + 1 < queued; ++b) pf.frame_iters += ws.h_count[b];
+            }
+            running = ws.h_count[pf.on ? queued - 1 : 0];
+            queued = 0;
             if (running == 0) {
                 if (phase == 0 && h_trace_c2v)
                     for (uint32_t t = it + 1; t <= trace_iters; ++t) {
@@ -239,8 +272,7 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
                                     h_trace_bel + (size_t)(it - 1) * F * cv.n,
                                     (size_t)F * cv.n * sizeof(float));
                     }
-                done = true;
-                break;
+                return GF2L_OK;
             }
         }
     }
@@ -393,23 +425,25 @@ extern "C" {
 int gf2l_create(int device, gf2l_ctx **out) {
     if (!out) return GF2L_ERR_INVALID;
     *out = nullptr;
-    int count = 0;
-    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
-        return GF2L_ERR_INVALID;
-    if (hipSetDevice(device) != hipSuccess) return GF2L_ERR_HIP;
-    gf2l_ctx *ctx = new gf2l_ctx;
-    ctx->device = device;
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete ctx;
-        return GF2L_ERR_HIP;
-    }
-    int cus = 0, lds = 0;
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-    hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
-    ctx->num_cus = cus > 0 ? cus : 256;
-    ctx->lds_limit = lds > 0 ? (size_t)lds : 65536;
-    *out = ctx;
-    return GF2L_OK;
+    return gf2l::guard(nullptr, [&]() -> int {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+            return GF2L_ERR_INVALID;
+        if (hipSetDevice(device) != hipSuccess) return GF2L_ERR_HIP;
+        gf2l_ctx *ctx = new gf2l_ctx;
+        ctx->device = device;
+        if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete ctx;
+            return GF2L_ERR_HIP;
+        }
+        int cus = 0, lds = 0;
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+        ctx->num_cus = cus > 0 ? cus : 256;
+        ctx->lds_limit = lds > 0 ? (size_t)lds : 65536;
+        *out = ctx;
+        return GF2L_OK;
+    });
 }
 
 void gf2l_destroy(gf2l_ctx *ctx) {
@@ -428,26 +462,30 @@ const char *gf2l_last_error(const gf2l_ctx *ctx) { return ctx ? ctx->err.c_str()
 
 int gf2l_synchronize(gf2l_ctx *ctx) {
     if (!ctx) return GF2L_ERR_INVALID;
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return GF2L_OK;
+    });
 }
 
 void *gf2l_stream(gf2l_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
 int gf2l_profile_enable(gf2l_ctx *ctx, int on) {
     if (!ctx) return GF2L_ERR_INVALID;
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    gf2l::Profile &pf = ctx->prof;
-    if (on && !pf.ev[0])
-        for (auto &e : pf.ev) GF2L_HIP(ctx, hipEventCreate(&e));
-    for (int k = 0; k < 3; ++k) {
-        pf.ms[k] = 0;
-        pf.launches[k] = 0;
-    }
-    pf.frame_iters = 0;
-    pf.on = on != 0;
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        gf2l::Profile &pf = ctx->prof;
+        if (on && !pf.ev[0])
+            for (auto &e : pf.ev) GF2L_HIP(ctx, hipEventCreate(&e));
+        for (int k = 0; k < 3; ++k) {
+            pf.ms[k] = 0;
+            pf.launches[k] = 0;
+        }
+        pf.frame_iters = 0;
+        pf.on = on != 0;
+        return GF2L_OK;
+    });
 }
 
 int gf2l_profile_read(gf2l_ctx *ctx, double *ms, uint64_t *launches, uint64_t *frame_iterations) {
@@ -462,42 +500,52 @@ int gf2l_profile_read(gf2l_ctx *ctx, double *ms, uint64_t *launches, uint64_t *f
 
 int gf2l_set_path(gf2l_ctx *ctx, int path) {
     if (!ctx) return GF2L_ERR_INVALID;
-    if (path < GF2L_PATH_AUTO || path > GF2L_PATH_STATE)
-        return fail(ctx, GF2L_ERR_INVALID, "unknown decoder path");
-    ctx->path = path;
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        if (path < GF2L_PATH_AUTO || path > GF2L_PATH_STATE)
+            return fail(ctx, GF2L_ERR_INVALID, "unknown decoder path");
+        ctx->path = path;
+        return GF2L_OK;
+    });
 }
 
 int gf2l_device_alloc(gf2l_ctx *ctx, size_t bytes, void **out) {
     if (!ctx || !out) return GF2L_ERR_INVALID;
     *out = nullptr;
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    GF2L_HIP(ctx, hipMalloc(out, bytes ? bytes : 1));
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        GF2L_HIP(ctx, hipMalloc(out, bytes ? bytes : 1));
+        return GF2L_OK;
+    });
 }
 
 int gf2l_device_free(gf2l_ctx *ctx, void *ptr) {
     if (!ctx) return GF2L_ERR_INVALID;
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    GF2L_HIP(ctx, hipFree(ptr));
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        GF2L_HIP(ctx, hipFree(ptr));
+        return GF2L_OK;
+    });
 }
 
 int gf2l_copy_to_device(gf2l_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (!ctx) return GF2L_ERR_INVALID;
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    GF2L_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        GF2L_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return GF2L_OK;
+    });
 }
 
 int gf2l_copy_to_host(gf2l_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (!ctx) return GF2L_ERR_INVALID;
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    GF2L_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        GF2L_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return GF2L_OK;
+    });
 }
 
 // True when the code is a staircase QC-IRA code: block-column nb-mb+j holds
@@ -519,90 +567,98 @@ int gf2l_frames_generate(gf2l_ctx *ctx, const gf2l_code *code, uint64_t seed, ui
                          size_t frames, float ebn0_lo_db, float ebn0_hi_db, float *d_llr,
                          uint64_t *d_codeword_words) {
     if (!ctx) return GF2L_ERR_INVALID;
-    if (!code || (!d_llr && frames)) return fail(ctx, GF2L_ERR_INVALID, "null argument");
-    if (frames > 0xffffffffull) return fail(ctx, GF2L_ERR_INVALID, "too many frames");
-    if (!is_staircase_ira(code) || code->view.z < 64)
-        return fail(ctx, GF2L_ERR_UNSUPPORTED,
-                    "frame generation needs a staircase QC-IRA code with lift >= 64");
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    GF2L_HIP(ctx, gf2l::launch_framegen(code->view, seed, first_frame, (uint32_t)frames, ebn0_lo_db,
-                                        ebn0_hi_db, d_llr, d_codeword_words, ctx->num_cus,
-                                        ctx->stream));
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        if (!code || (!d_llr && frames)) return fail(ctx, GF2L_ERR_INVALID, "null argument");
+        if (frames > 0xffffffffull) return fail(ctx, GF2L_ERR_INVALID, "too many frames");
+        if (!is_staircase_ira(code) || code->view.z < 64)
+            return fail(ctx, GF2L_ERR_UNSUPPORTED,
+                        "frame generation needs a staircase QC-IRA code with lift >= 64");
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        GF2L_HIP(ctx, gf2l::launch_framegen(code->view, seed, first_frame, (uint32_t)frames, ebn0_lo_db,
+                                            ebn0_hi_db, d_llr, d_codeword_words, ctx->num_cus,
+                                            ctx->stream));
+        return GF2L_OK;
+    });
 }
 
 int gf2l_frames_generate_zero(gf2l_ctx *ctx, const gf2l_code *code, uint64_t seed, uint64_t first_frame,
                               size_t frames, float ebn0_lo_db, float ebn0_hi_db, float *d_llr) {
     if (!ctx) return GF2L_ERR_INVALID;
-    if (!code || (!d_llr && frames)) return fail(ctx, GF2L_ERR_INVALID, "null argument");
-    if (frames > 0xffffffffull) return fail(ctx, GF2L_ERR_INVALID, "too many frames");
-    const uint32_t n = code->view.n, k = code->view.k;
-    if (n == 0 || k == 0) return fail(ctx, GF2L_ERR_INVALID, "code rate must be positive");
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    GF2L_HIP(ctx, gf2l::launch_framegen_zero(n, k, seed, first_frame, (uint32_t)frames, ebn0_lo_db, ebn0_hi_db,
-                                             d_llr, ctx->num_cus, ctx->stream));
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        if (!code || (!d_llr && frames)) return fail(ctx, GF2L_ERR_INVALID, "null argument");
+        if (frames > 0xffffffffull) return fail(ctx, GF2L_ERR_INVALID, "too many frames");
+        const uint32_t n = code->view.n, k = code->view.k;
+        if (n == 0 || k == 0) return fail(ctx, GF2L_ERR_INVALID, "code rate must be positive");
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        GF2L_HIP(ctx, gf2l::launch_framegen_zero(n, k, seed, first_frame, (uint32_t)frames, ebn0_lo_db, ebn0_hi_db,
+                                                 d_llr, ctx->num_cus, ctx->stream));
+        return GF2L_OK;
+    });
 }
 
 int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value) {
     if (!ctx) return GF2L_ERR_INVALID;
-    switch (key) {
-        case GF2L_TUNE_WG_PER_CU:
-            if (value < 0 || value > 32) return fail(ctx, GF2L_ERR_INVALID, "wg_per_cu out of [0,32]");
-            ctx->tune.wg_per_cu = value;
-            return GF2L_OK;
-        case GF2L_TUNE_XCD_MAP:
-            ctx->tune.xcd_map = value != 0;
-            return GF2L_OK;
-        case GF2L_TUNE_GRAPH_ITERS:
-            if (value < 1 || value > 64) return fail(ctx, GF2L_ERR_INVALID, "graph_iters out of [1,64]");
-            ctx->tune.graph_iters = value;
-            return GF2L_OK;
-        case GF2L_TUNE_CHUNK:
-            if (value < -1 || value > 64) return fail(ctx, GF2L_ERR_INVALID, "chunk out of [-1,64]");
-            ctx->tune.chunk = value;
-            return GF2L_OK;
-        default:
-            return fail(ctx, GF2L_ERR_INVALID, "unknown tuning key");
-    }
+    return gf2l::guard(ctx, [&]() -> int {
+        switch (key) {
+            case GF2L_TUNE_WG_PER_CU:
+                if (value < 0 || value > 32) return fail(ctx, GF2L_ERR_INVALID, "wg_per_cu out of [0,32]");
+                ctx->tune.wg_per_cu = value;
+                return GF2L_OK;
+            case GF2L_TUNE_XCD_MAP:
+                ctx->tune.xcd_map = value != 0;
+                return GF2L_OK;
+            case GF2L_TUNE_GRAPH_ITERS:
+                if (value < 1 || value > 64) return fail(ctx, GF2L_ERR_INVALID, "graph_iters out of [1,64]");
+                ctx->tune.graph_iters = value;
+                return GF2L_OK;
+            case GF2L_TUNE_CHUNK:
+                if (value < -1 || value > 64) return fail(ctx, GF2L_ERR_INVALID, "chunk out of [-1,64]");
+                ctx->tune.chunk = value;
+                return GF2L_OK;
+            default:
+                return fail(ctx, GF2L_ERR_INVALID, "unknown tuning key");
+        }
+    });
 }
 
 int gf2l_code_qc(gf2l_ctx *ctx, uint32_t mb, uint32_t nb, uint32_t z, const int32_t *shifts,
                  gf2l_code **out) {
     if (!ctx || !out) return GF2L_ERR_INVALID;
     *out = nullptr;
-    // QuasiCyclicLdpc::new panics (core.rs:519-540)
-    if (mb == 0) return fail(ctx, GF2L_ERR_INVALID, "Base matrix must have at least one row");
-    if (z == 0) return fail(ctx, GF2L_ERR_INVALID, "Expansion factor must be positive");
-    if (nb == 0 || !shifts) return fail(ctx, GF2L_ERR_INVALID, "empty base matrix");
-    if ((uint64_t)mb * z > (1ull << 28) || (uint64_t)nb * z > (1ull << 28))
-        return fail(ctx, GF2L_ERR_UNSUPPORTED, "code too large (more than 2^28 checks or variables)");
-    if (z > (1u << 26)) return fail(ctx, GF2L_ERR_UNSUPPORTED, "lift above 2^26");
-    for (uint32_t br = 0; br < mb; ++br)
-        for (uint32_t bc = 0; bc < nb; ++bc) {
-            const int32_t s = shifts[(size_t)br * nb + bc];
-            if (s != -1 && (s < 0 || (uint32_t)s >= z))  // core.rs:589-596
-                return fail(ctx, GF2L_ERR_INVALID,
-                            "Shift value " + std::to_string(s) + " at position (" + std::to_string(br) +
-                                "," + std::to_string(bc) + ") must be -1 or in range [0," +
-                                std::to_string(z) + ")");
+    return gf2l::guard(ctx, [&]() -> int {
+        // QuasiCyclicLdpc::new panics (core.rs:519-540)
+        if (mb == 0) return fail(ctx, GF2L_ERR_INVALID, "Base matrix must have at least one row");
+        if (z == 0) return fail(ctx, GF2L_ERR_INVALID, "Expansion factor must be positive");
+        if (nb == 0 || !shifts) return fail(ctx, GF2L_ERR_INVALID, "empty base matrix");
+        if ((uint64_t)mb * z > (1ull << 28) || (uint64_t)nb * z > (1ull << 28))
+            return fail(ctx, GF2L_ERR_UNSUPPORTED, "code too large (more than 2^28 checks or variables)");
+        if (z > (1u << 26)) return fail(ctx, GF2L_ERR_UNSUPPORTED, "lift above 2^26");
+        for (uint32_t br = 0; br < mb; ++br)
+            for (uint32_t bc = 0; bc < nb; ++bc) {
+                const int32_t s = shifts[(size_t)br * nb + bc];
+                if (s != -1 && (s < 0 || (uint32_t)s >= z))  // core.rs:589-596
+                    return fail(ctx, GF2L_ERR_INVALID,
+                                "Shift value " + std::to_string(s) + " at position (" + std::to_string(br) +
+                                    "," + std::to_string(bc) + ") must be -1 or in range [0," +
+                                    std::to_string(z) + ")");
+            }
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        gf2l_code *c = new gf2l_code;
+        c->device = ctx->device;
+        c->shifts.assign(shifts, shifts + (size_t)mb * nb);
+        c->h_row_ptr.assign(mb + 1, 0);
+        for (uint32_t br = 0; br < mb; ++br) {
+            for (uint32_t bc = 0; bc < nb; ++bc) {
+                const int32_t s = shifts[(size_t)br * nb + bc];
+                if (s >= 0) c->h_row_edge.push_back(make_uint2(bc, (uint32_t)s));
+            }
+            c->h_row_ptr[br + 1] = (uint32_t)c->h_row_edge.size();
         }
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    gf2l_code *c = new gf2l_code;
-    c->device = ctx->device;
-    c->shifts.assign(shifts, shifts + (size_t)mb * nb);
-    c->h_row_ptr.assign(mb + 1, 0);
-    for (uint32_t br = 0; br < mb; ++br) {
-        for (uint32_t bc = 0; bc < nb; ++bc) {
-            const int32_t s = shifts[(size_t)br * nb + bc];
-            if (s >= 0) c->h_row_edge.push_back(make_uint2(bc, (uint32_t)s));
-        }
-        c->h_row_ptr[br + 1] = (uint32_t)c->h_row_edge.size();
-    }
-    const int rc = gf2l::build_qc_view(ctx, c, mb, nb, z, 0);
-    if (rc != GF2L_OK) return rc;
-    *out = c;
-    return GF2L_OK;
+        const int rc = gf2l::build_qc_view(ctx, c, mb, nb, z, 0);
+        if (rc != GF2L_OK) return rc;
+        *out = c;
+        return GF2L_OK;
+    });
 }
 
 int gf2l_code_info(const gf2l_code *code, uint32_t *n, uint32_t *m, uint32_t *k, uint64_t *nnz) {
@@ -626,8 +682,10 @@ void gf2l_code_destroy(gf2l_code *code) {
 int gf2l_decode_batch(gf2l_ctx *ctx, const gf2l_code *code, const float *llr, size_t frames,
                       uint32_t max_it, uint64_t *msg_words, uint32_t *iterations,
                       uint8_t *converged, uint8_t *syndrome_ok) {
-    return decode_host(ctx, code, llr, frames, max_it, msg_words, iterations, converged, syndrome_ok,
-                       nullptr, nullptr, 0);
+    return gf2l::guard(ctx, [&]() -> int {
+        return decode_host(ctx, code, llr, frames, max_it, msg_words, iterations, converged, syndrome_ok,
+                           nullptr, nullptr, 0);
+    });
 }
 
 int gf2l_decode_batch_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr,
@@ -635,13 +693,15 @@ int gf2l_decode_batch_device(gf2l_ctx *ctx, const gf2l_code *code, const float *
                              uint32_t *d_iterations, uint8_t *d_converged,
                              uint8_t *d_syndrome_ok) {
     if (!ctx) return GF2L_ERR_INVALID;
-    if (!code) return fail(ctx, GF2L_ERR_INVALID, "null code");
-    if (frames == 0) return GF2L_OK;
-    if (!d_llr || !d_msg_words || !d_iterations || !d_converged || !d_syndrome_ok)
-        return fail(ctx, GF2L_ERR_INVALID, "null buffer");
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    return decode_device(ctx, code, d_llr, frames, max_it, nullptr, 0, d_msg_words, d_iterations,
-                         d_converged, d_syndrome_ok, nullptr, nullptr, nullptr, 0);
+    return gf2l::guard(ctx, [&]() -> int {
+        if (!code) return fail(ctx, GF2L_ERR_INVALID, "null code");
+        if (frames == 0) return GF2L_OK;
+        if (!d_llr || !d_msg_words || !d_iterations || !d_converged || !d_syndrome_ok)
+            return fail(ctx, GF2L_ERR_INVALID, "null buffer");
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        return decode_device(ctx, code, d_llr, frames, max_it, nullptr, 0, d_msg_words, d_iterations,
+                             d_converged, d_syndrome_ok, nullptr, nullptr, nullptr, 0);
+    });
 }
 
 int gf2l_decode_trace(gf2l_ctx *ctx, const gf2l_code *code, const float *llr, size_t frames,
@@ -649,11 +709,13 @@ int gf2l_decode_trace(gf2l_ctx *ctx, const gf2l_code *code, const float *llr, si
                       uint64_t *msg_words, uint32_t *iterations, uint8_t *converged,
                       uint8_t *syndrome_ok) {
     if (!ctx) return GF2L_ERR_INVALID;
-    if (trace_iters > max_it) return fail(ctx, GF2L_ERR_INVALID, "trace_iters > max_it");
-    if (trace_iters && (!c2v_out || !belief_out)) return fail(ctx, GF2L_ERR_INVALID, "null trace");
-    return decode_host(ctx, code, llr, frames, max_it, msg_words, iterations, converged, syndrome_ok,
-                       trace_iters ? c2v_out : nullptr, trace_iters ? belief_out : nullptr,
-                       trace_iters);
+    return gf2l::guard(ctx, [&]() -> int {
+        if (trace_iters > max_it) return fail(ctx, GF2L_ERR_INVALID, "trace_iters > max_it");
+        if (trace_iters && (!c2v_out || !belief_out)) return fail(ctx, GF2L_ERR_INVALID, "null trace");
+        return decode_host(ctx, code, llr, frames, max_it, msg_words, iterations, converged, syndrome_ok,
+                           trace_iters ? c2v_out : nullptr, trace_iters ? belief_out : nullptr,
+                           trace_iters);
+    });
 }
 
 }  // extern "C"

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -67,16 +69,22 @@ struct Workspace {
     uint64_t *hb = nullptr;      // [frames][nb][zw] hard decisions
     uint32_t *lists = nullptr;   // [2][frames] running frame lists
     uint32_t *counts = nullptr;  // [0..1] running counts; item counters from kSchedOffset
-    uint32_t *h_count = nullptr; // pinned host mirror of one count
+    uint32_t *h_count = nullptr; // pinned host mirror of kProfileSlots running counts
     float *llr_perm = nullptr;   // DVB codes: [frames][n] LLRs in quasi-cyclic order
     size_t cap_llr = 0;
 };
 
+// Iterations the HBM paths enqueue between two readbacks of the running
+// count at most (GF2L_TUNE_GRAPH_ITERS is clamped to it), and the number of
+// per-iteration event / count slots kept for them.
+constexpr int kProfileSlots = 64;
+
 // Accumulated per-kernel timings of the HBM paths (HIP events recorded
-// around each launch on the context's stream).
+// around each launch on the context's stream; four per iteration, for the
+// iterations enqueued between two readbacks).
 struct Profile {
     bool on = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[4 * kProfileSlots] = {};
     double ms[3] = {0, 0, 0};         // check, variable, syndrome kernels
     uint64_t launches[3] = {0, 0, 0};
     uint64_t frame_iters = 0;         // sum over launches of frames processed
@@ -146,8 +154,42 @@ namespace gf2l {
 
 // Records `msg` as the context's last error and returns `code`.
 inline int fail(gf2l_ctx *ctx, int code, const std::string &msg) {
-    if (ctx) ctx->err = msg;
+    if (ctx) {
+        try {
+            ctx->err = msg;
+        } catch (...) {
+            ctx->err.clear();
+        }
+    }
     return code;
+}
+
+// Runs f() and turns any C++ exception into an error code, so that none
+// crosses the C ABI: std::bad_alloc -> GF2L_ERR_NOMEM, anything else ->
+// GF2L_ERR_INTERNAL.  Every extern "C" entry point that allocates host
+// memory (vectors, strings, new) runs its body through this.
+template <typename F>
+int guard(gf2l_ctx *ctx, F &&f) noexcept {
+    auto note = [ctx](const char *msg) noexcept {
+        if (!ctx) return;
+        try {
+            ctx->err = msg;
+        } catch (...) {
+            ctx->err.clear();
+        }
+    };
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        note("host allocation failed");
+        return -3;  // GF2L_ERR_NOMEM
+    } catch (const std::exception &e) {
+        note(e.what());
+        return -5;  // GF2L_ERR_INTERNAL
+    } catch (...) {
+        note("unknown C++ exception");
+        return -5;
+    }
 }
 
 // Returns from the enclosing function with a GF2L error when a HIP call fails.
@@ -167,6 +209,18 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
                   uint32_t max_it, const float *d_w, uint32_t fb_it, uint64_t *d_msg,
                   uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb,
                   float *h_trace_c2v, float *h_trace_bel, uint32_t trace_iters);
+
+// Iteration loop of the HBM paths (STATE / STREAM) over a workspace the
+// caller has prepared (gf2ldpc_api.hip); allocates nothing.  Used by
+// decode_device after growing the context's workspace and by the
+// no-allocation entry point over its static scratch array.
+int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const float *d_llr, uint32_t F,
+            uint32_t max_it, const float *d_w, uint32_t fb_it, uint64_t *d_msg, uint32_t *d_iters,
+            uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb, float *h_trace_c2v, float *h_trace_bel,
+            uint32_t trace_iters, float *d_tc, float *d_tb);
+// Decoder path (GF2L_PATH_*) the context would run a quasi-cyclic code on,
+// or -1 when a forced path cannot run it.
+int resolve_path(const gf2l_ctx *ctx, const CodeView &cv);
 
 // Uploads the tables of a quasi-cyclic code whose block-row lists
 // (h_row_ptr, h_row_edge) are filled and sets its view.  dvb_q > 0: DVB

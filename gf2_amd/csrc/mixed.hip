@@ -746,49 +746,51 @@ extern "C" {
 int gf2l_codeset_create(gf2l_ctx *ctx, const gf2l_code *const *codes, uint32_t count, gf2l_codeset **out) {
     if (!ctx || !out) return GF2L_ERR_INVALID;
     *out = nullptr;
-    if (!codes || count == 0 || count > gf2l::kMaxCodes)
-        return fail(ctx, GF2L_ERR_INVALID, "a code set holds 1 to 16 codes");
-    std::vector<gf2l::CodeView> views;
-    uint32_t max_dc = 0, max_dv = 0, max_z = 0, max_words = 0;
-    for (uint32_t c = 0; c < count; ++c) {
-        const gf2l_code *code = codes[c];
-        if (!code) return fail(ctx, GF2L_ERR_INVALID, "null code in a code set");
-        if (code->is_csr || code->view.dvb_q)
-            return fail(ctx, GF2L_ERR_UNSUPPORTED, "code sets hold quasi-cyclic codes (not DVB)");
-        if (code->device != ctx->device) return fail(ctx, GF2L_ERR_INVALID, "code built on another device");
-        if (code->view.max_dc > gf2l::kMaxStateDegree)
-            return fail(ctx, GF2L_ERR_UNSUPPORTED, "check degree above 27 in a code set");
-        views.push_back(code->view);
-        max_dc = std::max(max_dc, code->view.max_dc);
-        max_dv = std::max(max_dv, code->view.max_dv);
-        max_z = std::max(max_z, code->view.z);
-        max_words = std::max(max_words, code->view.nb * code->view.zw);
-    }
-    if ((size_t)max_words * 8 > 65536)
-        return fail(ctx, GF2L_ERR_UNSUPPORTED, "hard decisions of a frame exceed 64 KiB of LDS");
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    gf2l_codeset *cs = new gf2l_codeset;
-    cs->codes.assign(codes, codes + count);
-    cs->max_dc = max_dc;
-    cs->max_dv = max_dv;
-    cs->max_z = max_z;
-    cs->max_words = max_words;
-    for (uint32_t c = 0; c < count; ++c) {
-        const uint32_t dc = codes[c]->view.max_dc, dv = codes[c]->view.max_dv;
-        cs->cn_mask[dc <= 8 ? 0 : dc <= 16 ? 1 : 2] |= 1u << c;
-        cs->vn_mask[dv <= 8 ? 0 : dv <= 16 ? 1 : 2] |= 1u << c;
-    }
-    cs->device = ctx->device;
-    hipError_t e = hipMalloc(&cs->d_views, count * sizeof(gf2l::CodeView));
-    if (e == hipSuccess)
-        e = hipMemcpy(cs->d_views, views.data(), count * sizeof(gf2l::CodeView), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        if (cs->d_views) hipFree(cs->d_views);
-        delete cs;
-        return fail(ctx, GF2L_ERR_HIP, std::string("code set upload: ") + hipGetErrorString(e));
-    }
-    *out = cs;
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        if (!codes || count == 0 || count > gf2l::kMaxCodes)
+            return fail(ctx, GF2L_ERR_INVALID, "a code set holds 1 to 16 codes");
+        std::vector<gf2l::CodeView> views;
+        uint32_t max_dc = 0, max_dv = 0, max_z = 0, max_words = 0;
+        for (uint32_t c = 0; c < count; ++c) {
+            const gf2l_code *code = codes[c];
+            if (!code) return fail(ctx, GF2L_ERR_INVALID, "null code in a code set");
+            if (code->is_csr || code->view.dvb_q)
+                return fail(ctx, GF2L_ERR_UNSUPPORTED, "code sets hold quasi-cyclic codes (not DVB)");
+            if (code->device != ctx->device) return fail(ctx, GF2L_ERR_INVALID, "code built on another device");
+            if (code->view.max_dc > gf2l::kMaxStateDegree)
+                return fail(ctx, GF2L_ERR_UNSUPPORTED, "check degree above 27 in a code set");
+            views.push_back(code->view);
+            max_dc = std::max(max_dc, code->view.max_dc);
+            max_dv = std::max(max_dv, code->view.max_dv);
+            max_z = std::max(max_z, code->view.z);
+            max_words = std::max(max_words, code->view.nb * code->view.zw);
+        }
+        if ((size_t)max_words * 8 > 65536)
+            return fail(ctx, GF2L_ERR_UNSUPPORTED, "hard decisions of a frame exceed 64 KiB of LDS");
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        gf2l_codeset *cs = new gf2l_codeset;
+        cs->codes.assign(codes, codes + count);
+        cs->max_dc = max_dc;
+        cs->max_dv = max_dv;
+        cs->max_z = max_z;
+        cs->max_words = max_words;
+        for (uint32_t c = 0; c < count; ++c) {
+            const uint32_t dc = codes[c]->view.max_dc, dv = codes[c]->view.max_dv;
+            cs->cn_mask[dc <= 8 ? 0 : dc <= 16 ? 1 : 2] |= 1u << c;
+            cs->vn_mask[dv <= 8 ? 0 : dv <= 16 ? 1 : 2] |= 1u << c;
+        }
+        cs->device = ctx->device;
+        hipError_t e = hipMalloc(&cs->d_views, count * sizeof(gf2l::CodeView));
+        if (e == hipSuccess)
+            e = hipMemcpy(cs->d_views, views.data(), count * sizeof(gf2l::CodeView), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            if (cs->d_views) hipFree(cs->d_views);
+            delete cs;
+            return fail(ctx, GF2L_ERR_HIP, std::string("code set upload: ") + hipGetErrorString(e));
+        }
+        *out = cs;
+        return GF2L_OK;
+    });
 }
 
 void gf2l_codeset_destroy(gf2l_codeset *cs) {
@@ -803,8 +805,10 @@ void gf2l_codeset_destroy(gf2l_codeset *cs) {
 int gf2l_decode_mixed_device(gf2l_ctx *ctx, gf2l_codeset *cs, const uint16_t *code_ids, size_t frames,
                              const float *d_llr, uint32_t max_it, uint64_t *d_msg_words,
                              uint32_t *d_iterations, uint8_t *d_converged, uint8_t *d_syndrome_ok) {
-    return gf2l::decode_mixed(ctx, cs, nullptr, code_ids, frames, d_llr, max_it, d_msg_words, d_iterations,
-                              d_converged, d_syndrome_ok, nullptr);
+    return gf2l::guard(ctx, [&]() -> int {
+        return gf2l::decode_mixed(ctx, cs, nullptr, code_ids, frames, d_llr, max_it, d_msg_words, d_iterations,
+                                  d_converged, d_syndrome_ok, nullptr);
+    });
 }
 
 int gf2l_decode_mixed_nms_device(gf2l_ctx *ctx, gf2l_codeset *cs, const gf2l_nms *const *decoders,
@@ -812,9 +816,11 @@ int gf2l_decode_mixed_nms_device(gf2l_ctx *ctx, gf2l_codeset *cs, const gf2l_nms
                                  uint32_t fallback_max_it, uint64_t *d_msg_words, uint32_t *d_iterations,
                                  uint8_t *d_converged, uint8_t *d_syndrome_ok, uint8_t *d_fallback) {
     if (!ctx) return GF2L_ERR_INVALID;
-    if (!decoders) return fail(ctx, GF2L_ERR_INVALID, "null decoder list");
-    return gf2l::decode_mixed(ctx, cs, decoders, code_ids, frames, d_llr, fallback_max_it, d_msg_words,
-                              d_iterations, d_converged, d_syndrome_ok, d_fallback);
+    return gf2l::guard(ctx, [&]() -> int {
+        if (!decoders) return fail(ctx, GF2L_ERR_INVALID, "null decoder list");
+        return gf2l::decode_mixed(ctx, cs, decoders, code_ids, frames, d_llr, fallback_max_it, d_msg_words,
+                                  d_iterations, d_converged, d_syndrome_ok, d_fallback);
+    });
 }
 
 }  // extern "C"

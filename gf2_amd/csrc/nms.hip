@@ -367,32 +367,34 @@ int gf2l_nms_create(gf2l_ctx *ctx, const gf2l_code *code, uint32_t iterations,
                     const float *init_weights, float init_value, gf2l_nms **out) {
     if (!ctx || !out) return GF2L_ERR_INVALID;
     *out = nullptr;
-    if (!code || iterations == 0) return fail(ctx, GF2L_ERR_INVALID, "need a code and iterations >= 1");
-    if (code->is_csr || code->view.dvb_q)
-        return fail(ctx, GF2L_ERR_UNSUPPORTED, "learned weights need a quasi-cyclic code (not DVB)");
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    gf2l_nms *q = new gf2l_nms;
-    q->code = code;
-    q->T = iterations;
-    q->nbe = code->view.nbe;
-    const size_t nw = (size_t)q->T * q->nbe;
-    std::vector<float> w(nw, init_value);
-    if (init_weights) std::memcpy(w.data(), init_weights, nw * sizeof(float));
-    hipError_t e = hipMalloc(&q->d_w, nw * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&q->d_m, nw * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&q->d_v, nw * sizeof(float));
-    if (e == hipSuccess) e = hipMemcpy(q->d_w, w.data(), nw * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(q->d_m, 0, nw * sizeof(float));
-    if (e == hipSuccess) e = hipMemset(q->d_v, 0, nw * sizeof(float));
-    if (e != hipSuccess) {
-        hipFree(q->d_w);
-        hipFree(q->d_m);
-        hipFree(q->d_v);
-        delete q;
-        return fail(ctx, GF2L_ERR_HIP, std::string("nms create: ") + hipGetErrorString(e));
-    }
-    *out = q;
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        if (!code || iterations == 0) return fail(ctx, GF2L_ERR_INVALID, "need a code and iterations >= 1");
+        if (code->is_csr || code->view.dvb_q)
+            return fail(ctx, GF2L_ERR_UNSUPPORTED, "learned weights need a quasi-cyclic code (not DVB)");
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        gf2l_nms *q = new gf2l_nms;
+        q->code = code;
+        q->T = iterations;
+        q->nbe = code->view.nbe;
+        const size_t nw = (size_t)q->T * q->nbe;
+        std::vector<float> w(nw, init_value);
+        if (init_weights) std::memcpy(w.data(), init_weights, nw * sizeof(float));
+        hipError_t e = hipMalloc(&q->d_w, nw * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&q->d_m, nw * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&q->d_v, nw * sizeof(float));
+        if (e == hipSuccess) e = hipMemcpy(q->d_w, w.data(), nw * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(q->d_m, 0, nw * sizeof(float));
+        if (e == hipSuccess) e = hipMemset(q->d_v, 0, nw * sizeof(float));
+        if (e != hipSuccess) {
+            hipFree(q->d_w);
+            hipFree(q->d_m);
+            hipFree(q->d_v);
+            delete q;
+            return fail(ctx, GF2L_ERR_HIP, std::string("nms create: ") + hipGetErrorString(e));
+        }
+        *out = q;
+        return GF2L_OK;
+    });
 }
 
 void gf2l_nms_destroy(gf2l_nms *q) {
@@ -415,91 +417,105 @@ int gf2l_nms_info(const gf2l_nms *q, uint32_t *iterations, uint32_t *weights_per
 
 int gf2l_nms_get_weights(gf2l_ctx *ctx, const gf2l_nms *q, float *w) {
     if (!ctx || !q || !w) return GF2L_ERR_INVALID;
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    GF2L_HIP(ctx, hipMemcpy(w, q->d_w, (size_t)q->T * q->nbe * sizeof(float), hipMemcpyDeviceToHost));
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        GF2L_HIP(ctx, hipMemcpy(w, q->d_w, (size_t)q->T * q->nbe * sizeof(float), hipMemcpyDeviceToHost));
+        return GF2L_OK;
+    });
 }
 
 int gf2l_nms_set_weights(gf2l_ctx *ctx, gf2l_nms *q, const float *w) {
     if (!ctx || !q || !w) return GF2L_ERR_INVALID;
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    GF2L_HIP(ctx, hipMemcpy(q->d_w, w, (size_t)q->T * q->nbe * sizeof(float), hipMemcpyHostToDevice));
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        GF2L_HIP(ctx, hipMemcpy(q->d_w, w, (size_t)q->T * q->nbe * sizeof(float), hipMemcpyHostToDevice));
+        return GF2L_OK;
+    });
 }
 
 int gf2l_nms_get_adam(gf2l_ctx *ctx, const gf2l_nms *q, float *m, float *v, uint64_t *step) {
     if (!ctx || !q) return GF2L_ERR_INVALID;
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    const size_t b = (size_t)q->T * q->nbe * sizeof(float);
-    if (m) GF2L_HIP(ctx, hipMemcpy(m, q->d_m, b, hipMemcpyDeviceToHost));
-    if (v) GF2L_HIP(ctx, hipMemcpy(v, q->d_v, b, hipMemcpyDeviceToHost));
-    if (step) *step = q->step;
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const size_t b = (size_t)q->T * q->nbe * sizeof(float);
+        if (m) GF2L_HIP(ctx, hipMemcpy(m, q->d_m, b, hipMemcpyDeviceToHost));
+        if (v) GF2L_HIP(ctx, hipMemcpy(v, q->d_v, b, hipMemcpyDeviceToHost));
+        if (step) *step = q->step;
+        return GF2L_OK;
+    });
 }
 
 int gf2l_nms_set_adam(gf2l_ctx *ctx, gf2l_nms *q, const float *m, const float *v, uint64_t step) {
     if (!ctx || !q || !m || !v) return GF2L_ERR_INVALID;
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    const size_t b = (size_t)q->T * q->nbe * sizeof(float);
-    GF2L_HIP(ctx, hipMemcpy(q->d_m, m, b, hipMemcpyHostToDevice));
-    GF2L_HIP(ctx, hipMemcpy(q->d_v, v, b, hipMemcpyHostToDevice));
-    q->step = step;
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const size_t b = (size_t)q->T * q->nbe * sizeof(float);
+        GF2L_HIP(ctx, hipMemcpy(q->d_m, m, b, hipMemcpyHostToDevice));
+        GF2L_HIP(ctx, hipMemcpy(q->d_v, v, b, hipMemcpyHostToDevice));
+        q->step = step;
+        return GF2L_OK;
+    });
 }
 
 int gf2l_nms_gradient(gf2l_ctx *ctx, gf2l_nms *q, const float *d_llr, const uint64_t *d_cw,
                       size_t frames, float *grad, float *loss) {
     if (!ctx || !q || !d_llr || !d_cw) return GF2L_ERR_INVALID;
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    int rc = gf2l::forward_backward(ctx, q, d_llr, d_cw, frames);
-    if (rc != GF2L_OK) return rc;
-    GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (grad)
-        GF2L_HIP(ctx, hipMemcpy(grad, q->grad, (size_t)q->T * q->nbe * sizeof(float), hipMemcpyDeviceToHost));
-    if (loss) {
-        float l = 0.0f;
-        GF2L_HIP(ctx, hipMemcpy(&l, q->loss, sizeof(float), hipMemcpyDeviceToHost));
-        *loss = l / ((float)q->T * (float)q->code->view.n * (float)frames);
-    }
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        int rc = gf2l::forward_backward(ctx, q, d_llr, d_cw, frames);
+        if (rc != GF2L_OK) return rc;
+        GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (grad)
+            GF2L_HIP(ctx, hipMemcpy(grad, q->grad, (size_t)q->T * q->nbe * sizeof(float), hipMemcpyDeviceToHost));
+        if (loss) {
+            float l = 0.0f;
+            GF2L_HIP(ctx, hipMemcpy(&l, q->loss, sizeof(float), hipMemcpyDeviceToHost));
+            *loss = l / ((float)q->T * (float)q->code->view.n * (float)frames);
+        }
+        return GF2L_OK;
+    });
 }
 
 int gf2l_nms_train_step(gf2l_ctx *ctx, gf2l_nms *q, const float *d_llr, const uint64_t *d_cw,
                         size_t frames, float lr, float *loss) {
     if (!ctx || !q || !d_llr || !d_cw) return GF2L_ERR_INVALID;
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    int rc = gf2l::forward_backward(ctx, q, d_llr, d_cw, frames);
-    if (rc != GF2L_OK) return rc;
-    q->step += 1;
-    const uint32_t n = q->T * q->nbe;
-    const float bc1 = 1.0f - std::pow(0.9f, (float)q->step);
-    const float bc2 = 1.0f - std::pow(0.999f, (float)q->step);
-    hipLaunchKernelGGL(gf2l::adam_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, q->d_w,
-                       q->d_m, q->d_v, q->grad, n, lr, bc1, bc2);
-    GF2L_HIP(ctx, hipGetLastError());
-    if (loss) {
-        float l = 0.0f;
-        GF2L_HIP(ctx, hipMemcpyAsync(&l, q->loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-        GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        *loss = l / ((float)q->T * (float)q->code->view.n * (float)frames);
-    }
-    return GF2L_OK;
+    return gf2l::guard(ctx, [&]() -> int {
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        int rc = gf2l::forward_backward(ctx, q, d_llr, d_cw, frames);
+        if (rc != GF2L_OK) return rc;
+        q->step += 1;
+        const uint32_t n = q->T * q->nbe;
+        const float bc1 = 1.0f - std::pow(0.9f, (float)q->step);
+        const float bc2 = 1.0f - std::pow(0.999f, (float)q->step);
+        hipLaunchKernelGGL(gf2l::adam_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, q->d_w,
+                           q->d_m, q->d_v, q->grad, n, lr, bc1, bc2);
+        GF2L_HIP(ctx, hipGetLastError());
+        if (loss) {
+            float l = 0.0f;
+            GF2L_HIP(ctx, hipMemcpyAsync(&l, q->loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+            GF2L_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            *loss = l / ((float)q->T * (float)q->code->view.n * (float)frames);
+        }
+        return GF2L_OK;
+    });
 }
 
 int gf2l_nms_decode_device(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr, size_t frames,
                            uint32_t fallback_max_it, uint64_t *d_msg_words, uint32_t *d_iterations,
                            uint8_t *d_converged, uint8_t *d_syndrome_ok, uint8_t *d_fallback) {
     if (!ctx || !q) return GF2L_ERR_INVALID;
-    if (frames == 0) return GF2L_OK;
-    if (!d_llr || !d_msg_words || !d_iterations || !d_converged || !d_syndrome_ok)
-        return fail(ctx, GF2L_ERR_INVALID, "null buffer");
-    GF2L_HIP(ctx, hipSetDevice(ctx->device));
-    return gf2l::decode_device(ctx, q->code, d_llr, frames, q->T, q->d_w, fallback_max_it, d_msg_words,
-                               d_iterations, d_converged, d_syndrome_ok, d_fallback, nullptr, nullptr, 0);
+    return gf2l::guard(ctx, [&]() -> int {
+        if (frames == 0) return GF2L_OK;
+        if (!d_llr || !d_msg_words || !d_iterations || !d_converged || !d_syndrome_ok)
+            return fail(ctx, GF2L_ERR_INVALID, "null buffer");
+        GF2L_HIP(ctx, hipSetDevice(ctx->device));
+        return gf2l::decode_device(ctx, q->code, d_llr, frames, q->T, q->d_w, fallback_max_it, d_msg_words,
+                                   d_iterations, d_converged, d_syndrome_ok, d_fallback, nullptr, nullptr, 0);
+    });
 }
 
 }  // extern "C"

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "gf2ldpc.h"
+#include "internal.h"
 
 namespace {
 
@@ -54,38 +55,40 @@ bool closes_4cycle(const std::vector<int32_t> &sh, uint32_t mb, uint32_t nb, uin
 
 extern "C" int gf2l_qc_ira_shifts(uint64_t seed, uint32_t mb, uint32_t nb, uint32_t z,
                                   const uint32_t *info_col_weight, int32_t *shifts_out) {
-    if (!info_col_weight || !shifts_out || mb == 0 || z == 0 || nb <= mb) return GF2L_ERR_INVALID;
-    const uint32_t kb = nb - mb;
-    for (uint32_t c = 0; c < kb; ++c)
-        if (info_col_weight[c] == 0 || info_col_weight[c] > mb) return GF2L_ERR_INVALID;
-    std::vector<int32_t> sh((size_t)mb * nb, -1);
-    std::vector<uint32_t> deg(mb, 0);
-    for (uint32_t j = 0; j < mb; ++j) {
-        sh[(size_t)j * nb + kb + j] = 0;
-        ++deg[j];
-        if (j + 1 < mb) {
-            sh[(size_t)(j + 1) * nb + kb + j] = 0;
-            ++deg[j + 1];
-        }
-    }
-    SplitMix64 rng{seed};
-    std::vector<std::pair<uint64_t, uint32_t>> order(mb);
-    for (uint32_t c = 0; c < kb; ++c) {
-        for (uint32_t r = 0; r < mb; ++r) order[r] = {((uint64_t)deg[r] << 40) | (rng.next() >> 24), r};
-        std::sort(order.begin(), order.end());
-        std::vector<uint32_t> rows;
-        for (uint32_t t = 0; t < info_col_weight[c]; ++t) rows.push_back(order[t].second);
-        std::sort(rows.begin(), rows.end());
-        for (uint32_t r : rows) {
-            int64_t s = 0;
-            for (int attempt = 0; attempt < 64; ++attempt) {
-                s = (int64_t)(rng.next() % z);
-                if (!closes_4cycle(sh, mb, nb, z, r, c, s)) break;
+    return gf2l::guard(nullptr, [&]() -> int {
+        if (!info_col_weight || !shifts_out || mb == 0 || z == 0 || nb <= mb) return GF2L_ERR_INVALID;
+        const uint32_t kb = nb - mb;
+        for (uint32_t c = 0; c < kb; ++c)
+            if (info_col_weight[c] == 0 || info_col_weight[c] > mb) return GF2L_ERR_INVALID;
+        std::vector<int32_t> sh((size_t)mb * nb, -1);
+        std::vector<uint32_t> deg(mb, 0);
+        for (uint32_t j = 0; j < mb; ++j) {
+            sh[(size_t)j * nb + kb + j] = 0;
+            ++deg[j];
+            if (j + 1 < mb) {
+                sh[(size_t)(j + 1) * nb + kb + j] = 0;
+                ++deg[j + 1];
             }
-            sh[(size_t)r * nb + c] = (int32_t)s;
-            ++deg[r];
         }
-    }
-    std::copy(sh.begin(), sh.end(), shifts_out);
-    return GF2L_OK;
+        SplitMix64 rng{seed};
+        std::vector<std::pair<uint64_t, uint32_t>> order(mb);
+        for (uint32_t c = 0; c < kb; ++c) {
+            for (uint32_t r = 0; r < mb; ++r) order[r] = {((uint64_t)deg[r] << 40) | (rng.next() >> 24), r};
+            std::sort(order.begin(), order.end());
+            std::vector<uint32_t> rows;
+            for (uint32_t t = 0; t < info_col_weight[c]; ++t) rows.push_back(order[t].second);
+            std::sort(rows.begin(), rows.end());
+            for (uint32_t r : rows) {
+                int64_t s = 0;
+                for (int attempt = 0; attempt < 64; ++attempt) {
+                    s = (int64_t)(rng.next() % z);
+                    if (!closes_4cycle(sh, mb, nb, z, r, c, s)) break;
+                }
+                sh[(size_t)r * nb + c] = (int32_t)s;
+                ++deg[r];
+            }
+        }
+        std::copy(sh.begin(), sh.end(), shifts_out);
+        return GF2L_OK;
+    });
 }

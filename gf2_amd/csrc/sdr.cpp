@@ -5,6 +5,7 @@
 // one stream), the code and the host staging of one call; it is not
 // thread-safe.
 #include <cstdint>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -22,19 +23,29 @@ struct gf2_ldpc_decoder {
 
 namespace {
 
-int32_t sdr_fail(gf2_ldpc_decoder *d, const std::string &msg) {
-    d->err = msg;
+int32_t sdr_fail(gf2_ldpc_decoder *d, const char *msg) noexcept {
+    try {
+        d->err = msg;
+    } catch (...) {
+        d->err.clear();
+    }
     return -2;
 }
 
 // Runs `frames` frames through gf2l_decode_batch and unpacks the first k hard
-// decisions of each into bytes (0 or 1).
-int32_t sdr_decode(gf2_ldpc_decoder *d, const float *llrs, size_t frames, uint8_t *output, int32_t *iterations) {
+// decisions of each into bytes (0 or 1).  No exception leaves: a failed host
+// allocation of the staging vectors is an error return.
+int32_t sdr_decode(gf2_ldpc_decoder *d, const float *llrs, size_t frames, uint8_t *output,
+                   int32_t *iterations) noexcept {
     const size_t kw = (d->k + 63) / 64;
-    d->words.resize(frames * kw);
-    d->iters.resize(frames);
-    d->conv.resize(frames);
-    d->synd.resize(frames);
+    try {
+        d->words.resize(frames * kw);
+        d->iters.resize(frames);
+        d->conv.resize(frames);
+        d->synd.resize(frames);
+    } catch (...) {
+        return sdr_fail(d, "host allocation failed");
+    }
     const int rc = gf2l_decode_batch(d->ctx, d->code, llrs, frames, d->max_it, d->words.data(), d->iters.data(),
                                      d->conv.data(), d->synd.data());
     if (rc != GF2L_OK) return sdr_fail(d, gf2l_last_error(d->ctx));
@@ -56,7 +67,8 @@ extern "C" {
 
 gf2_ldpc_decoder *gf2_ldpc_decoder_create(uint32_t frame_size, uint32_t code_rate, uint32_t max_iterations) {
     if (frame_size > 1 || code_rate > 5) return nullptr;
-    gf2_ldpc_decoder *d = new gf2_ldpc_decoder;
+    gf2_ldpc_decoder *d = new (std::nothrow) gf2_ldpc_decoder;
+    if (!d) return nullptr;
     if (gf2l_create(0, &d->ctx) != GF2L_OK ||
         gf2l_code_dvb_t2(d->ctx, (int)frame_size, (int)code_rate, &d->code) != GF2L_OK) {
         if (d->ctx) gf2l_destroy(d->ctx);

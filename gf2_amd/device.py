@@ -33,12 +33,13 @@ class DeviceBuffer:
         _lib.check(self.ctx.handle, _lib.load().gf2l_copy_to_device(
             self.ctx.handle, self.ptr, a.ctypes.data, a.nbytes))
 
-    def download(self, dtype, shape) -> np.ndarray:
+    def download(self, dtype, shape, offset: int = 0) -> np.ndarray:
+        """Copies np.prod(shape) elements starting `offset` bytes into the buffer."""
         out = np.empty(shape, dtype=dtype)
-        if out.nbytes > self.nbytes:
+        if offset < 0 or offset + out.nbytes > self.nbytes:
             raise ValueError("requested more bytes than the buffer holds")
         _lib.check(self.ctx.handle, _lib.load().gf2l_copy_to_host(
-            self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes))
+            self.ctx.handle, out.ctypes.data, self.ptr + offset, out.nbytes))
         return out
 
     def free(self) -> None:

@@ -133,6 +133,59 @@ def sweep(code: LdpcCode, ebn0_list: List[float], frames_per_batch: int, max_fra
     return out
 
 
+def _rust_float(x: float) -> str:
+    """Rust's `{}` Display of an f64: the shortest digits that round-trip,
+    positional, no exponent, no trailing '.0' (3.0 -> "3", 1e-5 -> "0.00001")."""
+    return np.format_float_positional(float(x), unique=True, trim="-")
+
+
+@dataclass
+class SimulationResult:
+    """crates/gf2-coding/src/simulation.rs:41-96: one Eb/N0 point."""
+    eb_n0_db: float
+    ber: float
+    num_bits: int
+    num_errors: int
+    fer: Optional[float] = None
+    num_frames: Optional[int] = None
+    num_frame_errors: Optional[int] = None
+
+    @classmethod
+    def from_point(cls, p: "FerPoint", k: int) -> "SimulationResult":
+        return cls(p.ebn0_db, p.ber, p.frames * k, p.bit_errors, p.fer, p.frames, p.frame_errors)
+
+    def is_complete(self, min_errors: int) -> bool:
+        """simulation.rs:79-81: enough bit errors collected."""
+        return self.num_errors >= min_errors
+
+    def to_csv_row(self) -> str:
+        """simulation.rs:83-96."""
+        head = [_rust_float(self.eb_n0_db), _rust_float(self.ber), str(self.num_bits), str(self.num_errors)]
+        if self.fer is not None and self.num_frames is not None and self.num_frame_errors is not None:
+            head += [_rust_float(self.fer), str(self.num_frames), str(self.num_frame_errors)]
+        return ",".join(head)
+
+
+def results_to_csv(results: List[SimulationResult], include_header: bool = True) -> str:
+    """SimulationRunner::results_to_csv (simulation.rs:209-229)."""
+    out = ""
+    if include_header:
+        if any(r.fer is not None for r in results):
+            out += "eb_n0_db,ber,num_bits,num_errors,fer,num_frames,num_frame_errors\n"
+        else:
+            out += "eb_n0_db,ber,num_bits,num_errors\n"
+    for r in results:
+        out += r.to_csv_row() + "\n"
+    return out
+
+
+def write_csv(points: List[FerPoint], path: str, k: int) -> None:
+    """Writes sweep points of a code with k information bits per frame as
+    results_to_csv with a header."""
+    with open(path, "w") as f:
+        f.write(results_to_csv([SimulationResult.from_point(p, k) for p in points], True))
+
+
 def calibrate(code: LdpcCode, target_fer: float, lo: float, hi: float, frames: int = 2048,
               steps: int = 7, decoder: Optional[Callable] = None, seed: int = 5, log=sys.stderr) -> float:
     """Eb/N0 (dB) where the FER crosses target_fer, by bisection on [lo, hi]
@@ -193,6 +246,7 @@ def main(argv=None) -> int:
     ap.add_argument("--target-fer", type=float, default=None,
                     help="find the Eb/N0 of this FER by bisection over the --ebn0 range ends")
     ap.add_argument("--json", default=None)
+    ap.add_argument("--csv", default=None, help="also write the points with results_to_csv's columns")
     ap.add_argument("--zero", action="store_true",
                     help="all-zero codewords (automatic for codes the device encoder does not cover)")
     a = ap.parse_args(argv)
@@ -213,6 +267,8 @@ def main(argv=None) -> int:
     if a.json:
         with open(a.json, "w") as f:
             json.dump({"code": label, "points": [asdict(p) for p in pts]}, f, indent=1)
+    if a.csv:
+        write_csv(pts, a.csv, code.k())
     return 0
 
 

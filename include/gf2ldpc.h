@@ -15,7 +15,9 @@
  *   - Every entry point returns GF2L_OK (0) or a negative GF2L_ERR_* code;
  *     gf2l_last_error() then holds a message.  Where the reference panics
  *     (invalid shift, LLR length mismatch) these calls return
- *     GF2L_ERR_INVALID with the reference's panic text.
+ *     GF2L_ERR_INVALID with the reference's panic text.  No C++ exception
+ *     crosses this boundary: a failed host allocation returns
+ *     GF2L_ERR_NOMEM, any other exception GF2L_ERR_INTERNAL.
  *   - A context owns one HIP stream and its device buffers.  It is not
  *     thread-safe: use one context per calling thread (the reference gives
  *     each rayon task its own decoder, core.rs:737-743).
@@ -41,6 +43,7 @@ extern "C" {
 #define GF2L_ERR_HIP (-2)         /* HIP runtime error                           */
 #define GF2L_ERR_NOMEM (-3)       /* device or host allocation failed            */
 #define GF2L_ERR_UNSUPPORTED (-4) /* shape outside what the kernels handle       */
+#define GF2L_ERR_INTERNAL (-5)    /* unexpected C++ exception inside the library */
 
 typedef struct gf2l_ctx gf2l_ctx;
 typedef struct gf2l_code gf2l_code;
@@ -91,9 +94,11 @@ int gf2l_set_path(gf2l_ctx *ctx, int path);
  *   GF2L_TUNE_XCD_MAP    1 (default): each frame's block-rows / block-columns
  *                        run on one XCD so its beliefs and check states are
  *                        re-read from that XCD's L2; 0: frame-major over the grid
- *   GF2L_TUNE_GRAPH_ITERS iterations captured per graph replay of the mixed-code
- *                        decoder (1..64, default 4): the host checks for running
- *                        frames once per replay
+ *   GF2L_TUNE_GRAPH_ITERS iterations per host check for running frames (1..64,
+ *                        default 4): captured per graph replay by the mixed-code
+ *                        decoder, enqueued back to back by the single-code
+ *                        decoder (iterations after the last frame retired do
+ *                        nothing)
  *   GF2L_TUNE_CHUNK      consecutive items a workgroup of the check-state
  *                        kernels takes per shared-counter fetch (1..64);
  *                        0: fixed-stride item assignment; -1 (default):

@@ -112,6 +112,16 @@ void or_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]
  * (see gf2l_frames_generate): writes n LLRs and ceil(n/64) codeword words. */
 void or_framegen(const int32_t *shifts, uint32_t mb, uint32_t nb, uint32_t z, uint64_t seed,
                  uint64_t g, float ebn0_lo, float ebn0_hi, float *llr, uint64_t *cw_words);
+/* All-zero codeword frame g of an n-bit, rate k/n code (same Eb/N0 and noise
+ * streams as or_framegen with every bit 0). */
+void or_framegen_zero(uint32_t n, uint32_t k, uint64_t seed, uint64_t g, float ebn0_lo, float ebn0_hi,
+                      float *llr);
+
+/* ---- structural restatement of the rayon path (ref_port.cpp) ---------- */
+void *port_code_new(uint32_t m, uint32_t n, const uint32_t *rc, size_t nnz);
+void port_code_free(void *code);
+void port_decode_batch(const void *code, const float *llr, size_t frames, uint32_t max_it,
+                       uint64_t *msg_words, or_result *res, int threads);
 
 #ifdef __cplusplus
 }

@@ -160,7 +160,7 @@ def test_gf2data_codes_match_oracle(ctx, tmp_path):
     b = gf2data.SpBitMatrixDual.from_coo(s.m, s.n, e).to_bytes()
     _, _, llr2 = dvb_random_frames(s.table, s.n, s.k, s.q, 16, 1.7, seed=2)
     same(decode_batch_arrays(LdpcCode.from_gf2data(b, ctx), llr2, 30),
-         decode_batch_arrays(LdpcCode.dvb_t2_short(CodeRate.RATE_3_5, ctx), llr2, 30), "dvb file")
+         oracle_code(s).decode_batch(llr2, 30, threads=8), "dvb file")
 
 
 def test_gf2data_loader_errors(ctx):
@@ -187,7 +187,10 @@ def test_gf2data_loader_errors(ctx):
     assert LdpcCode.from_gf2data(bytes(chk) + bytes(32), ctx).nnz() == 4
 
 
-def test_sdr_api_matches_batch_decoder(ctx):
+def test_sdr_api_matches_oracle(ctx):
+    """The GNU Radio-style entry points (SDR_INTEGRATION.md:39-57) against the
+    oracle decoding the same frames on the DVB-T2 short 1/2 code built from
+    build_dvb_edges (dvb_t2/builder.rs:74-118)."""
     lib = _lib.load()
     assert lib.gf2_ldpc_decoder_create(2, 0, 50) is None
     assert lib.gf2_ldpc_decoder_create(0, 6, 50) is None
@@ -199,7 +202,7 @@ def test_sdr_api_matches_batch_decoder(ctx):
         assert (n.value, k.value) == (16200, 7200)
         s = codes.dvb_t2(0, 0)
         info, _, llr = dvb_random_frames(s.table, s.n, s.k, s.q, 12, 1.2, seed=11)
-        msg, it, conv, _ = decode_batch_arrays(LdpcCode.dvb_t2_short(CodeRate.RATE_1_2, ctx), llr, 50)
+        msg, it, conv, _ = oracle_code(s).decode_batch(llr, 50, threads=8)
         bits = frames.unpack_words(msg, s.k)
         assert 0 < conv.sum() < 12, "need converging and failing frames"
         out = np.zeros(s.k, dtype=np.uint8)

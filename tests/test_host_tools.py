@@ -61,3 +61,22 @@ def test_apply_rejects_unknown_knob():
     autotune.apply(Dummy(), {"wg_per_cu": 2, "chunk": 8, "path": _lib.PATH_STATE})
     with pytest.raises(ValueError):
         autotune.apply(Dummy(), {"frames_per_wg": 2})
+
+
+def test_simulation_result_csv_matches_reference_format():
+    """simulation.rs:276-312 (test_csv_export, test_simulation_result_complete)
+    and the column layout of results_to_csv (simulation.rs:209-229)."""
+    from gf2_amd.fer import FerPoint, SimulationResult, results_to_csv
+    r = SimulationResult(eb_n0_db=3.0, ber=0.01, num_bits=10000, num_errors=100)
+    csv = results_to_csv([r], True)
+    assert csv == "eb_n0_db,ber,num_bits,num_errors\n3,0.01,10000,100\n"
+    assert r.is_complete(50) and not r.is_complete(200)
+    p = FerPoint(ebn0_db=1.5, frames=4096, frame_errors=12, bit_errors=345, fer=12 / 4096,
+                 ber=345 / (4096 * 32400), mean_iterations=24.0, decode_seconds=1.0, info_gbps=1.0)
+    row = SimulationResult.from_point(p, 32400).to_csv_row().split(",")
+    assert row[0] == "1.5" and row[2] == str(4096 * 32400) and row[3] == "345"
+    assert float(row[1]) == p.ber and float(row[4]) == p.fer and row[5:] == ["4096", "12"]
+    assert "e" not in row[1]  # Rust's f64 Display never uses an exponent
+    both = results_to_csv([r, SimulationResult.from_point(p, 32400)], True)
+    assert both.splitlines()[0] == "eb_n0_db,ber,num_bits,num_errors,fer,num_frames,num_frame_errors"
+    assert results_to_csv([r], False) == "3,0.01,10000,100\n"
