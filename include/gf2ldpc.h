@@ -250,6 +250,30 @@ int gf2l_decode_batch_device(gf2l_ctx *ctx, const gf2l_code *code, const float *
                              uint32_t *d_iterations, uint8_t *d_converged,
                              uint8_t *d_syndrome_ok);
 
+/* gf2l_decode_batch_device without any allocation.  No reference
+ * counterpart: it is for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, semantics and results as
+ * gf2l_decode_batch_device; the working buffers (check states, beliefs, hard
+ * decisions, running lists and counters, permuted DVB LLRs) come from one
+ * static device array of GF2L_NOALLOC_SCRATCH_BYTES per GPU, which that GPU
+ * holds from the first call on it until the process ends.
+ *   - Quasi-cyclic codes (DVB-T2 included) with check degree <= 27 that run
+ *     on the LDS or STATE path; others return GF2L_ERR_UNSUPPORTED.
+ *   - A batch whose buffers exceed the array returns GF2L_ERR_INVALID and a
+ *     message with the largest batch of that code that fits (e.g. about 90
+ *     frames of an N=64800 rate-1/2 code; N=2304 codes run in LDS and need
+ *     none).
+ *   - Every call on one GPU shares that GPU's array: calls that overlap on
+ *     one GPU (several contexts or threads) race; serialize them.
+ * The host reads the running count back between iterations (a synchronous
+ * copy into pageable memory), so the call itself cannot be captured. */
+#define GF2L_NOALLOC_SCRATCH_BYTES (64u << 20)
+int gf2l_decode_batch_device_noalloc(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr,
+                                     size_t frames, uint32_t max_it, uint64_t *d_msg_words,
+                                     uint32_t *d_iterations, uint8_t *d_converged,
+                                     uint8_t *d_syndrome_ok);
+
 /* Per-iteration state dump for parity testing (host buffers).  Decodes like
  * gf2l_decode_batch and, after each iteration t < trace_iters, copies
  *   c2v_out[t][f][e*z + i]   check->variable message of block edge e
