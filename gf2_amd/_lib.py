@@ -28,6 +28,15 @@ TUNE_WG_PER_CU = 1
 TUNE_XCD_MAP = 2
 TUNE_GRAPH_ITERS = 3
 TUNE_CHUNK = 4
+TUNE_LAYERED_COLUMNS = 5
+
+KERNEL_CHECK = 0
+KERNEL_VARIABLE = 1
+KERNEL_SYNDROME = 2
+KERNEL_LAYERED = 3
+
+FALLBACK_RESTART = 0
+FALLBACK_CONTINUE = 1
 
 _c_void_pp = ctypes.POINTER(ctypes.c_void_p)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -60,6 +69,8 @@ SIGNATURES = {
     "gf2l_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "gf2l_profile_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p]),
+    "gf2l_profile_read_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_void_p]),
     "gf2l_code_qc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                     ctypes.c_uint32, _i32p, _c_void_pp]),
     "gf2l_code_edges": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
@@ -94,6 +105,9 @@ SIGNATURES = {
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "gf2l_nms_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                        ctypes.c_void_p, ctypes.c_float, _c_void_pp]),
+    "gf2l_nms_create_layered": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                               ctypes.c_void_p, ctypes.c_float, _c_void_pp]),
+    "gf2l_nms_set_fallback": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "gf2l_nms_destroy": (None, [ctypes.c_void_p]),
     "gf2l_nms_info": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p]),
     "gf2l_nms_get_weights": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

@@ -486,8 +486,6 @@ __device__ __forceinline__ void vn_state_item(const CodeView &cv, uint32_t bc, c
 // ---------------------------------------------------------------------------
 // Syndrome and message extraction of one frame by one workgroup (any size).
 
-// Bits pos .. pos+63 (cyclically mod z) of a z-bit vector stored in zw
-// words whose bits at z and above are zero.  Needs z >= 64.
 // Slot value the syndrome pass leaves in the next running list for a frame
 // that stopped.
 constexpr uint32_t kRetiredSlot = 0xffffffffu;
@@ -522,6 +520,8 @@ __device__ __forceinline__ uint32_t compact_list(uint32_t *__restrict__ list, ui
     return base;
 }
 
+// Bits pos .. pos+63 (cyclically mod z) of a z-bit vector stored in zw
+// words whose bits at z and above are zero.  Needs z >= 64.
 __device__ __forceinline__ uint64_t cyc64(const uint64_t *v, uint32_t z, uint32_t zw, uint32_t pos) {
     const uint32_t wi = pos >> 6, sh = pos & 63u;
     uint64_t r = v[wi] >> sh;
@@ -531,14 +531,12 @@ __device__ __forceinline__ uint64_t cyc64(const uint64_t *v, uint32_t z, uint32_
     return r;
 }
 
-// Copies the frame's nb*zw hard-decision words hbf into sh (LDS) and
-// returns nonzero (uniform over the workgroup) when a check is unsatisfied.
-__device__ __forceinline__ int frame_unsat(const CodeView &cv, uint64_t *sh, const uint64_t *__restrict__ hbf) {
-    const uint32_t words = cv.nb * cv.zw;
+// Returns nonzero (uniform over the workgroup) when a check is unsatisfied
+// by the frame's nb*zw hard-decision words in sh (LDS, complete and visible
+// to the whole workgroup).
+__device__ __forceinline__ int words_unsat(const CodeView &cv, const uint64_t *sh) {
     const uint32_t lane = threadIdx.x;
     const uint32_t z = cv.z, zw = cv.zw;
-    for (uint32_t w = lane; w < words; w += blockDim.x) sh[w] = hbf[w];
-    __syncthreads();
     uint64_t bad = 0;
     if (z >= 64) {
         for (uint32_t t = lane; t < cv.mb * zw; t += blockDim.x) {
@@ -566,6 +564,15 @@ __device__ __forceinline__ int frame_unsat(const CodeView &cv, uint64_t *sh, con
         }
     }
     return __syncthreads_or(bad != 0);
+}
+
+// Copies the frame's nb*zw hard-decision words hbf into sh (LDS) and
+// returns nonzero (uniform over the workgroup) when a check is unsatisfied.
+__device__ __forceinline__ int frame_unsat(const CodeView &cv, uint64_t *sh, const uint64_t *__restrict__ hbf) {
+    const uint32_t words = cv.nb * cv.zw;
+    for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) sh[w] = hbf[w];
+    __syncthreads();
+    return words_unsat(cv, sh);
 }
 
 // Writes the first k hard decisions (the message of a systematic code) from

@@ -101,11 +101,12 @@ int resolve_path(const gf2l_ctx *ctx, const gf2l::CodeView &cv) {
 
 // Runs the decoder on device buffers.  Without weights: plain min-sum for up
 // to max_it iterations.  With weights d_w ([max_it][nbe]): max_it weighted
-// iterations, then frames still failing the syndrome are decoded again from
-// their LLRs by up to fb_it plain iterations (d_fb marks them).  trace_*
-// (host) and trace_iters are only used by gf2l_decode_trace.
+// iterations, then frames still failing the syndrome get up to fb_it plain
+// iterations, from their LLRs or (fb_continue) from the weighted state
+// (d_fb marks them).  trace_* (host) and trace_iters are only used by
+// gf2l_decode_trace.
 int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size_t frames,
-                  uint32_t max_it, const float *d_w, uint32_t fb_it, uint64_t *d_msg,
+                  uint32_t max_it, const float *d_w, uint32_t fb_it, bool fb_continue, uint64_t *d_msg,
                   uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb,
                   float *h_trace_c2v, float *h_trace_bel, uint32_t trace_iters) {
     const gf2l::CodeView &cv = code->view;
@@ -140,7 +141,7 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
             GF2L_HIP(ctx, hipMalloc(&d_tc, (size_t)trace_iters * F * E * sizeof(float)));
             GF2L_HIP(ctx, hipMalloc(&d_tb, (size_t)trace_iters * F * cv.n * sizeof(float)));
         }
-        GF2L_HIP(ctx, gf2l::launch_lds_decode(cv, d_llr, F, max_it, d_w, fb_it, d_msg, d_iters,
+        GF2L_HIP(ctx, gf2l::launch_lds_decode(cv, d_llr, F, max_it, d_w, fb_it, fb_continue, d_msg, d_iters,
                                               d_conv, d_synd, d_fb, d_tc, d_tb, trace_iters, s));
         if (d_tc) {
             GF2L_HIP(ctx, hipMemcpyAsync(h_trace_c2v, d_tc, (size_t)trace_iters * F * E * sizeof(float),
@@ -166,22 +167,22 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
         float *a, *b;
         ~Free() { hipFree(a); hipFree(b); }
     } free_trace{d_tc, d_tb};
-    return run_hbm(ctx, cv, path, ws, d_llr, F, max_it, d_w, fb_it, d_msg, d_iters, d_conv, d_synd, d_fb,
-                   h_trace_c2v, h_trace_bel, trace_iters, d_tc, d_tb);
+    return run_hbm(ctx, cv, path, ws, d_llr, F, max_it, d_w, fb_it, fb_continue, d_msg, d_iters, d_conv,
+                   d_synd, d_fb, h_trace_c2v, h_trace_bel, trace_iters, d_tc, d_tb);
 }
 
 // The iteration loop of the HBM paths over a prepared workspace: LLR
 // permutation of DVB codes, then phase 0 (max_it iterations, weighted by d_w
-// when given) and, with fb_it > 0, phase 1 (fb_it plain iterations restarted
-// from the LLRs for frames still failing).  Iterations are enqueued in
+// when given) and, with fb_it > 0, phase 1 (fb_it plain iterations for
+// frames still failing, restarted from the LLRs or continued).  Iterations are enqueued in
 // batches of GF2L_TUNE_GRAPH_ITERS; the host reads the running count once per
 // batch (every iteration when tracing) and stops once no frame runs.
 // Iterations enqueued after the last frame retired find an empty list and
 // do nothing.  Allocates nothing: ws.h_count may point at pageable memory.
 int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const float *d_llr, uint32_t F,
-            uint32_t max_it, const float *d_w, uint32_t fb_it, uint64_t *d_msg, uint32_t *d_iters,
-            uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb, float *h_trace_c2v, float *h_trace_bel,
-            uint32_t trace_iters, float *d_tc, float *d_tb) {
+            uint32_t max_it, const float *d_w, uint32_t fb_it, bool fb_continue, uint64_t *d_msg,
+            uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb, float *h_trace_c2v,
+            float *h_trace_bel, uint32_t trace_iters, float *d_tc, float *d_tb) {
     hipStream_t s = ctx->stream;
     const size_t E = (size_t)cv.nbe * cv.z;
     gf2l::Profile &pf = ctx->prof;
@@ -209,22 +210,26 @@ int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const fl
     for (uint32_t phase = 0; phase < n_phase; ++phase) {
         const uint32_t n_it = phase == 0 ? max_it : fb_it;
         const float *w = phase == 0 ? d_w : nullptr;
-        if (phase == 1 && path == gf2l::kStream) {  // restart the survivors from their LLRs
+        if (phase == 1 && path == gf2l::kStream && !fb_continue) {  // restart the survivors from their LLRs
             GF2L_HIP(ctx, hipMemcpyAsync(ws.belief, d_llr, (size_t)F * cv.n * sizeof(float),
                                          hipMemcpyDeviceToDevice, s));
             GF2L_HIP(ctx, hipMemsetAsync(ws.c2v, 0, (size_t)F * E * sizeof(float), s));
         }
+        // a continued fallback never runs the first-iteration kernels, and its
+        // first check update subtracts the last weighted messages
+        const bool cont = phase == 1 && fb_continue;
         for (uint32_t it = 1; it <= n_it; ++it, ++g) {
             gf2l::IterArgs a{};
             a.llr = d_llr;
             a.frames = F;
-            a.it = it;
+            a.it = cont ? max_it + it : it;
             a.cur = g & 1u;
             a.it_report = (phase == 0 ? 0 : max_it) + it;
             a.last = it == n_it && phase + 1 == n_phase;
             a.fb_phase = (int)phase;
             a.w_prev = (w && it >= 2) ? w + (size_t)(it - 2) * cv.nbe : nullptr;
             a.w_cur = w ? w + (size_t)(it - 1) * cv.nbe : nullptr;
+            if (cont && it == 1) a.w_prev = d_w + (size_t)(max_it - 1) * cv.nbe;
             a.msg = d_msg;
             a.iters = d_iters;
             a.conv = d_conv;
@@ -319,7 +324,7 @@ int decode_host(gf2l_ctx *ctx, const gf2l_code *code, const float *llr, size_t f
     uint8_t *d_synd = d_conv + b_c;
     hipStream_t s = ctx->stream;
     GF2L_HIP(ctx, hipMemcpyAsync(d_llr, llr, frames * cv.n * sizeof(float), hipMemcpyHostToDevice, s));
-    rc = decode_device(ctx, code, d_llr, frames, max_it, nullptr, 0, d_msg, d_it, d_conv, d_synd,
+    rc = decode_device(ctx, code, d_llr, frames, max_it, nullptr, 0, false, d_msg, d_it, d_conv, d_synd,
                        nullptr, trace_c2v, trace_bel, trace_iters);
     if (rc != GF2L_OK) return rc;
     GF2L_HIP(ctx, hipMemcpyAsync(msg, d_msg, frames * kw * 8, hipMemcpyDeviceToHost, s));
@@ -453,6 +458,8 @@ void gf2l_destroy(gf2l_ctx *ctx) {
     free_workspace(ctx->ws);
     for (auto &e : ctx->prof.ev)
         if (e) hipEventDestroy(e);
+    for (auto &e : ctx->prof.lev)
+        if (e) hipEventDestroy(e);
     hipFree(ctx->d_io);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -476,13 +483,16 @@ int gf2l_profile_enable(gf2l_ctx *ctx, int on) {
     return gf2l::guard(ctx, [&]() -> int {
         GF2L_HIP(ctx, hipSetDevice(ctx->device));
         gf2l::Profile &pf = ctx->prof;
-        if (on && !pf.ev[0])
+        if (on && !pf.ev[0]) {
             for (auto &e : pf.ev) GF2L_HIP(ctx, hipEventCreate(&e));
-        for (int k = 0; k < 3; ++k) {
+            for (auto &e : pf.lev) GF2L_HIP(ctx, hipEventCreate(&e));
+        }
+        for (int k = 0; k < 4; ++k) {
             pf.ms[k] = 0;
             pf.launches[k] = 0;
         }
         pf.frame_iters = 0;
+        pf.layered_frame_iters = 0;
         pf.on = on != 0;
         return GF2L_OK;
     });
@@ -495,6 +505,16 @@ int gf2l_profile_read(gf2l_ctx *ctx, double *ms, uint64_t *launches, uint64_t *f
         if (launches) launches[k] = ctx->prof.launches[k];
     }
     if (frame_iterations) *frame_iterations = ctx->prof.frame_iters;
+    return GF2L_OK;
+}
+
+int gf2l_profile_read_kernel(gf2l_ctx *ctx, int kernel, double *ms, uint64_t *launches,
+                             uint64_t *frame_iterations) {
+    if (!ctx || kernel < 0 || kernel > GF2L_KERNEL_LAYERED) return GF2L_ERR_INVALID;
+    const gf2l::Profile &pf = ctx->prof;
+    if (ms) *ms = pf.ms[kernel];
+    if (launches) *launches = pf.launches[kernel];
+    if (frame_iterations) *frame_iterations = kernel == GF2L_KERNEL_LAYERED ? pf.layered_frame_iters : pf.frame_iters;
     return GF2L_OK;
 }
 
@@ -615,6 +635,10 @@ int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value) {
                 if (value < -1 || value > 64) return fail(ctx, GF2L_ERR_INVALID, "chunk out of [-1,64]");
                 ctx->tune.chunk = value;
                 return GF2L_OK;
+            case GF2L_TUNE_LAYERED_COLUMNS:
+                if (value < 0) return fail(ctx, GF2L_ERR_INVALID, "layered columns must be >= 0");
+                ctx->tune.layered_cols = value;
+                return GF2L_OK;
             default:
                 return fail(ctx, GF2L_ERR_INVALID, "unknown tuning key");
         }
@@ -699,7 +723,7 @@ int gf2l_decode_batch_device(gf2l_ctx *ctx, const gf2l_code *code, const float *
         if (!d_llr || !d_msg_words || !d_iterations || !d_converged || !d_syndrome_ok)
             return fail(ctx, GF2L_ERR_INVALID, "null buffer");
         GF2L_HIP(ctx, hipSetDevice(ctx->device));
-        return decode_device(ctx, code, d_llr, frames, max_it, nullptr, 0, d_msg_words, d_iterations,
+        return decode_device(ctx, code, d_llr, frames, max_it, nullptr, 0, false, d_msg_words, d_iterations,
                              d_converged, d_syndrome_ok, nullptr, nullptr, nullptr, 0);
     });
 }

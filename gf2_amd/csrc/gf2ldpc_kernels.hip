@@ -307,11 +307,13 @@ __global__ void state_expand_kernel(CodeView cv, const uint32_t *__restrict__ st
 // beliefs and LLRs live in LDS.  Used when (E + 2n) floats fit.
 // Phase 0 runs max_it iterations (weighted by w[it-1][e] when w is given);
 // a frame that still fails the syndrome is, when fb_it > 0, decoded again
-// from its channel LLRs by fb_it plain iterations (phase 1).
+// from its channel LLRs by fb_it plain iterations (phase 1), or with
+// fb_continue by fb_it plain iterations continued from the phase-0 beliefs
+// and (weighted) messages.
 template <int DMAX, int DVMAX>
 __global__ void __launch_bounds__(512)
 lds_decode_kernel(CodeView cv, const float *__restrict__ llr, uint32_t frames, uint32_t max_it,
-                  const float *__restrict__ w, uint32_t fb_it, uint64_t *__restrict__ msg,
+                  const float *__restrict__ w, uint32_t fb_it, int fb_continue, uint64_t *__restrict__ msg,
                   uint32_t *__restrict__ iters, uint8_t *__restrict__ conv,
                   uint8_t *__restrict__ synd, uint8_t *__restrict__ fb,
                   float *__restrict__ trace_c2v, float *__restrict__ trace_bel,
@@ -332,8 +334,10 @@ lds_decode_kernel(CodeView cv, const float *__restrict__ llr, uint32_t frames, u
                 if (converged || fb_it == 0) break;
                 fell_back = 1;
             }
-            for (uint32_t v = threadIdx.x; v < n; v += blockDim.x) s_bel[v] = s_llr[v];
-            for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) s_c2v[e] = 0.0f;
+            if (phase == 0 || !fb_continue) {
+                for (uint32_t v = threadIdx.x; v < n; v += blockDim.x) s_bel[v] = s_llr[v];
+                for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) s_c2v[e] = 0.0f;
+            }
             __syncthreads();
             for (uint32_t it = 1; it <= n_it; ++it) {
                 const float *w_it = (phase == 0 && w) ? w + (size_t)(it - 1) * cv.nbe : nullptr;
@@ -493,14 +497,14 @@ uint32_t hbm_grid(uint32_t items, int num_cus, int wg_per_cu) {
 
 template <int DMAX, int DVMAX>
 hipError_t launch_lds_t(const CodeView &cv, const float *llr, uint32_t frames, uint32_t max_it,
-                        const float *w, uint32_t fb_it, uint64_t *msg, uint32_t *iters,
+                        const float *w, uint32_t fb_it, bool fb_continue, uint64_t *msg, uint32_t *iters,
                         uint8_t *conv, uint8_t *synd, uint8_t *fb, float *tc, float *tb, uint32_t ti,
                         hipStream_t s) {
     const size_t bytes = lds_path_bytes(cv);
     auto k = lds_decode_kernel<DMAX, DVMAX>;
     hipError_t err = set_lds(k, bytes);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k, dim3(frames), dim3(512), bytes, s, cv, llr, frames, max_it, w, fb_it,
+    hipLaunchKernelGGL(k, dim3(frames), dim3(512), bytes, s, cv, llr, frames, max_it, w, fb_it, (int)fb_continue,
                        msg, iters, conv, synd, fb, tc, tb, ti);
     return hipGetLastError();
 }
@@ -537,14 +541,14 @@ size_t lds_path_bytes(const CodeView &cv) {
 }
 
 hipError_t launch_lds_decode(const CodeView &cv, const float *llr, uint32_t frames, uint32_t max_it,
-                             const float *w, uint32_t fb_it, uint64_t *msg, uint32_t *iters,
+                             const float *w, uint32_t fb_it, bool fb_continue, uint64_t *msg, uint32_t *iters,
                              uint8_t *conv, uint8_t *synd, uint8_t *fb, float *trace_c2v,
                              float *trace_bel, uint32_t trace_iters, hipStream_t s) {
     if (frames == 0) return hipSuccess;
     const int dm = pick_dmax(cv.max_dc), dvm = pick_dvmax(cv.max_dv);
-#define GF2L_LDS(DM, DV)                                                                        \
-    if (dm == DM && dvm == DV)                                                                  \
-        return launch_lds_t<DM, DV>(cv, llr, frames, max_it, w, fb_it, msg, iters, conv, synd, \
+#define GF2L_LDS(DM, DV)                                                                                     \
+    if (dm == DM && dvm == DV)                                                                               \
+        return launch_lds_t<DM, DV>(cv, llr, frames, max_it, w, fb_it, fb_continue, msg, iters, conv, synd, \
                                     fb, trace_c2v, trace_bel, trace_iters, s);
     GF2L_LDS(8, 4) GF2L_LDS(8, 8) GF2L_LDS(8, 16) GF2L_LDS(8, 32)
     GF2L_LDS(16, 4) GF2L_LDS(16, 8) GF2L_LDS(16, 16) GF2L_LDS(16, 32)

@@ -85,9 +85,11 @@ constexpr int kProfileSlots = 64;
 struct Profile {
     bool on = false;
     hipEvent_t ev[4 * kProfileSlots] = {};
-    double ms[3] = {0, 0, 0};         // check, variable, syndrome kernels
-    uint64_t launches[3] = {0, 0, 0};
-    uint64_t frame_iters = 0;         // sum over launches of frames processed
+    hipEvent_t lev[2] = {};           // around the layered kernel
+    double ms[4] = {0, 0, 0, 0};      // check, variable, syndrome, layered kernels
+    uint64_t launches[4] = {0, 0, 0, 0};
+    uint64_t frame_iters = 0;         // sum over HBM-path launches of frames processed
+    uint64_t layered_frame_iters = 0; // layered iterations run, summed over frames
 };
 
 // Launch-shape tunables of the HBM paths (set by gf2l_set_tuning or the
@@ -97,6 +99,7 @@ struct Tuning {
     int xcd_map = 1;     // 1: frames pinned to XCDs (L2 reuse), 0: frame-major over the grid
     int graph_iters = 4; // iterations per graph replay of the mixed-code decoder
     int chunk = -1;      // items per counter fetch of the check-state kernels (0: fixed stride, -1: automatic)
+    int layered_cols = 0; // block-columns of beliefs the layered decoder keeps in LDS (0: as many as fit)
 };
 
 // Word offset of the item counters in Workspace::counts and its size (two
@@ -117,6 +120,41 @@ struct IterArgs {
     uint64_t *msg;
     uint32_t *iters;
     uint8_t *conv, *synd, *fb;
+};
+
+// Arguments of the layered learned decoder's forward kernel (layered.hip).
+struct LayeredArgs {
+    CodeView cv;
+    const float *llr;        // [F][n] channel LLRs
+    const float *w;          // [T][nbe] weights, block edges in row order
+    uint32_t F, T;           // frames, layered iterations
+    uint32_t X;              // block-columns whose beliefs live in LDS (0..X-1)
+    int fallback;            // frames failing after T: 1 -> continuation list, 0 -> retire
+    uint32_t *state;         // [F][mb][zp][3] compressed check states
+    float *belief;           // [F][nb][zp] beliefs of block-columns >= X (all at hand-over)
+    uint32_t *ctr;           // [0] next frame, [1] layered frame-iterations run (zeroed by the host)
+    uint32_t *list, *count;  // running list of the continuation and its length
+    uint64_t *msg;
+    uint32_t *iters;
+    uint8_t *conv, *synd, *fb;
+    uint32_t *tape_state;    // training: [T][F][mb][zp] uint4 (min1, min2, meta, idx2 | argmin input signs)
+    float *tape_bel;         // training: [T][F][nb][zp] beliefs after each iteration
+};
+
+// Arguments of the layered backward kernel (layered.hip).
+struct LayeredBwdArgs {
+    CodeView cv;
+    const float *w;           // [T][nbe]
+    uint32_t F, T, X;
+    const uint32_t *tape_state;
+    const float *tape_bel;
+    const uint64_t *cw;       // [F][ceil(n/64)] codewords
+    float *g_glob;            // [F][nb][zp] dL/db of block-columns >= X
+    float *H;                 // [F][nbe][z] dL/d(message) carried to the previous iteration
+    float *gpart;             // [T][nbe][F] per-frame weight gradients
+    float scale;              // 1 / (T n F)
+    float *loss;              // summed loss (float atomics; logging only)
+    uint32_t *ctr;            // [0] next frame (zeroed by the host)
 };
 
 }  // namespace gf2l
@@ -203,10 +241,12 @@ int guard(gf2l_ctx *ctx, F &&f) noexcept {
 
 // Decoder driver (gf2ldpc_api.hip): plain min-sum for max_it iterations, or
 // with weights d_w ([max_it][nbe]) max_it weighted iterations followed by up
-// to fb_it plain iterations from scratch for frames still failing the
-// syndrome.  Device buffers; may synchronize the context's stream.
+// to fb_it plain iterations for frames still failing the syndrome, restarted
+// from their LLRs or, with fb_continue, continued from the weighted state
+// (the first plain check update subtracts the last weighted messages).
+// Device buffers; may synchronize the context's stream.
 int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size_t frames,
-                  uint32_t max_it, const float *d_w, uint32_t fb_it, uint64_t *d_msg,
+                  uint32_t max_it, const float *d_w, uint32_t fb_it, bool fb_continue, uint64_t *d_msg,
                   uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb,
                   float *h_trace_c2v, float *h_trace_bel, uint32_t trace_iters);
 
@@ -215,9 +255,9 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
 // decode_device after growing the context's workspace and by the
 // no-allocation entry point over its static scratch array.
 int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const float *d_llr, uint32_t F,
-            uint32_t max_it, const float *d_w, uint32_t fb_it, uint64_t *d_msg, uint32_t *d_iters,
-            uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb, float *h_trace_c2v, float *h_trace_bel,
-            uint32_t trace_iters, float *d_tc, float *d_tb);
+            uint32_t max_it, const float *d_w, uint32_t fb_it, bool fb_continue, uint64_t *d_msg,
+            uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb, float *h_trace_c2v,
+            float *h_trace_bel, uint32_t trace_iters, float *d_tc, float *d_tb);
 // Decoder path (GF2L_PATH_*) the context would run a quasi-cyclic code on,
 // or -1 when a forced path cannot run it.
 int resolve_path(const gf2l_ctx *ctx, const CodeView &cv);
@@ -236,8 +276,18 @@ hipError_t launch_dvb_permute(const CodeView &cv, const float *llr, float *out, 
 int ensure_buffers(gf2l_ctx *ctx, size_t frames, size_t c2v, size_t state, size_t bel, size_t hb, size_t llr);
 
 // Weights [iterations][nbe] (device), weighted iterations and code of a
-// learned decoder (nms.hip).
-const float *nms_weights(const gf2l_nms *q, uint32_t *iterations, const gf2l_code **code);
+// learned decoder, and whether it is a flooding decoder whose fallback
+// restarts from the LLRs (nms.hip).
+const float *nms_weights(const gf2l_nms *q, uint32_t *iterations, const gf2l_code **code, bool *flooding_restart);
+
+// Layered learned decoder (layered.hip): LDS bytes of a workgroup holding
+// X block-columns of beliefs, the largest X that fits lds_limit, and the
+// forward (tape: the training forward, no early stop, no outputs) and
+// backward launchers.
+size_t layered_lds_bytes(const CodeView &cv, uint32_t X);
+uint32_t layered_lds_columns(const CodeView &cv, size_t lds_limit);
+hipError_t launch_layered_fwd(const LayeredArgs &a, bool tape, int num_cus, hipStream_t s);
+hipError_t launch_layered_bwd(const LayeredBwdArgs &a, int num_cus, hipStream_t s);
 
 // Plain min-sum on a general sparse code (csr.hip).
 int decode_csr(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, uint32_t F, uint32_t max_it,
@@ -246,9 +296,10 @@ int decode_csr(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, uint32_
 // Kernel launchers (gf2ldpc_kernels.hip).  All enqueue on `s`.
 size_t lds_path_bytes(const CodeView &cv);
 // Phase 0: max_it iterations, weighted by w[it-1][e] if w != NULL; then, for
-// frames still failing the syndrome, fb_it plain iterations from scratch.
+// frames still failing the syndrome, fb_it plain iterations from scratch or,
+// with fb_continue, from the phase-0 state.
 hipError_t launch_lds_decode(const CodeView &cv, const float *llr, uint32_t frames, uint32_t max_it,
-                             const float *w, uint32_t fb_it, uint64_t *msg, uint32_t *iters,
+                             const float *w, uint32_t fb_it, bool fb_continue, uint64_t *msg, uint32_t *iters,
                              uint8_t *conv, uint8_t *synd, uint8_t *fb, float *trace_c2v,
                              float *trace_bel, uint32_t trace_iters, hipStream_t s);
 hipError_t launch_hbm_init(const CodeView &cv, Workspace &ws, int path, const float *llr,

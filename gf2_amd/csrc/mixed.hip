@@ -619,9 +619,13 @@ int decode_mixed(gf2l_ctx *ctx, gf2l_codeset *cs, const gf2l_nms *const *decoder
             if (!decoders[c]) return fail(ctx, GF2L_ERR_INVALID, "null learned decoder for code " + std::to_string(c));
             uint32_t Tc = 0;
             const gf2l_code *code = nullptr;
-            w[c] = nms_weights(decoders[c], &Tc, &code);
+            bool restart = false;
+            w[c] = nms_weights(decoders[c], &Tc, &code, &restart);
             if (code != cs->codes[c])
                 return fail(ctx, GF2L_ERR_INVALID, "learned decoder " + std::to_string(c) + " is for another code");
+            if (!restart)
+                return fail(ctx, GF2L_ERR_UNSUPPORTED,
+                            "mixed batches take flooding learned decoders with the restarting fallback");
             if (c > 0 && Tc != T)
                 return fail(ctx, GF2L_ERR_INVALID, "learned decoders differ in their iteration count");
             T = Tc;

@@ -29,21 +29,26 @@ struct gf2l_nms {
     const gf2l_code *code = nullptr;
     uint32_t T = 0;           // weighted iterations
     uint32_t nbe = 0;         // weights per iteration (base-graph edges)
+    bool layered = false;     // layered schedule (gf2l_nms_create_layered), else flooding
+    bool fb_continue = false; // the plain fallback continues from the learned state (layered: always)
+    uint32_t X = 0;           // layered: block-columns whose beliefs a workgroup keeps in LDS
     float *d_w = nullptr;     // [T][nbe]
     float *d_m = nullptr;     // Adam first moment
     float *d_v = nullptr;     // Adam second moment
     uint64_t step = 0;
     // training workspace, grown on demand
     size_t cap = 0;
-    uint32_t *tape_state = nullptr;  // [T][F][mb][zp][3]
+    uint32_t *tape_state = nullptr;  // [T][F][mb][zp][3] (layered: [4])
     float *tape_bel = nullptr;       // [T][F][nb][zp]
-    float *g_a = nullptr, *g_b = nullptr;  // [F][nbe][z] dL/dv2c
+    float *g_a = nullptr, *g_b = nullptr;  // [F][nbe][z] dL/dv2c (layered: g_a carries dL/dmessage)
     float *g_bel = nullptr;          // [F][nb][zp]
     float *gpart = nullptr;          // [T][nbe][F]
     float *grad = nullptr;           // [T][nbe]
     float *loss = nullptr;           // 1
     uint32_t *list = nullptr;        // [F] identity list
-    uint32_t *count = nullptr;       // 1
+    uint32_t *count = nullptr;       // 4 words: running count; layered frame counters
+    uint32_t *l_state = nullptr;     // layered training forward: [F][mb][zp][3] check states
+    float *l_bel = nullptr;          // layered training forward: [F][nb][zp] beliefs outside LDS
 };
 
 namespace gf2l {
@@ -273,10 +278,14 @@ void free_train(gf2l_nms *q) {
     hipFree(q->loss);
     hipFree(q->list);
     hipFree(q->count);
+    hipFree(q->l_state);
+    hipFree(q->l_bel);
     q->tape_state = nullptr;
     q->tape_bel = nullptr;
     q->g_a = q->g_b = q->g_bel = q->gpart = q->grad = q->loss = nullptr;
     q->list = q->count = nullptr;
+    q->l_state = nullptr;
+    q->l_bel = nullptr;
     q->cap = 0;
 }
 
@@ -285,23 +294,189 @@ int ensure_train(gf2l_ctx *ctx, gf2l_nms *q, size_t F) {
     free_train(q);
     const CodeView &cv = q->code->view;
     const size_t T = q->T, E = (size_t)cv.nbe * cv.z;
-    GF2L_HIP(ctx, hipMalloc(&q->tape_state, T * F * 3 * cv.mb * cv.zp * sizeof(uint32_t)));
+    const size_t rec = q->layered ? 4 : 3;  // words per tape record of a check
+    GF2L_HIP(ctx, hipMalloc(&q->tape_state, T * F * rec * cv.mb * cv.zp * sizeof(uint32_t)));
     GF2L_HIP(ctx, hipMalloc(&q->tape_bel, T * F * cv.nb * cv.zp * sizeof(float)));
     GF2L_HIP(ctx, hipMalloc(&q->g_a, F * E * sizeof(float)));
-    GF2L_HIP(ctx, hipMalloc(&q->g_b, F * E * sizeof(float)));
+    if (!q->layered) GF2L_HIP(ctx, hipMalloc(&q->g_b, F * E * sizeof(float)));
     GF2L_HIP(ctx, hipMalloc(&q->g_bel, F * cv.nb * cv.zp * sizeof(float)));
     GF2L_HIP(ctx, hipMalloc(&q->gpart, T * cv.nbe * F * sizeof(float)));
     GF2L_HIP(ctx, hipMalloc(&q->grad, T * cv.nbe * sizeof(float)));
     GF2L_HIP(ctx, hipMalloc(&q->loss, sizeof(float)));
     GF2L_HIP(ctx, hipMalloc(&q->list, F * sizeof(uint32_t)));
-    GF2L_HIP(ctx, hipMalloc(&q->count, sizeof(uint32_t)));
+    GF2L_HIP(ctx, hipMalloc(&q->count, 4 * sizeof(uint32_t)));
+    if (q->layered) {
+        GF2L_HIP(ctx, hipMalloc(&q->l_state, F * 3 * cv.mb * cv.zp * sizeof(uint32_t)));
+        GF2L_HIP(ctx, hipMalloc(&q->l_bel, F * cv.nb * cv.zp * sizeof(float)));
+    }
     q->cap = F;
+    return GF2L_OK;
+}
+
+// Block-columns of beliefs the layered kernels keep in LDS.  Default: as
+// many as fit in a third of the LDS, so three frames share a compute unit
+// and hide each other's memory latency (measured on the N=64800 code: 30
+// columns and three workgroups per CU decode 1.5x faster than 107 columns
+// and one); GF2L_TUNE_LAYERED_COLUMNS sets it (up to what fits).
+uint32_t layered_x(const gf2l_ctx *ctx, const gf2l_nms *q) {
+    const int t = ctx->tune.layered_cols;
+    if (t > 0) return std::min<uint32_t>((uint32_t)t, q->X);
+    return std::min<uint32_t>(q->X, layered_lds_columns(q->code->view, ctx->lds_limit / 3));
+}
+
+// Layered schedule: training forward (tape, all T iterations) and backward;
+// leaves the summed gradient in q->grad and the summed loss in q->loss.
+int layered_forward_backward(gf2l_ctx *ctx, gf2l_nms *q, const float *d_llr, const uint64_t *d_cw, size_t frames) {
+    const CodeView &cv = q->code->view;
+    if (frames == 0 || frames > 0xffffffffull) return fail(ctx, GF2L_ERR_INVALID, "bad frame count");
+    const uint32_t F = (uint32_t)frames, T = q->T;
+    int rc = ensure_train(ctx, q, F);
+    if (rc != GF2L_OK) return rc;
+    hipStream_t s = ctx->stream;
+    GF2L_HIP(ctx, hipMemsetAsync(q->count, 0, 4 * sizeof(uint32_t), s));
+    GF2L_HIP(ctx, hipMemsetAsync(q->loss, 0, sizeof(float), s));
+    LayeredArgs a{};
+    a.cv = cv;
+    a.llr = d_llr;
+    a.w = q->d_w;
+    a.F = F;
+    a.T = T;
+    a.X = layered_x(ctx, q);
+    a.fallback = 0;
+    a.state = q->l_state;
+    a.belief = q->l_bel;
+    a.ctr = q->count;
+    a.list = q->list;
+    a.count = q->count + 3;
+    a.tape_state = q->tape_state;
+    a.tape_bel = q->tape_bel;
+    GF2L_HIP(ctx, launch_layered_fwd(a, true, ctx->num_cus, s));
+    LayeredBwdArgs b{};
+    b.cv = cv;
+    b.w = q->d_w;
+    b.F = F;
+    b.T = T;
+    b.X = a.X;
+    b.tape_state = q->tape_state;
+    b.tape_bel = q->tape_bel;
+    b.cw = d_cw;
+    b.g_glob = q->g_bel;
+    b.H = q->g_a;
+    b.gpart = q->gpart;
+    b.scale = 1.0f / ((float)T * (float)cv.n * (float)F);
+    b.loss = q->loss;
+    b.ctr = q->count + 2;
+    GF2L_HIP(ctx, launch_layered_bwd(b, ctx->num_cus, s));
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3(T * cv.nbe), dim3(256), 0, s, q->gpart, F, q->grad);
+    GF2L_HIP(ctx, hipGetLastError());
+    return GF2L_OK;
+}
+
+__global__ void clear_words_kernel(uint32_t *p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// Layered decode: the layered kernel over all frames (early stop after any
+// iteration), then plain flooding min-sum continued from the layered state
+// for the frames it handed over, up to fb_it iterations (reported as T + u;
+// d_fb marks them).  Device buffers; synchronizes the context's stream.
+int layered_decode(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr, uint32_t F, uint32_t fb_it,
+                   uint64_t *d_msg, uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb) {
+    const CodeView &cv = q->code->view;
+    int rc = ensure_buffers(ctx, F, 0, 3ull * cv.mb * cv.zp * F, (size_t)cv.nb * cv.zp * F,
+                            (size_t)cv.nb * cv.zw * F, 0);
+    if (rc != GF2L_OK) return rc;
+    Workspace &ws = ctx->ws;
+    Profile &pf = ctx->prof;
+    hipStream_t s = ctx->stream;
+    // running counts, item counters and the layered frame counters start at zero
+    hipLaunchKernelGGL(clear_words_kernel, dim3((kCountsWords + 255) / 256), dim3(256), 0, s, ws.counts, kCountsWords);
+    GF2L_HIP(ctx, hipGetLastError());
+    LayeredArgs a{};
+    a.cv = cv;
+    a.llr = d_llr;
+    a.w = q->d_w;
+    a.F = F;
+    a.T = q->T;
+    a.X = layered_x(ctx, q);
+    a.fallback = fb_it > 0;
+    a.state = ws.state;
+    a.belief = ws.belief;
+    a.ctr = ws.counts + 2;
+    a.list = ws.lists;  // list 0: read by the first continuation iteration
+    a.count = ws.counts;
+    a.msg = d_msg;
+    a.iters = d_iters;
+    a.conv = d_conv;
+    a.synd = d_synd;
+    a.fb = d_fb;
+    if (pf.on) GF2L_HIP(ctx, hipEventRecord(pf.lev[0], s));
+    GF2L_HIP(ctx, launch_layered_fwd(a, false, ctx->num_cus, s));
+    if (pf.on) GF2L_HIP(ctx, hipEventRecord(pf.lev[1], s));
+    // frames handed over, and the layered frame-iterations run
+    GF2L_HIP(ctx, hipMemcpyAsync(ws.h_count, ws.counts, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    GF2L_HIP(ctx, hipStreamSynchronize(s));
+    uint32_t running = ws.h_count[0];
+    if (pf.on) {
+        float ms = 0.0f;
+        GF2L_HIP(ctx, hipEventElapsedTime(&ms, pf.lev[0], pf.lev[1]));
+        pf.ms[3] += ms;
+        pf.launches[3] += 1;
+        pf.layered_frame_iters += ws.h_count[3];
+    }
+    if (!running) return GF2L_OK;
+    // plain flooding continuation over list 0 (never the first-iteration
+    // kernels: beliefs and check states come from the layered decoder, whose
+    // last messages were weighted by w_T)
+    const uint32_t batch = (uint32_t)std::max(1, std::min(ctx->tune.graph_iters, kProfileSlots));
+    uint32_t queued = 0;
+    for (uint32_t u = 1; u <= fb_it; ++u) {
+        IterArgs it{};
+        it.llr = d_llr;
+        it.frames = F;
+        it.it = 2;
+        it.cur = (u - 1) & 1u;
+        it.it_report = q->T + u;
+        it.last = u == fb_it;
+        it.fb_phase = 1;
+        it.w_prev = u == 1 ? q->d_w + (size_t)(q->T - 1) * cv.nbe : nullptr;
+        it.w_cur = nullptr;
+        it.msg = d_msg;
+        it.iters = d_iters;
+        it.conv = d_conv;
+        it.synd = d_synd;
+        it.fb = d_fb;
+        GF2L_HIP(ctx, launch_hbm_iteration(cv, ws, kState, it, ctx->num_cus, ctx->tune, s,
+                                           pf.on ? pf.ev + 4 * queued : nullptr));
+        const bool readback = queued + 1 >= batch || u == fb_it;
+        if (pf.on || readback)
+            GF2L_HIP(ctx, hipMemcpyAsync(ws.h_count + (pf.on ? queued : 0), ws.counts + (u & 1u), sizeof(uint32_t),
+                                         hipMemcpyDeviceToHost, s));
+        ++queued;
+        if (!readback) continue;
+        GF2L_HIP(ctx, hipStreamSynchronize(s));
+        if (pf.on) {
+            for (uint32_t b = 0; b < queued; ++b)
+                for (int k = 0; k < 3; ++k) {
+                    float ms = 0.0f;
+                    GF2L_HIP(ctx, hipEventElapsedTime(&ms, pf.ev[4 * b + k], pf.ev[4 * b + k + 1]));
+                    pf.ms[k] += ms;
+                    pf.launches[k] += 1;
+                }
+            pf.frame_iters += running;
+            for (uint32_t b = 0; b + 1 < queued; ++b) pf.frame_iters += ws.h_count[b];
+        }
+        running = ws.h_count[pf.on ? queued - 1 : 0];
+        queued = 0;
+        if (!running) break;
+    }
     return GF2L_OK;
 }
 
 // Forward pass with tape + backward pass; leaves the summed gradient in
 // q->grad and the summed loss in q->loss.
 int forward_backward(gf2l_ctx *ctx, gf2l_nms *q, const float *d_llr, const uint64_t *d_cw, size_t frames) {
+    if (q->layered) return layered_forward_backward(ctx, q, d_llr, d_cw, frames);
     const CodeView &cv = q->code->view;
     if (cv.max_dc > kMaxStateDegree) return fail(ctx, GF2L_ERR_UNSUPPORTED, "check degree above 27");
     if (frames == 0 || frames > 0xffffffffull) return fail(ctx, GF2L_ERR_INVALID, "bad frame count");
@@ -359,6 +534,48 @@ int forward_backward(gf2l_ctx *ctx, gf2l_nms *q, const float *d_llr, const uint6
 }  // namespace
 }  // namespace gf2l
 
+namespace gf2l {
+namespace {
+
+// Shared constructor of the flooding and layered learned decoders.
+int create_nms(gf2l_ctx *ctx, const gf2l_code *code, uint32_t iterations, const float *init_weights,
+               float init_value, bool layered, gf2l_nms **out) {
+    if (!code || iterations == 0) return fail(ctx, GF2L_ERR_INVALID, "need a code and iterations >= 1");
+    if (code->is_csr || code->view.dvb_q)
+        return fail(ctx, GF2L_ERR_UNSUPPORTED, "learned weights need a quasi-cyclic code (not DVB)");
+    if (layered && code->view.max_dc > kMaxStateDegree)
+        return fail(ctx, GF2L_ERR_UNSUPPORTED, "the layered decoder needs check degree <= 27");
+    GF2L_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t nw = (size_t)iterations * code->view.nbe;
+    std::vector<float> w(nw, init_value);
+    if (init_weights) std::memcpy(w.data(), init_weights, nw * sizeof(float));
+    gf2l_nms *q = new gf2l_nms;
+    q->code = code;
+    q->T = iterations;
+    q->nbe = code->view.nbe;
+    q->layered = layered;
+    q->fb_continue = layered;
+    q->X = layered ? layered_lds_columns(code->view, ctx->lds_limit) : 0;
+    hipError_t e = hipMalloc(&q->d_w, nw * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&q->d_m, nw * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&q->d_v, nw * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(q->d_w, w.data(), nw * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(q->d_m, 0, nw * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(q->d_v, 0, nw * sizeof(float));
+    if (e != hipSuccess) {
+        hipFree(q->d_w);
+        hipFree(q->d_m);
+        hipFree(q->d_v);
+        delete q;
+        return fail(ctx, GF2L_ERR_HIP, std::string("nms create: ") + hipGetErrorString(e));
+    }
+    *out = q;
+    return GF2L_OK;
+}
+
+}  // namespace
+}  // namespace gf2l
+
 using gf2l::fail;
 
 extern "C" {
@@ -367,32 +584,24 @@ int gf2l_nms_create(gf2l_ctx *ctx, const gf2l_code *code, uint32_t iterations,
                     const float *init_weights, float init_value, gf2l_nms **out) {
     if (!ctx || !out) return GF2L_ERR_INVALID;
     *out = nullptr;
+    return gf2l::guard(ctx, [&]() -> int { return gf2l::create_nms(ctx, code, iterations, init_weights, init_value, false, out); });
+}
+
+int gf2l_nms_create_layered(gf2l_ctx *ctx, const gf2l_code *code, uint32_t iterations, const float *init_weights,
+                            float init_value, gf2l_nms **out) {
+    if (!ctx || !out) return GF2L_ERR_INVALID;
+    *out = nullptr;
+    return gf2l::guard(ctx, [&]() -> int { return gf2l::create_nms(ctx, code, iterations, init_weights, init_value, true, out); });
+}
+
+int gf2l_nms_set_fallback(gf2l_ctx *ctx, gf2l_nms *q, int mode) {
+    if (!ctx || !q) return GF2L_ERR_INVALID;
     return gf2l::guard(ctx, [&]() -> int {
-        if (!code || iterations == 0) return fail(ctx, GF2L_ERR_INVALID, "need a code and iterations >= 1");
-        if (code->is_csr || code->view.dvb_q)
-            return fail(ctx, GF2L_ERR_UNSUPPORTED, "learned weights need a quasi-cyclic code (not DVB)");
-        GF2L_HIP(ctx, hipSetDevice(ctx->device));
-        gf2l_nms *q = new gf2l_nms;
-        q->code = code;
-        q->T = iterations;
-        q->nbe = code->view.nbe;
-        const size_t nw = (size_t)q->T * q->nbe;
-        std::vector<float> w(nw, init_value);
-        if (init_weights) std::memcpy(w.data(), init_weights, nw * sizeof(float));
-        hipError_t e = hipMalloc(&q->d_w, nw * sizeof(float));
-        if (e == hipSuccess) e = hipMalloc(&q->d_m, nw * sizeof(float));
-        if (e == hipSuccess) e = hipMalloc(&q->d_v, nw * sizeof(float));
-        if (e == hipSuccess) e = hipMemcpy(q->d_w, w.data(), nw * sizeof(float), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemset(q->d_m, 0, nw * sizeof(float));
-        if (e == hipSuccess) e = hipMemset(q->d_v, 0, nw * sizeof(float));
-        if (e != hipSuccess) {
-            hipFree(q->d_w);
-            hipFree(q->d_m);
-            hipFree(q->d_v);
-            delete q;
-            return fail(ctx, GF2L_ERR_HIP, std::string("nms create: ") + hipGetErrorString(e));
-        }
-        *out = q;
+        if (mode != GF2L_FALLBACK_RESTART && mode != GF2L_FALLBACK_CONTINUE)
+            return fail(ctx, GF2L_ERR_INVALID, "unknown fallback mode");
+        if (q->layered && mode == GF2L_FALLBACK_RESTART)
+            return fail(ctx, GF2L_ERR_UNSUPPORTED, "a layered decoder's fallback continues from its state");
+        q->fb_continue = mode == GF2L_FALLBACK_CONTINUE;
         return GF2L_OK;
     });
 }
@@ -513,7 +722,12 @@ int gf2l_nms_decode_device(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr,
         if (!d_llr || !d_msg_words || !d_iterations || !d_converged || !d_syndrome_ok)
             return fail(ctx, GF2L_ERR_INVALID, "null buffer");
         GF2L_HIP(ctx, hipSetDevice(ctx->device));
-        return gf2l::decode_device(ctx, q->code, d_llr, frames, q->T, q->d_w, fallback_max_it, d_msg_words,
+        if (q->layered) {
+            if (frames > 0xffffffffull) return fail(ctx, GF2L_ERR_INVALID, "too many frames in one batch");
+            return gf2l::layered_decode(ctx, q, d_llr, (uint32_t)frames, fallback_max_it, d_msg_words, d_iterations,
+                                        d_converged, d_syndrome_ok, d_fallback);
+        }
+        return gf2l::decode_device(ctx, q->code, d_llr, frames, q->T, q->d_w, fallback_max_it, q->fb_continue, d_msg_words,
                                    d_iterations, d_converged, d_syndrome_ok, d_fallback, nullptr, nullptr, 0);
     });
 }
@@ -522,9 +736,10 @@ int gf2l_nms_decode_device(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr,
 
 namespace gf2l {
 
-const float *nms_weights(const gf2l_nms *q, uint32_t *iterations, const gf2l_code **code) {
+const float *nms_weights(const gf2l_nms *q, uint32_t *iterations, const gf2l_code **code, bool *flooding_restart) {
     *iterations = q->T;
     *code = q->code;
+    *flooding_restart = !q->layered && !q->fb_continue;
     return q->d_w;
 }
 

@@ -112,7 +112,7 @@ extern "C" int gf2l_decode_batch_device_noalloc(gf2l_ctx *ctx, const gf2l_code *
         }
         const int path = gf2l::resolve_path(ctx, cv);
         if (path == gf2l::kLds) {  // the whole frame state lives in LDS: no scratch at all
-            GF2L_HIP(ctx, gf2l::launch_lds_decode(cv, d_llr, F, max_it, nullptr, 0, d_msg_words, d_iterations,
+            GF2L_HIP(ctx, gf2l::launch_lds_decode(cv, d_llr, F, max_it, nullptr, 0, false, d_msg_words, d_iterations,
                                                   d_converged, d_syndrome_ok, nullptr, nullptr, nullptr, 0, s));
             return GF2L_OK;
         }
@@ -150,7 +150,7 @@ extern "C" int gf2l_decode_batch_device_noalloc(gf2l_ctx *ctx, const gf2l_code *
         ws.counts = reinterpret_cast<uint32_t *>(base + c.counts);
         ws.llr_perm = cv.dvb_q ? reinterpret_cast<float *>(base + c.llr) : nullptr;
         ws.h_count = h_counts;
-        return gf2l::run_hbm(ctx, cv, path, ws, d_llr, F, max_it, nullptr, 0, d_msg_words, d_iterations,
+        return gf2l::run_hbm(ctx, cv, path, ws, d_llr, F, max_it, nullptr, 0, false, d_msg_words, d_iterations,
                              d_converged, d_syndrome_ok, nullptr, nullptr, nullptr, 0, nullptr, nullptr);
     });
 }

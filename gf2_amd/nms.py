@@ -23,20 +23,37 @@ from .device import DeviceBuffer, generate_frames
 from .ldpc import LdpcCode
 
 
+SCHEDULES = ("flooding", "layered")
+FALLBACKS = ("restart", "continue")
+
+
 class NmsDecoder:
+    """schedule 'flooding' (gf2l_nms_create: all checks, then all variables,
+    per iteration) or 'layered' (gf2l_nms_create_layered: block-rows in turn,
+    beliefs updated in place).  fallback 'restart' re-decodes failing frames
+    from their LLRs (flooding only), 'continue' runs plain min-sum on from the
+    learned decoder's state (the layered decoder's only mode)."""
+
     def __init__(self, code: LdpcCode, iterations: int = 10, weights: Optional[np.ndarray] = None,
-                 init: float = 1.0):
+                 init: float = 1.0, schedule: str = "flooding", fallback: Optional[str] = None):
+        if schedule not in SCHEDULES:
+            raise ValueError(f"schedule must be one of {SCHEDULES}")
         self.code = code
         self.ctx = code.context
+        self.schedule = schedule
         lib = _lib.load()
         h = ctypes.c_void_p()
         w = None
         if weights is not None:
             w = np.ascontiguousarray(weights, dtype=np.float32)
-        _lib.check(self.ctx.handle, lib.gf2l_nms_create(
+        create = lib.gf2l_nms_create_layered if schedule == "layered" else lib.gf2l_nms_create
+        _lib.check(self.ctx.handle, create(
             self.ctx.handle, code.handle, iterations, w.ctypes.data if w is not None else None,
             init, ctypes.byref(h)))
         self._h = h
+        self._fallback = "continue" if schedule == "layered" else "restart"
+        if fallback is not None:
+            self.fallback = fallback
         t, nbe = ctypes.c_uint32(), ctypes.c_uint32()
         lib.gf2l_nms_info(h, ctypes.byref(t), ctypes.byref(nbe))
         self.iterations, self.nbe = t.value, nbe.value
@@ -46,6 +63,18 @@ class NmsDecoder:
     @property
     def handle(self):
         return self._h
+
+    @property
+    def fallback(self) -> str:
+        return self._fallback
+
+    @fallback.setter
+    def fallback(self, mode: str) -> None:
+        if mode not in FALLBACKS:
+            raise ValueError(f"fallback must be one of {FALLBACKS}")
+        _lib.check(self.ctx.handle, _lib.load().gf2l_nms_set_fallback(self.ctx.handle, self._h,
+                                                                       FALLBACKS.index(mode)))
+        self._fallback = mode
 
     # -- parameters -------------------------------------------------------
     @property
@@ -79,7 +108,7 @@ class NmsDecoder:
         m, v, step = self.adam_state()
         info = {"iterations": self.iterations, "base_edges": self.nbe, "n": self.code.n(),
                 "k": self.code.k(), "shifts": self.code.qc.base_matrix.tolist(),
-                "z": self.code.qc.expansion_factor()}
+                "z": self.code.qc.expansion_factor(), "schedule": self.schedule, "fallback": self.fallback}
         info.update(meta or {})
         np.savez_compressed(path, weights=self.weights, adam_m=m, adam_v=v, step=np.array(step, dtype=np.uint64),
                  meta=np.array(json.dumps(info)))
@@ -92,7 +121,8 @@ class NmsDecoder:
                 raise ValueError("checkpoint was trained for a different code")
             if not np.array_equal(np.array(info["shifts"]), code.qc.base_matrix):
                 raise ValueError("checkpoint was trained for a different base matrix")
-            dec = cls(code, int(info["iterations"]), z["weights"])
+            dec = cls(code, int(info["iterations"]), z["weights"], schedule=info.get("schedule", "flooding"),
+                      fallback=info.get("fallback"))
             dec.set_adam_state(z["adam_m"], z["adam_v"], int(z["step"]))
         return dec
 

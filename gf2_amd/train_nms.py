@@ -52,22 +52,23 @@ def recipe(name: str) -> dict:
     raise ValueError(f"unknown code {name!r}")
 
 
-def default_path(code: str, iterations: int = 10) -> str:
-    return os.path.join(HERE, "weights", f"nms_{code}_t{iterations}.npz")
+def default_path(code: str, iterations: int = 10, schedule: str = "flooding") -> str:
+    tag = "_layered" if schedule == "layered" else ""
+    return os.path.join(HERE, "weights", f"nms_{code}{tag}_t{iterations}.npz")
 
 
 def train_one(name: str, iterations: int, ctx: Context, seed: int, out: str = None, init: float = 1.0,
-              **override) -> str:
+              schedule: str = "flooding", **override) -> str:
     r = recipe(name)
     r.update({k: v for k, v in override.items() if v is not None})
     spec = r["spec"]()
     code = LdpcCode(spec.qc(), ctx)
-    dec = NmsDecoder(code, iterations, init=init)
+    dec = NmsDecoder(code, iterations, init=init, schedule=schedule)
     t0 = time.perf_counter()
     losses = train(dec, r["steps"], r["batch"], r["lr"], r["lo"], r["hi"], seed=seed,
                    log_every=max(1, r["steps"] // 4))
     dt = time.perf_counter() - t0
-    out = out or default_path(name, iterations)
+    out = out or default_path(name, iterations, schedule)
     os.makedirs(os.path.dirname(out), exist_ok=True)
     dec.save(out, meta={"code_name": spec.name, "init": init,
                         "recipe": {k: v for k, v in r.items() if k != "spec"},
@@ -89,6 +90,7 @@ def main(argv=None) -> int:
     ap.add_argument("--hi", type=float, help="highest training Eb/N0 (dB)")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--init", type=float, default=1.0, help="initial value of every weight")
+    ap.add_argument("--schedule", choices=["flooding", "layered"], default="flooding")
     ap.add_argument("--out")
     ap.add_argument("--out-dir", help="directory for the checkpoints (default gf2_amd/weights)")
     a = ap.parse_args(argv)
@@ -97,9 +99,9 @@ def main(argv=None) -> int:
     for name in names:
         out = a.out if not a.all_mixed else None
         if a.out_dir:
-            out = os.path.join(a.out_dir, os.path.basename(default_path(name, a.iterations)))
-        train_one(name, a.iterations, ctx, a.seed, out, init=a.init, steps=a.steps, batch=a.batch, lr=a.lr,
-                  lo=a.lo, hi=a.hi)
+            out = os.path.join(a.out_dir, os.path.basename(default_path(name, a.iterations, a.schedule)))
+        train_one(name, a.iterations, ctx, a.seed, out, init=a.init, schedule=a.schedule, steps=a.steps,
+                  batch=a.batch, lr=a.lr, lo=a.lo, hi=a.hi)
     return 0
 
 

@@ -71,6 +71,16 @@ int gf2l_profile_enable(gf2l_ctx *ctx, int on);
 /* ms[3] and launches[3] for the check, variable and syndrome kernels;
  * frame_iterations = frames processed summed over iterations. */
 int gf2l_profile_read(gf2l_ctx *ctx, double *ms, uint64_t *launches, uint64_t *frame_iterations);
+/* One kernel's figures: summed ms, launches, and frame-iterations processed
+ * (for the check / variable / syndrome kernels the HBM paths' frames summed
+ * over iterations, for the layered learned decoder's kernel its iterations
+ * summed over frames). */
+#define GF2L_KERNEL_CHECK 0
+#define GF2L_KERNEL_VARIABLE 1
+#define GF2L_KERNEL_SYNDROME 2
+#define GF2L_KERNEL_LAYERED 3
+int gf2l_profile_read_kernel(gf2l_ctx *ctx, int kernel, double *ms, uint64_t *launches,
+                             uint64_t *frame_iterations);
 
 /* Decoder path selection (tests force each path; default automatic).
  *   GF2L_PATH_AUTO    LDS when a frame's state fits, else STATE when every
@@ -102,11 +112,15 @@ int gf2l_set_path(gf2l_ctx *ctx, int path);
  *   GF2L_TUNE_CHUNK      consecutive items a workgroup of the check-state
  *                        kernels takes per shared-counter fetch (1..64);
  *                        0: fixed-stride item assignment; -1 (default):
- *                        8 when every variable degree is at most 8, else 0 */
+ *                        8 when every variable degree is at most 8, else 0
+ *   GF2L_TUNE_LAYERED_COLUMNS  block-columns of beliefs the layered learned
+ *                        decoder keeps in LDS per workgroup (the rest in
+ *                        global memory); 0 (default): as many as fit */
 #define GF2L_TUNE_WG_PER_CU 1
 #define GF2L_TUNE_XCD_MAP 2
 #define GF2L_TUNE_GRAPH_ITERS 3
 #define GF2L_TUNE_CHUNK 4
+#define GF2L_TUNE_LAYERED_COLUMNS 5
 int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value);
 
 /* ---- codes ------------------------------------------------------------ */
@@ -303,6 +317,31 @@ typedef struct gf2l_nms gf2l_nms;
  * or all equal to init_value. */
 int gf2l_nms_create(gf2l_ctx *ctx, const gf2l_code *code, uint32_t iterations,
                     const float *init_weights, float init_value, gf2l_nms **out);
+/* A learned decoder with the layered (row-sequential) schedule: the
+ * block-rows of the code are processed in ascending order and each updates
+ * the beliefs of its variables in place,
+ *   x = b - w_{t-1}(e) m_{t-1},  m_t = sign * min over the other inputs,
+ *   b = x + w_t(e) m_t
+ * (the reference's alpha * min-sum, llr.rs:352-355, per base edge and
+ * iteration, on a layered instead of a flooding schedule; the reference has
+ * no layered decoder).  Iterations stop early when the hard decisions
+ * satisfy every check.  Its fallback always continues from its state
+ * (GF2L_FALLBACK_CONTINUE).  The same calls train and decode it as a
+ * flooding decoder (gradient, train_step, decode_device, weights, Adam).
+ * Check degree <= 27; the beliefs of as many block-columns as fit stay in
+ * LDS (GF2L_TUNE_LAYERED_COLUMNS). */
+int gf2l_nms_create_layered(gf2l_ctx *ctx, const gf2l_code *code, uint32_t iterations,
+                            const float *init_weights, float init_value, gf2l_nms **out);
+/* What the plain fallback of gf2l_nms_decode_device starts from:
+ *   GF2L_FALLBACK_RESTART   the channel LLRs, zero messages (default of a
+ *                           flooding decoder)
+ *   GF2L_FALLBACK_CONTINUE  the learned decoder's beliefs and last messages:
+ *                           the first plain check update subtracts
+ *                           w_T(e) m_T, later ones the plain messages (the
+ *                           only mode of a layered decoder) */
+#define GF2L_FALLBACK_RESTART 0
+#define GF2L_FALLBACK_CONTINUE 1
+int gf2l_nms_set_fallback(gf2l_ctx *ctx, gf2l_nms *nms, int mode);
 void gf2l_nms_destroy(gf2l_nms *nms);
 int gf2l_nms_info(const gf2l_nms *nms, uint32_t *iterations, uint32_t *weights_per_iteration);
 int gf2l_nms_get_weights(gf2l_ctx *ctx, const gf2l_nms *nms, float *weights);
@@ -356,8 +395,9 @@ void gf2l_codeset_destroy(gf2l_codeset *set);
 int gf2l_decode_mixed_device(gf2l_ctx *ctx, gf2l_codeset *set, const uint16_t *code_ids, size_t frames,
                              const float *d_llr, uint32_t max_it, uint64_t *d_msg_words,
                              uint32_t *d_iterations, uint8_t *d_converged, uint8_t *d_syndrome_ok);
-/* Learned decoding of a mixed batch: decoders[c] is a learned decoder
- * (gf2l_nms_create) of code c of the set, all with the same number T of
+/* Learned decoding of a mixed batch: decoders[c] is a flooding learned
+ * decoder with the restarting fallback (gf2l_nms_create) of code c of the
+ * set, all with the same number T of
  * weighted iterations.  Every frame gets exactly the results of
  * gf2l_nms_decode_device with its code's decoder: T weighted iterations with
  * early stop, then, for frames still failing the syndrome, plain min-sum

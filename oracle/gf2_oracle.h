@@ -117,6 +117,26 @@ void or_framegen(const int32_t *shifts, uint32_t mb, uint32_t nb, uint32_t z, ui
 void or_framegen_zero(uint32_t n, uint32_t k, uint64_t seed, uint64_t g, float ebn0_lo, float ebn0_hi,
                       float *llr);
 
+/* ---- layered learned decoder (layered_oracle.c) ------------------------
+ * Layered weighted min-sum over a quasi-cyclic code (block-rows in order, one
+ * weight per block edge and iteration, w[t*nbe + e], block edges in
+ * block-row-major order), T iterations with early stop, then for a frame
+ * still failing up to fb_it plain flooding iterations continued from the
+ * layered state.  fell_back = 1 when the continuation ran.  trace_bel
+ * (optional, [T + fb_it][n]) receives the beliefs after each layered and
+ * each continuation iteration (rows of iterations not run are untouched). */
+void or_layered_decode(const int32_t *shifts, uint32_t mb, uint32_t nb, uint32_t z, const float *llr, uint32_t T,
+                       const float *w, uint32_t fb_it, uint64_t *msg_words, or_result *res, uint8_t *fell_back,
+                       float *trace_bel);
+/* or_layered_decode over `frames` frames (llr frame-major), `threads` threads. */
+void or_layered_decode_batch(const int32_t *shifts, uint32_t mb, uint32_t nb, uint32_t z, const float *llr,
+                             size_t frames, uint32_t T, const float *w, uint32_t fb_it, uint64_t *msg_words,
+                             or_result *res, uint8_t *fell_back, int threads);
+/* Loss (mean BCE of the beliefs after each of T layered iterations, no early
+ * stop) and its gradient w.r.t. w[t*nbe + e], double precision. */
+double or_layered_loss_grad(const int32_t *shifts, uint32_t mb, uint32_t nb, uint32_t z, const float *llr,
+                            const uint8_t *bits, uint32_t T, const double *w, double *grad);
+
 /* ---- structural restatement of the rayon path (ref_port.cpp) ---------- */
 void *port_code_new(uint32_t m, uint32_t n, const uint32_t *rc, size_t nnz);
 void port_code_free(void *code);

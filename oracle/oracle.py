@@ -74,6 +74,17 @@ def load():
         lib.or_framegen_zero.restype = None
         lib.or_framegen_zero.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64,
                                          ctypes.c_float, ctypes.c_float, vp]
+        lib.or_layered_decode.restype = None
+        lib.or_layered_decode.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp,
+                                          ctypes.c_uint32, vp, ctypes.c_uint32, vp, ctypes.POINTER(OrResult),
+                                          vp, vp]
+        lib.or_layered_decode_batch.restype = None
+        lib.or_layered_decode_batch.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp,
+                                                ctypes.c_size_t, ctypes.c_uint32, vp, ctypes.c_uint32, vp, vp,
+                                                vp, ctypes.c_int]
+        lib.or_layered_loss_grad.restype = ctypes.c_double
+        lib.or_layered_loss_grad.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp, vp,
+                                             ctypes.c_uint32, vp, vp]
         lib.port_code_new.restype = vp
         lib.port_code_new.argtypes = [ctypes.c_uint32, ctypes.c_uint32, vp, ctypes.c_size_t]
         lib.port_code_free.restype = None
@@ -203,6 +214,61 @@ class Code:
         g = np.zeros(T * self.nnz, dtype=np.float64)
         loss = load().or_nms_loss_grad(self._p, _p(llr), _p(b), T, _p(w), _p(g))
         return loss, g.reshape(T, self.nnz)
+
+
+def layered_decode_batch(base: np.ndarray, z: int, llr: np.ndarray, w: np.ndarray, fb_it: int, threads: int = 8):
+    """Layered learned decoder (layered_oracle.c): T = w.shape[0] layered
+    weighted iterations with early stop, then up to fb_it plain flooding
+    iterations continued from the layered state.  w is [T, nbe] block-edge
+    weights (block-row-major).  Returns (msg, iterations, converged,
+    syndrome_ok, fell_back)."""
+    base = np.ascontiguousarray(base, dtype=np.int32)
+    mb, nb = base.shape
+    n, k = nb * z, (nb - mb) * z
+    llr = np.ascontiguousarray(llr, dtype=np.float32).reshape(-1, n)
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    F = llr.shape[0]
+    msg = np.zeros((F, (k + 63) // 64), dtype=np.uint64)
+    res = (OrResult * max(F, 1))()
+    fb = np.zeros(F, dtype=np.uint8)
+    load().or_layered_decode_batch(_p(base), mb, nb, z, _p(llr), F, w.shape[0], _p(w), fb_it, _p(msg), res,
+                                   _p(fb), threads)
+    it = np.array([res[i].iterations for i in range(F)], dtype=np.uint32)
+    conv = np.array([res[i].converged for i in range(F)], dtype=bool)
+    synd = np.array([res[i].syndrome_ok for i in range(F)], dtype=bool)
+    return msg, it, conv, synd, fb.astype(bool)
+
+
+def layered_trace(base: np.ndarray, z: int, llr: np.ndarray, w: np.ndarray, fb_it: int = 0):
+    """One frame: beliefs after each layered and continuation iteration
+    [T + fb_it, n] (rows after an early stop stay NaN), the message words and
+    (iterations, converged, syndrome_ok, fell_back)."""
+    base = np.ascontiguousarray(base, dtype=np.int32)
+    mb, nb = base.shape
+    n, k = nb * z, (nb - mb) * z
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    tb = np.full((w.shape[0] + fb_it, n), np.nan, dtype=np.float32)
+    msg = np.zeros(max((k + 63) // 64, 1), dtype=np.uint64)
+    res = OrResult()
+    fb = np.zeros(1, dtype=np.uint8)
+    l1 = np.ascontiguousarray(llr, dtype=np.float32).reshape(n)
+    load().or_layered_decode(_p(base), mb, nb, z, _p(l1), w.shape[0], _p(w), fb_it, _p(msg), ctypes.byref(res),
+                             _p(fb), _p(tb))
+    return tb, msg, (res.iterations, bool(res.converged), bool(res.syndrome_ok), bool(fb[0]))
+
+
+def layered_loss_grad(base: np.ndarray, z: int, llr: np.ndarray, bits: np.ndarray, w: np.ndarray):
+    """Double-precision loss and gradient [T, nbe] of the layered decoder
+    (T = w.shape[0] iterations, no early stop)."""
+    base = np.ascontiguousarray(base, dtype=np.int32)
+    mb, nb = base.shape
+    n = nb * z
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    g = np.zeros_like(w)
+    l1 = np.ascontiguousarray(llr, dtype=np.float32).reshape(n)
+    b = np.ascontiguousarray(bits, dtype=np.uint8).reshape(n)
+    loss = load().or_layered_loss_grad(_p(base), mb, nb, z, _p(l1), _p(b), w.shape[0], _p(w), _p(g))
+    return loss, g
 
 
 def framegen(base: np.ndarray, z: int, seed: int, frame_id: int, ebn0_lo: float,
