@@ -3,26 +3,32 @@
 Workload: the N=64800, K=32400 (rate 1/2) QC-IRA code with Z=360 and the
 DVB-T2 normal-1/2 degree profile (90x180 base graph, E=226,440 edges), 4096
 frames per batch of BPSK/AWGN channel LLRs (FP32) at Eb/N0 = 1.62 dB, where
-plain min-sum BP reaches FER about 1e-3.  Frames are generated on the GPU
-(seeded Philox, staircase encoder) before the timed region; a step decodes the
-whole resident batch and leaves its outputs in HBM.
+plain min-sum BP reaches FER about 1e-3 (profiles/n64800_fer_sweep.md).
+Frames are generated on the GPU (seeded Philox, staircase encoder) before
+the timed region; a step decodes the whole resident batch and leaves its
+outputs in HBM.
 
-Two decoders are timed on the same batch:
-  learned  the learned normalized min-sum decoder, 10 weighted flooding
-           iterations (weights trained on the GPU, gf2_amd/weights/), frames
-           failing the syndrome re-decoded by plain 50-iteration BP  -> value
-  plain    flooding min-sum BP, 50 iterations, bit-exact with the reference
-           decoder                                                  -> config
-and, when its checkpoint exists, a 50-iteration weighted network with the
-same fallback (config.deeper_learned), which converges most frames itself.
+Decoders timed on the same batch:
+  learned   the layered learned normalized min-sum decoder: 10 weighted
+            row-sequential iterations with early stop (weights trained on the
+            GPU, gf2_amd/weights/nms_n64800_layered_t10.npz); frames still
+            failing continue with plain flooding min-sum from that state for
+            up to 50 iterations                                      -> value
+  plain     flooding min-sum BP, 50 iterations, bit-exact with the reference
+            decoder                                                  -> config
+  flooding  the 10-iteration flooding network (gf2_amd/weights/
+            nms_n64800_t10.npz) with the restarting and with the continuing
+            fallback                                                 -> config
 The JSON line carries
-  roofline      achieved HBM bandwidth of the dominant kernel of the learned
-                run, from HIP events recorded by the library around every
-                launch in the timed region, against the 8 TB/s HBM3E peak
-  cpu_baseline  the structural C++ restatement of the reference's rayon
-                decode_batch (oracle/ref_port.cpp) on a bounded sample of the
-                same frames, on this host's cores; beside it the same
-                arithmetic on flat arrays (oracle/gf2_oracle.c)
+  roofline        the dominant kernel of the learned run, from HIP events the
+                  library records around every launch in the timed region
+                  (per-kernel units from gf2l_profile_read_kernel), against
+                  the 8 TB/s HBM3E peak; its LDS figure beside it
+  roofline_plain  the same for the plain decoder's dominant kernel
+  cpu_baseline    the structural C++ restatement of the reference's rayon
+                  decode_batch (oracle/ref_port.cpp) on a bounded sample of
+                  the same frames, on this host's CPU share and on one thread;
+                  beside it the same arithmetic on flat arrays
 
     python bench.py [--steps K] [--warmup W] [--frames F] [--ebn0 X]
 """
@@ -41,14 +47,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-# HBM bytes per frame-iteration of the check-state kernels on this workload,
-# measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (weighted-iteration
-# dispatches, all 1024 frames running) and scaled by the counts of a plain
-# copy of known size (bytes read = 2 x FETCH_SIZE, written = 1 x WRITE_SIZE):
-# profiles/n64800_state_kernels_bench.md.
-MEASURED_TRAFFIC_PER_FI = {"check": 827_374 + 389_200, "variable": 1_173_078 + 271_543}
-# Plain min-sum FER on this code is about 1e-3 here: the gf2_amd.fer sweep
-# measured 2.1e-3 at 1.60 dB (16384 frames) and 0 errors in 16384 at 1.65 dB.
+# LDS: 256 CUs x 128 bytes per clock x 2.4 GHz
+LDS_PEAK_GBPS = 256 * 128 * 2.4
+# HBM bytes per frame-iteration measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE on this workload (python bench.py --steps 1 --warmup 0 --no-cpu,
+# profiles/n64800_layered_bench.md), scaled by what the counters read for a
+# plain copy of known size (gf2_amd/hbm_probe: FETCH_SIZE counts half the
+# bytes read, WRITE_SIZE the bytes written).
+MEASURED_TRAFFIC_PER_FI = {"layered": 1832398, "check": 1201488, "variable": 1477744}
+# CPU threads of one GPU's share of the box
+CPU_SHARE = 16
 DEFAULT_EBN0 = 1.62
 
 
@@ -60,37 +68,46 @@ def parse():
     ap.add_argument("--frames", type=int, default=4096)
     ap.add_argument("--ebn0", type=float, default=DEFAULT_EBN0)
     ap.add_argument("--max-it", type=int, default=50)
-    ap.add_argument("--weights", default=os.path.join(ROOT, "gf2_amd", "weights", "nms_n64800_t10.npz"))
-    ap.add_argument("--deep-weights", default=os.path.join(ROOT, "gf2_amd", "weights", "nms_n64800_t50.npz"),
-                    help="a deeper learned decoder timed beside the headline one ('' to skip)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--weights", default=os.path.join(ROOT, "gf2_amd", "weights", "nms_n64800_layered_t10.npz"))
+    ap.add_argument("--flooding-weights", default=os.path.join(ROOT, "gf2_amd", "weights", "nms_n64800_t10.npz"),
+                    help="the 10-iteration flooding network timed beside the headline one ('' to skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--wg-per-cu", type=int, default=0, help="override the launch tuning")
-    ap.add_argument("--xcd-map", type=int, default=-1, help="override the launch tuning")
-    ap.add_argument("--path", default="auto", choices=["auto", "state", "stream"])
+    ap.add_argument("--layered-columns", type=int, default=0, help="override GF2L_TUNE_LAYERED_COLUMNS")
     return ap.parse_args()
 
 
 def cpu_baseline(spec, base, llr, max_it, seconds):
     """Time the structural restatement of the rayon path on a bounded sample."""
     from oracle import oracle as orc
-    threads = max(1, min(16, os.cpu_count() or 1))
+    nproc = os.cpu_count() or 1
+    threads = max(1, min(CPU_SHARE, nproc))
     oc = orc.Code.from_qc(base, spec.z)
-    done = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds and done + threads <= llr.shape[0]:
-        oc.port_decode_batch(llr[done:done + threads], max_it, threads)
-        done += threads
-    dt = time.perf_counter() - t0
+
+    def port(th, budget):
+        done = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget and done + th <= llr.shape[0]:
+            oc.port_decode_batch(llr[done:done + th], max_it, th)
+            done += th
+        return done, time.perf_counter() - t0
+
+    done, dt = port(threads, seconds)
     out = {"value": done * spec.k / dt / 1e9, "unit": "Gbit/s", "cores": threads, "kind": "port",
-           "sample": f"first {done} frames of the same batch, {dt:.1f} s, plain min-sum "
-                     f"{max_it} iterations, oracle/ref_port.cpp with {threads} worker threads"}
+           "nproc": nproc,
+           "sample": f"first {done} frames of the same batch, {dt:.1f} s, plain min-sum {max_it} iterations, "
+                     f"oracle/ref_port.cpp with {threads} worker threads (the box's CPU share for one GPU; "
+                     f"nproc = {nproc})"}
+    d1, t1 = port(1, seconds / 3)
+    if d1:
+        out["single_thread"] = {"value": d1 * spec.k / t1 / 1e9, "cores": 1,
+                                "sample": f"first {d1} frames, {t1:.1f} s, oracle/ref_port.cpp on one thread"}
     # the same arithmetic on flat CSR/CSC arrays (the oracle's decoder), so the
     # GPU/CPU ratio does not rest on the reference's per-frame allocations
     n2 = 0
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds / 2 and n2 + 4 * threads <= llr.shape[0]:
+    while time.perf_counter() - t0 < seconds / 3 and n2 + 4 * threads <= llr.shape[0]:
         oc.decode_batch(llr[n2:n2 + 4 * threads], max_it, threads=threads)
         n2 += 4 * threads
     dt2 = time.perf_counter() - t0
@@ -116,12 +133,8 @@ def main():
     ctx = Context(0)
     code = LdpcCode(spec.qc(), ctx)
     lib = _lib.load()
-    if a.wg_per_cu:
-        _lib.check(ctx.handle, lib.gf2l_set_tuning(ctx.handle, _lib.TUNE_WG_PER_CU, a.wg_per_cu))
-    if a.xcd_map >= 0:
-        _lib.check(ctx.handle, lib.gf2l_set_tuning(ctx.handle, _lib.TUNE_XCD_MAP, a.xcd_map))
-    ctx.set_path({"auto": _lib.PATH_AUTO, "state": _lib.PATH_STATE,
-                  "stream": _lib.PATH_STREAM}[a.path])
+    if a.layered_columns:
+        ctx.set_tuning(_lib.TUNE_LAYERED_COLUMNS, a.layered_columns)
     F, kw = a.frames, (spec.k + 63) // 64
     d_llr, d_cw = generate_frames(code, a.seed, 0, F, a.ebn0, a.ebn0)
     info = channel.unpack_words(d_cw.download(np.uint64, (F, (spec.n + 63) // 64)), spec.n)[:, :spec.k]
@@ -135,9 +148,11 @@ def main():
             outs[3].ptr))
         ctx.synchronize()
 
-    def learned_step():
-        learned.decode_device(d_llr, F, a.max_it, *outs)
-        ctx.synchronize()
+    def nms_step(dec):
+        def step():
+            dec.decode_device(d_llr, F, a.max_it, *outs)
+            ctx.synchronize()
+        return step
 
     def run(step, profile):
         for _ in range(a.warmup):
@@ -152,57 +167,82 @@ def main():
         dt = time.perf_counter() - t0
         prof = None
         if profile:
-            ms = (ctypes.c_double * 3)()
-            la = (ctypes.c_uint64 * 3)()
-            fi = ctypes.c_uint64()
-            lib.gf2l_profile_read(ctx.handle, ms, la, ctypes.byref(fi))
+            prof = {}
+            for k, name in enumerate(["check", "variable", "syndrome", "layered"]):
+                ms, la, fi = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
+                lib.gf2l_profile_read_kernel(ctx.handle, k, ctypes.byref(ms), ctypes.byref(la), ctypes.byref(fi))
+                prof[name] = (ms.value, la.value, fi.value)
             lib.gf2l_profile_enable(ctx.handle, 0)
-            prof = (list(ms), list(la), fi.value)
-        msg = outs[0].download(np.uint64, (F, kw))
         it = outs[1].download(np.uint32, F)
         conv = outs[2].download(np.uint8, F).astype(bool)
         fb = outs[4].download(np.uint8, F).astype(bool)
-        bits = channel.unpack_words(msg, spec.k)
+        bits = channel.unpack_words(outs[0].download(np.uint64, (F, kw)), spec.k)
         fe = int(((~conv) | (bits != info).any(axis=1)).sum())
         return dt, it, fe, fb, prof
 
-    dt_p, it_p, fe_p, _, _ = run(plain_step, False)
-    deep = None
-    if a.deep_weights and os.path.exists(a.deep_weights):
-        # a deeper weighted network with the same fallback, reported beside
-        # the specified 10-iteration one (not the headline value)
-        deep_dec = NmsDecoder.load(code, a.deep_weights)
+    gbps = lambda dt: a.steps * F * spec.k / dt / 1e9
+    dt_p, it_p, fe_p, _, prof_p = run(plain_step, True)
+    flooding = None
+    if a.flooding_weights and os.path.exists(a.flooding_weights):
+        fdec = NmsDecoder.load(code, a.flooding_weights)
+        flooding = {"weights": os.path.relpath(a.flooding_weights, ROOT), "weighted_iterations": fdec.iterations}
+        for mode in ("restart", "continue"):
+            fdec.fallback = mode
+            dt_f, it_f, fe_f, fb_f, _ = run(nms_step(fdec), False)
+            flooding[mode] = {"gbps": round(gbps(dt_f), 4), "fer": fe_f / F,
+                              "mean_iterations": round(float(it_f.mean()), 3),
+                              "fallback_fraction": round(float(fb_f.mean()), 4),
+                              "over_plain": round(dt_p / dt_f, 4)}
+    dt_l, it_l, fe_l, fb_l, prof_l = run(nms_step(learned), True)
 
-        def deep_step():
-            deep_dec.decode_device(d_llr, F, a.max_it, *outs)
-            ctx.synchronize()
+    cv_n, M, z = spec.n, spec.n - spec.k, spec.z
+    nb = spec.n // z
+    xc = ctypes.c_uint32()
+    _lib.check(ctx.handle, lib.gf2l_nms_layered_columns(ctx.handle, learned.handle, ctypes.byref(xc)))
+    X = xc.value
+    edges_lds = int((base[:, :X] >= 0).sum()) * z
+    E = int((base >= 0).sum()) * z
+    # Algorithmic bytes per frame-iteration of each kernel (unique data it must move):
+    #   check     read the frame's beliefs once (4N), read and write the 12-byte state of every check (24M)
+    #   variable  read every check state once (12M), read L (4N), write beliefs (4N) and hard bits (N/8)
+    #   syndrome  read the hard bits (N/8)
+    #   layered   HBM: read and write the 12-byte state of every check (24M), read and write the
+    #             beliefs of block-columns outside LDS once per edge (8 E_glob) and read them once
+    #             for the hard decisions (4 z (nb - X)); LDS: the same for the X columns in LDS
+    per_fi = {"check": 4 * cv_n + 24 * M, "variable": 12 * M + 8 * cv_n + cv_n // 8, "syndrome": cv_n // 8,
+              "layered": 24 * M + 8 * (E - edges_lds) + 4 * z * (nb - X)}
+    lds_per_fi = 8 * edges_lds + 4 * z * X
+    kernels = {"check": "cn_state_kernel", "variable": "vn_state_kernel", "syndrome": "syn_words_kernel",
+               "layered": "lnms_fwd_kernel"}
 
-        dt_d, it_d, fe_d, fb_d, _ = run(deep_step, False)
-        deep = {"weights": os.path.relpath(a.deep_weights, ROOT), "weighted_iterations": deep_dec.iterations,
-                "gbps": round(a.steps * F * spec.k / dt_d / 1e9, 4), "fer": fe_d / F,
-                "mean_iterations": round(float(it_d.mean()), 3),
-                "fallback_fraction": round(float(fb_d.mean()), 4),
-                "over_plain": round(dt_p / dt_d, 4)}
-    dt_l, it_l, fe_l, fb_l, (ms, launches, frame_iters) = run(learned_step, True)
-    gbps_p = a.steps * F * spec.k / dt_p / 1e9
-    gbps_l = a.steps * F * spec.k / dt_l / 1e9
+    def roofline(prof):
+        dom = max(prof, key=lambda k: prof[k][0])
+        ms, launches, units = prof[dom]
+        achieved = per_fi[dom] * units / (ms * 1e-3) / 1e9
+        r = {"bound": "hbm", "kernel": kernels[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+             "traffic": (round(MEASURED_TRAFFIC_PER_FI[dom] * units / launches)
+                         if dom in MEASURED_TRAFFIC_PER_FI and launches else None),
+             "traffic_unit": "HBM bytes per launch (rocprofv3 bytes per frame-iteration on this workload x "
+                             "frame-iterations per launch)",
+             "avg_launch_us": round(ms / launches * 1e3, 2), "launches": int(launches),
+             "frame_iterations": int(units), "bytes_per_frame_iteration": per_fi[dom],
+             "kernel_ms": {k: round(v[0], 3) for k, v in prof.items() if v[1]}}
+        if dom == "layered":
+            lds = lds_per_fi * units / (ms * 1e-3) / 1e9
+            r["lds"] = {"achieved": round(lds, 1), "peak": round(LDS_PEAK_GBPS, 1), "unit": "GB/s",
+                        "frac": round(lds / LDS_PEAK_GBPS, 4), "bytes_per_frame_iteration": lds_per_fi,
+                        "lds_columns": X}
+        if dom == "variable":
+            # the whole iteration against the uncompressed message model 16E + 4N
+            fi = prof["variable"][2]
+            t_it = sum(prof[k][0] for k in ("check", "variable", "syndrome"))
+            r["message_model_effective_gbps"] = round((16 * E + 4 * cv_n) * fi / (t_it * 1e-3) / 1e9, 1)
+        return r
 
-    N, M = spec.n, spec.n - spec.k
-    # Algorithmic bytes per frame-iteration of each kernel of the compressed
-    # check-state path (the unique data each kernel must move):
-    #   check     read the frame's beliefs once (4N), read and write the
-    #             12-byte state of every check (24M)
-    #   variable  read every check state once (12M), read L (4N), write
-    #             beliefs (4N) and hard-decision bits (N/8)
-    #   syndrome  read the hard-decision bits (N/8)
-    per_fi = {"check": 4 * N + 24 * M, "variable": 12 * M + 8 * N + N // 8, "syndrome": N // 8}
-    names = ["check", "variable", "syndrome"]
-    kernels = ["cn_state_kernel", "vn_state_kernel", "syn_words_kernel"]
-    dom = max(range(3), key=lambda k: ms[k])
-    achieved = per_fi[names[dom]] * frame_iters / (ms[dom] * 1e-3) / 1e9
     out = {
         "metric": "info Gbit/s on 1 MI355X at FER 1e-3, learned vs plain BP, N=64800 r=1/2",
-        "value": round(gbps_l, 4),
+        "value": round(gbps(dt_l), 4),
         "unit": "Gbit/s",
         "n_gpus": 1,
         "steps": a.steps,
@@ -216,30 +256,21 @@ def main():
         "config": {
             "workload": "QC-IRA N=64800 K=32400 Z=360 (E=226440), 4096 frames/batch, Eb/N0 "
                         f"{a.ebn0} dB",
-            "decoder": "learned NMS, 10 weighted flooding iterations + plain BP fallback (50)",
+            "decoder": f"learned NMS, {learned.iterations} weighted layered iterations + plain BP "
+                       f"continuing from that state ({a.max_it})",
             "weights": os.path.relpath(a.weights, ROOT),
-            "learned_gbps": round(gbps_l, 4), "learned_fer": fe_l / F,
+            "learned_gbps": round(gbps(dt_l), 4), "learned_fer": fe_l / F,
             "learned_mean_iterations": round(float(it_l.mean()), 3),
             "learned_fallback_fraction": round(float(fb_l.mean()), 4),
-            "plain_bp_gbps": round(gbps_p, 4), "plain_bp_fer": fe_p / F,
+            "plain_bp_gbps": round(gbps(dt_p), 4), "plain_bp_fer": fe_p / F,
             "plain_bp_mean_iterations": round(float(it_p.mean()), 3),
             "plain_bp_ms_per_step": round(dt_p / a.steps * 1e3, 3),
-            "learned_over_plain": round(gbps_l / gbps_p, 4),
+            "learned_over_plain": round(dt_p / dt_l, 4),
+            "flooding_learned": flooding,
             "frames": F, "ebn0_db": a.ebn0, "max_iterations": a.max_it,
-            "deeper_learned": deep,
         },
-        "roofline": {
-            "bound": "hbm", "kernel": kernels[dom],
-            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": (round(MEASURED_TRAFFIC_PER_FI[names[dom]] * frame_iters / launches[dom])
-                        if a.path in ("auto", "state") and names[dom] in MEASURED_TRAFFIC_PER_FI else None),
-            "traffic_unit": "bytes per launch (measured bytes per frame-iteration x mean frame-iterations per launch)",
-            "avg_launch_us": round(ms[dom] / launches[dom] * 1e3, 2),
-            "launches": int(launches[dom]),
-            "bytes_per_frame_iteration": per_fi[names[dom]],
-            "kernel_ms": {names[k]: round(ms[k], 3) for k in range(3)},
-        },
+        "roofline": roofline({k: v for k, v in prof_l.items() if v[1]}),
+        "roofline_plain": roofline({k: v for k, v in prof_p.items() if v[1]}),
         "cpu_baseline": None,
     }
     if not a.no_cpu:

@@ -108,6 +108,7 @@ SIGNATURES = {
     "gf2l_nms_create_layered": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                ctypes.c_void_p, ctypes.c_float, _c_void_pp]),
     "gf2l_nms_set_fallback": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "gf2l_nms_layered_columns": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _u32p]),
     "gf2l_nms_destroy": (None, [ctypes.c_void_p]),
     "gf2l_nms_info": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p]),
     "gf2l_nms_get_weights": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

@@ -594,6 +594,12 @@ int gf2l_nms_create_layered(gf2l_ctx *ctx, const gf2l_code *code, uint32_t itera
     return gf2l::guard(ctx, [&]() -> int { return gf2l::create_nms(ctx, code, iterations, init_weights, init_value, true, out); });
 }
 
+int gf2l_nms_layered_columns(gf2l_ctx *ctx, const gf2l_nms *q, uint32_t *columns) {
+    if (!ctx || !q || !columns) return GF2L_ERR_INVALID;
+    *columns = q->layered ? gf2l::layered_x(ctx, q) : 0;
+    return GF2L_OK;
+}
+
 int gf2l_nms_set_fallback(gf2l_ctx *ctx, gf2l_nms *q, int mode) {
     if (!ctx || !q) return GF2L_ERR_INVALID;
     return gf2l::guard(ctx, [&]() -> int {

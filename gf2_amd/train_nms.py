@@ -6,6 +6,8 @@
     python -m gf2_amd.train_nms --all-mixed   (one checkpoint per Z=360 code)
     python -m gf2_amd.train_nms --code n64800 --iterations 50 --steps 200 --init 0.9 \
         --lr 3e-3 --lo 1.4 --hi 1.8
+    python -m gf2_amd.train_nms --code n64800 --schedule layered
+        [--out gf2_amd/weights/nms_n64800_layered_t10.npz]
 
 Recipes (defaults per code):
   n2304   N=2304 K=1152 Z=96 QC-IRA: 1024 steps x 4096 frames = 2^22 frames,
@@ -16,6 +18,9 @@ Recipes (defaults per code):
   Z=360 codes of the mixed set: 300 steps x 256 frames (N=64800) or x 1024
           frames (N=16200), Adam lr 1e-2, Eb/N0 uniform in [e - 1.0, e + 0.5]
           dB around the code's plain-BP FER-1e-2 point e (codes.FER_1E2_EBN0).
+  layered schedule (--schedule layered): 1024 steps x 4096 frames = 2^22
+          frames, Adam lr 1e-3, weights start at 0.85; Eb/N0 uniform in
+          [1, 3] dB (n2304) or [1.4, 1.8] dB (n64800).
 Frames come from the on-device generator (seeded Philox, staircase encoder,
 BPSK/AWGN); every step draws fresh frames.
 """
@@ -33,6 +38,13 @@ from .nms import NmsDecoder, train
 RECIPES = {
     "n2304": dict(spec=codes.n2304_r12, steps=1024, batch=4096, lr=1e-3, lo=1.0, hi=3.0),
     "n64800": dict(spec=codes.n64800_r12, steps=400, batch=256, lr=1e-2, lo=1.0, hi=2.0),
+}
+# Layered schedule: 2^22 frames in batches of 4096 with Adam lr 1e-3 (the
+# N=2304 recipe's scale), Eb/N0 around the code's FER-1e-3 region, weights
+# starting at 0.85 (the best constant for this schedule).
+LAYERED_RECIPES = {
+    "n2304": dict(spec=codes.n2304_r12, steps=1024, batch=4096, lr=1e-3, lo=1.0, hi=3.0, init=0.85),
+    "n64800": dict(spec=codes.n64800_r12, steps=1024, batch=4096, lr=1e-3, lo=1.4, hi=1.8, init=0.85),
 }
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -57,10 +69,12 @@ def default_path(code: str, iterations: int = 10, schedule: str = "flooding") ->
     return os.path.join(HERE, "weights", f"nms_{code}{tag}_t{iterations}.npz")
 
 
-def train_one(name: str, iterations: int, ctx: Context, seed: int, out: str = None, init: float = 1.0,
+def train_one(name: str, iterations: int, ctx: Context, seed: int, out: str = None, init: float = None,
               schedule: str = "flooding", **override) -> str:
-    r = recipe(name)
+    r = dict(LAYERED_RECIPES[name]) if schedule == "layered" and name in LAYERED_RECIPES else recipe(name)
     r.update({k: v for k, v in override.items() if v is not None})
+    init = init if init is not None else r.pop("init", 1.0)
+    r.pop("init", None)
     spec = r["spec"]()
     code = LdpcCode(spec.qc(), ctx)
     dec = NmsDecoder(code, iterations, init=init, schedule=schedule)
@@ -89,7 +103,8 @@ def main(argv=None) -> int:
     ap.add_argument("--lo", type=float, help="lowest training Eb/N0 (dB)")
     ap.add_argument("--hi", type=float, help="highest training Eb/N0 (dB)")
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--init", type=float, default=1.0, help="initial value of every weight")
+    ap.add_argument("--init", type=float, default=None,
+                    help="initial value of every weight (default 1.0; the layered recipes' own value)")
     ap.add_argument("--schedule", choices=["flooding", "layered"], default="flooding")
     ap.add_argument("--out")
     ap.add_argument("--out-dir", help="directory for the checkpoints (default gf2_amd/weights)")

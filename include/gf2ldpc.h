@@ -342,6 +342,9 @@ int gf2l_nms_create_layered(gf2l_ctx *ctx, const gf2l_code *code, uint32_t itera
 #define GF2L_FALLBACK_RESTART 0
 #define GF2L_FALLBACK_CONTINUE 1
 int gf2l_nms_set_fallback(gf2l_ctx *ctx, gf2l_nms *nms, int mode);
+/* Block-columns of beliefs a layered decoder keeps in LDS when it runs on
+ * this context (0 for a flooding decoder). */
+int gf2l_nms_layered_columns(gf2l_ctx *ctx, const gf2l_nms *nms, uint32_t *columns);
 void gf2l_nms_destroy(gf2l_nms *nms);
 int gf2l_nms_info(const gf2l_nms *nms, uint32_t *iterations, uint32_t *weights_per_iteration);
 int gf2l_nms_get_weights(gf2l_ctx *ctx, const gf2l_nms *nms, float *weights);
