@@ -29,6 +29,8 @@ TUNE_XCD_MAP = 2
 TUNE_GRAPH_ITERS = 3
 TUNE_CHUNK = 4
 TUNE_LAYERED_COLUMNS = 5
+TUNE_LAYERED_KERNEL = 6
+LAYERED_AUTO, LAYERED_GENERAL, LAYERED_STAIRCASE = 0, 1, 2
 
 KERNEL_CHECK = 0
 KERNEL_VARIABLE = 1

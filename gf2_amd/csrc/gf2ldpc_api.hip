@@ -639,6 +639,10 @@ int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value) {
                 if (value < 0) return fail(ctx, GF2L_ERR_INVALID, "layered columns must be >= 0");
                 ctx->tune.layered_cols = value;
                 return GF2L_OK;
+            case GF2L_TUNE_LAYERED_KERNEL:
+                if (value < 0 || value > 2) return fail(ctx, GF2L_ERR_INVALID, "layered kernel out of [0,2]");
+                ctx->tune.layered_kernel = value;
+                return GF2L_OK;
             default:
                 return fail(ctx, GF2L_ERR_INVALID, "unknown tuning key");
         }

@@ -99,7 +99,8 @@ struct Tuning {
     int xcd_map = 1;     // 1: frames pinned to XCDs (L2 reuse), 0: frame-major over the grid
     int graph_iters = 4; // iterations per graph replay of the mixed-code decoder
     int chunk = -1;      // items per counter fetch of the check-state kernels (0: fixed stride, -1: automatic)
-    int layered_cols = 0; // block-columns of beliefs the layered decoder keeps in LDS (0: as many as fit)
+    int layered_cols = 0; // block-columns of beliefs the general layered kernel keeps in LDS (0: automatic)
+    int layered_kernel = 0; // 0: staircase kernel where it applies, 1: general kernel, 2: staircase kernel
 };
 
 // Word offset of the item counters in Workspace::counts and its size (two
@@ -139,6 +140,7 @@ struct LayeredArgs {
     uint8_t *conv, *synd, *fb;
     uint32_t *tape_state;    // training: [T][F][mb][zp] uint4 (min1, min2, meta, idx2 | argmin input signs)
     float *tape_bel;         // training: [T][F][nb][zp] beliefs after each iteration
+    uint64_t stair_deg[4];   // staircase kernel: bit r set when block-row r has the higher of its two degrees
 };
 
 // Arguments of the layered backward kernel (layered.hip).
@@ -288,6 +290,13 @@ size_t layered_lds_bytes(const CodeView &cv, uint32_t X);
 uint32_t layered_lds_columns(const CodeView &cv, size_t lds_limit);
 hipError_t launch_layered_fwd(const LayeredArgs &a, bool tape, int num_cus, hipStream_t s);
 hipError_t launch_layered_bwd(const LayeredBwdArgs &a, int num_cus, hipStream_t s);
+// Staircase kernel (decoding only, X = kb information columns in LDS): its
+// LDS bytes, the row degree of its instantiation for a code (0: the code
+// lacks the structure; deg_mask, when given, receives the block-rows of that
+// degree as LayeredArgs::stair_deg) and its launcher.
+size_t layered_stair_bytes(const CodeView &cv);
+uint32_t layered_stair_degree(const gf2l_code *code, uint64_t *deg_mask = nullptr);
+hipError_t launch_layered_stair(const LayeredArgs &a, uint32_t dh, int num_cus, hipStream_t s);
 
 // Plain min-sum on a general sparse code (csr.hip).
 int decode_csr(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, uint32_t F, uint32_t max_it,

@@ -201,6 +201,334 @@ __global__ void __launch_bounds__(512) lnms_fwd_kernel(LayeredArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Staircase kernel (decoding only).  For a QC-IRA code whose parity part is a
+// block staircase of unshifted circulants (block-row r has the parity
+// block-columns kb+r-1 and kb+r as its last two edges, shift 0), check i of
+// every block-row touches parity bit i of its parity columns: those beliefs
+// are private to thread i (the workgroup has exactly z threads).  The kb
+// information columns live in LDS; the parity beliefs and the check states
+// stay in the frame's global rows, each touched only by its owner thread.
+// So
+//   - the layers are separated by an LDS-only barrier (no wait for the
+//     thread's outstanding global loads and stores);
+//   - the state and the belief of column kb+r that layer r reads are loaded
+//     P layers earlier, P a divisor of mb (the same thread stored them at
+//     least two layers before the load is issued, in program order);
+//   - column kb+r, updated by layer r, reaches layer r+1 in a register;
+//   - the hard decisions of a parity column are taken (ballot) when its
+//     belief is final for the iteration, those of the information columns
+//     from LDS after the last layer.
+// The arithmetic and its order are those of the general kernel with X = kb.
+// The vector-memory counter retires loads and stores in issue order, and
+// the compiler's wait for a loaded register is exact only where every path
+// to it issued the same memory operations: so the loop body has no early
+// exit (iterations end at group boundaries, P dividing mb), and every global
+// store is unconditional (block-row 0 stores its position D-2 value into the
+// unused global row of block-column kb-1, every block-row its column kb+r
+// as it stands; the information rows are rewritten at hand-over).
+
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// LDS pointers typed as such, so every access is a ds_ instruction (a
+// generic pointer held in a struct compiles to flat accesses, which count
+// against the vector-memory counter and would wait for the loads ahead).
+typedef __attribute__((address_space(3))) float lds_float;
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+struct StairFrame {
+    lds_u64 *hb;       // [nb][zw] hard decisions
+    const lds_u32 *tab; // [nbe][4] per edge: LDS byte address of its column's
+                        // beliefs (information columns), shift, w_t, w_{t-1}
+    float *gb;         // the frame's global belief rows [nb][zp]
+    uint3 *st;         // the frame's check states [mb][zp]
+    const float *llr;  // the frame's channel LLRs [n]
+    uint32_t kb, i;    // i = threadIdx.x < z
+};
+
+__device__ __forceinline__ float lds_ld(uint32_t addr) { return *(const lds_float *)(size_t)addr; }
+__device__ __forceinline__ void lds_st(uint32_t addr, float v) { *(lds_float *)(size_t)addr = v; }
+// x * (-1)^s for s in {0, 1}: a sign flip, exact
+__device__ __forceinline__ float flip(float x, uint32_t s) { return __uint_as_float(__float_as_uint(x) ^ (s << 31)); }
+
+// Layer br (edges e0..e0+D-1) of iteration t (t2: t >= 2).  (o, bo): this
+// layer's check states and belief of column kb+br, loaded ahead.  Code
+// tables and weights come from LDS: no scalar memory loads in the layer loop
+// (their counter would also wait for the LDS accesses).  In iteration 1 the
+// old message is rebuilt from a zero state with weight 0: x = b - (+0), which
+// is b exactly (also for -0 and infinities).
+//
+// The check rule is the general kernel's (CheckAcc) specialised to D <= 8:
+// with fewer than nine inputs every input of the reference's min-sum falls in
+// the remainder after the full 8-lane chunks, whose sign rule is x < 0, so the
+// output sign of position p is the parity of (x < 0) over the other inputs;
+// w * (+-m) is computed as +-(w * m) (IEEE multiplication: the same value).
+template <int D>
+__device__ __forceinline__ void stair_layer(const LayeredArgs &a, const StairFrame &S, uint32_t br, uint32_t e0,
+                                            bool t2, const uint3 o, const float bo, float &carry) {
+    static_assert(D >= 2 && D <= 8, "the staircase layer's sign rule needs 2 <= D <= 8");
+    const CodeView &cv = a.cv;
+    const bool cin = br > 0;  // column kb+br-1 at position D-2
+    uint32_t cb[D], sh[D];
+    float wt[D], wp[D];
+#pragma unroll
+    for (int p = 0; p < D; ++p) {
+        const lds_u32 *r = S.tab + 4 * (e0 + p);
+        cb[p] = r[0];
+        sh[p] = r[1];
+        wt[p] = __uint_as_float(r[2]);
+        const float wpv = __uint_as_float(r[3]);  // read unconditionally (stale in iteration 1)
+        wp[p] = t2 ? wpv : 0.0f;
+    }
+    const float m1o = t2 ? __uint_as_float(o.x) : 0.0f, m2o = t2 ? __uint_as_float(o.y) : 0.0f;
+    const uint32_t mo = t2 ? o.z : 0u, io = mo & 31u;
+    float x[D];
+    uint32_t ad[D];
+#pragma unroll
+    for (int p = 0; p < D; ++p) {
+        float bv;
+        ad[p] = cb[p] + 4u * wrap_add(S.i, sh[p], cv.z);
+        if (p == D - 1) {
+            bv = bo;
+        } else if (p == D - 2) {
+            // read either way, no branch among the reads (the table gives
+            // parity edges the address of column 0: in bounds, unused)
+            const float lv = lds_ld(ad[p]);
+            bv = cin ? carry : lv;
+        } else {
+            bv = lds_ld(ad[p]);
+        }
+        const float mag = (uint32_t)p == io ? m2o : m1o;
+        x[p] = bv - flip(wp[p] * mag, (mo >> (5 + p)) & 1u);
+    }
+    float m1 = __builtin_inff(), m2 = __builtin_inff();
+    uint32_t idx = 0, lt = 0;
+#pragma unroll
+    for (int p = 0; p < D; ++p) {  // strict comparisons: the first of equal magnitudes keeps idx
+        const float av = fabsf(x[p]);
+        const bool l1 = av < m1, l2 = av < m2;
+        m2 = l1 ? m1 : (l2 ? av : m2);
+        m1 = l1 ? av : m1;
+        idx = l1 ? (uint32_t)p : idx;
+        lt |= (x[p] < 0.0f ? 1u : 0u) << p;
+    }
+    const uint32_t sg = ((__popc(lt) & 1u) ? (1u << D) - 1u : 0u) ^ lt;  // output sign bits
+    const uint32_t meta = idx | (sg << 5);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int p = 0; p < D; ++p) {
+        const float mag = (uint32_t)p == idx ? m2 : m1;
+        const float v = x[p] + flip(wt[p] * mag, (sg >> p) & 1u);
+        if (p == D - 1) {
+            carry = v;  // column kb+br: to layer br+1, final now for the last block-row
+            S.gb[(S.kb + br) * cv.zp + S.i] = v;
+            if (br == cv.mb - 1) {
+                const uint64_t m = __ballot(v < 0.0f);
+                if (lane == 0) S.hb[(cv.nb - 1) * cv.zw + wave] = m;
+            }
+        } else if (p == D - 2) {
+            // column kb+br-1, final for this iteration (block-row 0: an
+            // information column; the global row of column kb-1 takes the
+            // store, unused until hand-over rewrites it)
+            S.gb[(S.kb + br - 1) * cv.zp + S.i] = v;
+            if (cin) {
+                const uint64_t m = __ballot(v < 0.0f);
+                if (lane == 0) S.hb[(S.kb + br - 1) * cv.zw + wave] = m;
+            } else {
+                lds_st(ad[p], v);
+            }
+        } else {
+            lds_st(ad[p], v);
+        }
+    }
+    S.st[br * cv.zp + S.i] = make_uint3(__float_as_uint(m1), __float_as_uint(m2), meta);
+}
+
+// Parity checks over the hard-decision words, the code's edge tables read
+// from LDS copies (rp, re); uniform over the workgroup.
+__device__ __forceinline__ uint64_t cyc64_lds(const lds_u64 *v, uint32_t z, uint32_t zw, uint32_t pos) {
+    const uint32_t wi = pos >> 6, sh = pos & 63u;
+    uint64_t r = v[wi] >> sh;
+    if (sh && wi + 1 < zw) r |= v[wi + 1] << (64u - sh);
+    const uint32_t avail = z - pos;
+    if (avail < 64u) r |= v[0] << avail;
+    return r;
+}
+
+// The staircase kernel runs z threads (zw waves): the block size comes from
+// the code view (a scalar), not from the dispatch packet (a vector load,
+// whose wait would also wait for every load ahead).
+__device__ __forceinline__ uint32_t stair_unsat(const CodeView &cv, const lds_u64 *sh, const lds_u32 *rp,
+                                                const lds_u64 *re, lds_u32 *s_bad) {
+    const uint32_t z = cv.z, zw = cv.zw, nthr = z;
+    uint64_t bad = 0;
+    if (z >= 64) {
+        for (uint32_t u = threadIdx.x; u < cv.mb * zw; u += nthr) {
+            const uint32_t br = u / zw, w = u - br * zw;
+            uint64_t acc = 0;
+            for (uint32_t e = rp[br]; e < rp[br + 1]; ++e) {
+                const uint64_t r = re[e];  // block-column | shift << 32
+                acc ^= cyc64_lds(sh + (uint32_t)r * zw, z, zw, wrap_add(64u * w, (uint32_t)(r >> 32), z));
+            }
+            if (w == zw - 1 && (z & 63u)) acc &= (1ull << (z & 63u)) - 1;
+            bad |= acc;
+        }
+    } else {
+        for (uint32_t c = threadIdx.x; c < cv.m; c += nthr) {
+            const uint32_t br = c / z, i = c - br * z;
+            uint32_t par = 0;
+            for (uint32_t e = rp[br]; e < rp[br + 1]; ++e) {
+                const uint64_t r = re[e];
+                par ^= (uint32_t)(sh[(uint32_t)r * zw] >> wrap_add(i, (uint32_t)(r >> 32), z)) & 1u;
+            }
+            bad |= par;
+        }
+    }
+    const uint64_t any = __ballot(bad != 0);
+    if ((threadIdx.x & 63u) == 0) s_bad[threadIdx.x >> 6] = any != 0;
+    lds_barrier();
+    uint32_t r = 0;
+    for (uint32_t w = 0; w < zw; ++w) r |= s_bad[w];
+    return uni(r);  // the same in every lane: keeps the iteration state scalar
+}
+
+template <int DH, int P>
+__global__ void __launch_bounds__(512) lnms_stair_kernel(LayeredArgs a) {
+    extern __shared__ __align__(16) uint64_t smem[];
+    __shared__ uint32_t s_f;
+    __shared__ uint32_t s_bad_arr[8];
+    const CodeView &cv = a.cv;
+    const uint32_t kb = cv.nb - cv.mb, kw = (cv.k + 63) / 64;
+    lds_u32 *s_tab = (lds_u32 *)smem;  // [nbe][4], 16-byte records
+    lds_u64 *s_hb = (lds_u64 *)(s_tab + 4 * cv.nbe);
+    lds_u64 *s_re = s_hb + cv.nb * cv.zw;  // row-order edges: block-column | shift << 32
+    lds_float *s_bel = (lds_float *)(s_re + cv.nbe);
+    lds_u32 *s_rp = (lds_u32 *)(s_bel + kb * cv.z);
+    lds_u32 *s_bad = (lds_u32 *)s_bad_arr;
+    const uint32_t i = threadIdx.x;  // < z: the block has z threads
+    const uint32_t bel0 = (uint32_t)(size_t)s_bel;  // LDS byte address of the beliefs
+    for (uint32_t e = i; e < cv.nbe; e += cv.z) {
+        const uint2 r = cv.row_edge[e];
+        s_re[e] = (uint64_t)r.x | ((uint64_t)r.y << 32);
+        s_tab[4 * e] = r.x < kb ? bel0 + 4u * r.x * cv.z : bel0;
+        s_tab[4 * e + 1] = r.x < kb ? r.y : 0u;
+    }
+    for (uint32_t r = i; r <= cv.mb; r += cv.z) s_rp[r] = cv.row_ptr[r];
+    StairFrame S;
+    S.hb = s_hb;
+    S.tab = s_tab;
+    S.kb = kb;
+    S.i = i;
+    // row-degree mask in scalar registers (an indexed kernel-argument read
+    // would be a scalar memory load inside the layer loop)
+    const uint64_t dm0 = a.stair_deg[0], dm1 = a.stair_deg[1], dm2 = a.stair_deg[2], dm3 = a.stair_deg[3];
+    for (;;) {
+        __syncthreads();  // the previous frame is done with s_f and the LDS
+        if (threadIdx.x == 0) s_f = atomicAdd(a.ctr, 1u);
+        __syncthreads();
+        const uint32_t f = uni(s_f);
+        if (f >= a.F) break;
+        S.gb = a.belief + (size_t)f * cv.nb * cv.zp;
+        S.st = reinterpret_cast<uint3 *>(a.state) + (size_t)f * cv.mb * cv.zp;
+        S.llr = a.llr + (size_t)f * cv.n;
+        uint3 sst[P];
+        float sbo[P];
+#pragma unroll
+        for (int u = 0; u < P; ++u) {  // layers 0..P-1 of iteration 1 read the LLRs
+            sbo[u] = S.llr[(kb + u) * cv.z + i];
+            sst[u] = make_uint3(0, 0, 0);
+        }
+        for (uint32_t c = 0; c < kb; ++c)  // information beliefs start at the channel LLRs
+            s_bel[c * cv.z + i] = S.llr[c * cv.z + i];
+        for (uint32_t e = i; e < cv.nbe; e += cv.z) {  // iteration 1's weights
+            s_tab[4 * e + 2] = __float_as_uint(a.w[e]);
+            s_tab[4 * e + 3] = 0u;
+        }
+        // wait for every load above: on entry to the layer loop nothing is
+        // outstanding, so the compiler's waits for the slots inside it count
+        // the loop's own memory operations only (vmcnt(0), other counters
+        // unrestricted)
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __syncthreads();
+        float carry = 0.0f;
+        int conv = 0;
+        uint32_t t = 1, br = 0, e0 = 0;  // br: first block-row of the group, e0: its first edge
+        for (;;) {
+            // the group's slots come back P layers on: the same block-rows of
+            // the next group, or of the next iteration after the last group
+            const bool wrap = br + P == cv.mb;
+            const uint32_t tp = t + (wrap ? 1u : 0u), bq = wrap ? 0u : br + P;
+            const float *src = tp == 1 ? S.llr + (S.kb + bq) * cv.z : S.gb + (S.kb + bq) * cv.zp;
+            const bool t2 = t >= 2;
+#pragma unroll
+            for (int u = 0; u < P; ++u) {
+                const uint32_t r = br + u;
+                const uint64_t mw = r < 64 ? dm0 : r < 128 ? dm1 : r < 192 ? dm2 : dm3;
+                const bool hi = (mw >> (r & 63u)) & 1ull;
+                if (hi)
+                    stair_layer<DH>(a, S, r, e0, t2, sst[u], sbo[u], carry);
+                else
+                    stair_layer<DH - 1>(a, S, r, e0, t2, sst[u], sbo[u], carry);
+                e0 += hi ? DH : DH - 1;
+                // loaded whether or not it will be used (past iteration T; the
+                // state in iteration 1): in bounds, one definition per slot
+                sbo[u] = src[u * (tp == 1 ? cv.z : cv.zp) + i];
+                sst[u] = S.st[(bq + u) * cv.zp + i];
+                lds_barrier();
+            }
+            br += P;
+            if (br == cv.mb) {  // end of iteration t
+                const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+                for (uint32_t s = wave; s < kb * cv.zw; s += cv.zw) {
+                    const uint32_t c = s / cv.zw, j = (s - c * cv.zw) * 64u + lane;
+                    const uint64_t m = __ballot(j < cv.z && s_bel[c * cv.z + (j < cv.z ? j : 0)] < 0.0f);
+                    if (lane == 0) s_hb[s] = m;
+                }
+                lds_barrier();
+                if (!stair_unsat(cv, s_hb, s_rp, s_re, s_bad)) {
+                    conv = 1;
+                    break;
+                }
+                if (t == a.T) break;
+                ++t;
+                br = 0;
+                e0 = 0;
+                // weights: those of t-1 move to the previous slot, iteration
+                // t's come in (each thread its own records; the last reads
+                // were before the barriers above)
+                for (uint32_t e = i; e < cv.nbe; e += cv.z) {
+                    s_tab[4 * e + 3] = s_tab[4 * e + 2];
+                    s_tab[4 * e + 2] = __float_as_uint(a.w[(size_t)(t - 1) * cv.nbe + e]);
+                }
+                lds_barrier();
+            }
+        }
+        const uint32_t its = conv ? t : a.T;
+        if (threadIdx.x == 0) atomicAdd(a.ctr + 1, its);
+        if (conv || !a.fallback) {
+            frame_message(cv, (const uint64_t *)s_hb, a.msg + (size_t)f * kw);
+            if (threadIdx.x == 0) {
+                a.iters[f] = its;
+                a.conv[f] = (uint8_t)conv;
+                a.synd[f] = (uint8_t)conv;
+                if (a.fb) a.fb[f] = 0;
+            }
+        } else {
+            // hand the frame to the plain continuation: the information
+            // beliefs to the frame's global rows (parity beliefs and check
+            // states are there), the frame on the running list
+            for (uint32_t c = 0; c < kb; ++c)
+                S.gb[c * cv.zp + i] = s_bel[c * cv.z + i];
+            if (threadIdx.x == 0) a.list[atomicAdd(a.count, 1u)] = f;
+        }
+    }
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -377,6 +705,82 @@ hipError_t launch_layered_fwd(const LayeredArgs &a, bool tape, int num_cus, hipS
         if (tape) GF2L_LFWD(27, true); else GF2L_LFWD(27, false);
     }
 #undef GF2L_LFWD
+    return hipGetLastError();
+}
+
+size_t layered_stair_bytes(const CodeView &cv) {
+    return (size_t)cv.nbe * 16 + (size_t)cv.nb * cv.zw * sizeof(uint64_t) + (size_t)cv.nbe * sizeof(uint2) +
+           (size_t)(cv.nb - cv.mb) * cv.z * sizeof(float) + (size_t)(cv.mb + 1) * sizeof(uint32_t);
+}
+
+// Load-ahead distance of the staircase kernel for mb block-rows: the largest
+// instantiated divisor of mb with two groups or more; 0 if none.
+uint32_t stair_ahead(uint32_t mb) {
+    for (uint32_t p : {10u, 6u})
+        if (mb % p == 0 && mb >= 2 * p) return p;
+    return 0;
+}
+
+// Highest row degree of the staircase kernel's instantiations; 0 when the
+// code does not have the structure it needs (block staircase of unshifted
+// circulants as the last edges of each block-row, information edges before
+// them, row degrees dh-1 and dh only, z <= 512, mb <= 256 a multiple of a
+// load-ahead distance with two groups or more).
+uint32_t layered_stair_degree(const gf2l_code *code, uint64_t *deg_mask) {
+    if (code->is_csr) return 0;
+    const CodeView &cv = code->view;
+    if (cv.wrap_bc != 0xffffffffu || cv.z > 512 || cv.nb <= cv.mb || cv.mb > 256 || !stair_ahead(cv.mb)) return 0;
+    const uint32_t kb = cv.nb - cv.mb;
+    uint32_t dmin = ~0u, dmax = 0;
+    for (uint32_t br = 0; br < cv.mb; ++br) {
+        const uint32_t e0 = code->h_row_ptr[br], d = code->h_row_ptr[br + 1] - e0;
+        if (d < 2) return 0;
+        const uint2 last = code->h_row_edge[e0 + d - 1];
+        if (last.x != kb + br || last.y != 0) return 0;
+        const uint32_t nin = br > 0 ? 2 : 1;
+        if (br > 0) {
+            const uint2 prev = code->h_row_edge[e0 + d - 2];
+            if (prev.x != kb + br - 1 || prev.y != 0) return 0;
+        }
+        for (uint32_t p = 0; p + nin < d; ++p)
+            if (code->h_row_edge[e0 + p].x >= kb) return 0;
+        dmin = std::min(dmin, d);
+        dmax = std::max(dmax, d);
+    }
+    if (dmin + 1 < dmax || (dmax != 6 && dmax != 7)) return 0;
+    if (deg_mask) {
+        for (int q = 0; q < 4; ++q) deg_mask[q] = 0;
+        for (uint32_t br = 0; br < cv.mb; ++br)
+            if (code->h_row_ptr[br + 1] - code->h_row_ptr[br] == dmax) deg_mask[br >> 6] |= 1ull << (br & 63u);
+    }
+    return dmax;
+}
+
+hipError_t launch_layered_stair(const LayeredArgs &a, uint32_t dh, int num_cus, hipStream_t s) {
+    const size_t bytes = layered_stair_bytes(a.cv);
+    const uint32_t block = a.cv.z;  // one thread per check of a block-row
+    const uint32_t per_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / std::max<size_t>(bytes, 1)));
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(a.F, (uint32_t)num_cus * per_cu));
+    const uint32_t p = stair_ahead(a.cv.mb);
+    if (a.cv.z == 0 || a.cv.z > 512 || !p) return hipErrorInvalidValue;
+#define GF2L_LSTAIR(DH, P)                                                              \
+    do {                                                                                \
+        auto k = lnms_stair_kernel<DH, P>;                                              \
+        hipError_t e = allow_lds(k, bytes);                                             \
+        if (e != hipSuccess) return e;                                                  \
+        hipLaunchKernelGGL(k, dim3(grid), dim3(block), bytes, s, a);                    \
+    } while (0)
+    if (dh == 6 && p == 6)
+        GF2L_LSTAIR(6, 6);
+    else if (dh == 6 && p == 10)
+        GF2L_LSTAIR(6, 10);
+    else if (dh == 7 && p == 6)
+        GF2L_LSTAIR(7, 6);
+    else if (dh == 7 && p == 10)
+        GF2L_LSTAIR(7, 10);
+    else
+        return hipErrorInvalidValue;
+#undef GF2L_LSTAIR
     return hipGetLastError();
 }
 

@@ -324,6 +324,24 @@ uint32_t layered_x(const gf2l_ctx *ctx, const gf2l_nms *q) {
     return std::min<uint32_t>(q->X, layered_lds_columns(q->code->view, ctx->lds_limit / 3));
 }
 
+// Kernel of a layered decode: the staircase kernel's row degree, 0 for the
+// general kernel, -1 when GF2L_TUNE_LAYERED_KERNEL asks for the staircase
+// kernel and the code (or the LDS) does not allow it.  Automatic choice: the
+// staircase kernel unless a column count was set for the general one.
+int layered_stair(const gf2l_ctx *ctx, const gf2l_nms *q) {
+    const int k = ctx->tune.layered_kernel;
+    if (k == 1 || (k == 0 && ctx->tune.layered_cols > 0)) return 0;
+    const uint32_t dh = layered_stair_degree(q->code);
+    if (dh && layered_stair_bytes(q->code->view) <= ctx->lds_limit) return (int)dh;
+    return k == 2 ? -1 : 0;
+}
+
+// Block-columns of beliefs a layered decode keeps in LDS.
+uint32_t layered_decode_x(const gf2l_ctx *ctx, const gf2l_nms *q) {
+    const CodeView &cv = q->code->view;
+    return layered_stair(ctx, q) > 0 ? cv.nb - cv.mb : layered_x(ctx, q);
+}
+
 // Layered schedule: training forward (tape, all T iterations) and backward;
 // leaves the summed gradient in q->grad and the summed loss in q->loss.
 int layered_forward_backward(gf2l_ctx *ctx, gf2l_nms *q, const float *d_llr, const uint64_t *d_cw, size_t frames) {
@@ -383,6 +401,9 @@ __global__ void clear_words_kernel(uint32_t *p, uint32_t n) {
 int layered_decode(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr, uint32_t F, uint32_t fb_it,
                    uint64_t *d_msg, uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb) {
     const CodeView &cv = q->code->view;
+    const int dh = layered_stair(ctx, q);
+    if (dh < 0)
+        return fail(ctx, GF2L_ERR_UNSUPPORTED, "the staircase layered kernel does not apply to this code");
     int rc = ensure_buffers(ctx, F, 0, 3ull * cv.mb * cv.zp * F, (size_t)cv.nb * cv.zp * F,
                             (size_t)cv.nb * cv.zw * F, 0);
     if (rc != GF2L_OK) return rc;
@@ -398,7 +419,8 @@ int layered_decode(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr, uint32_
     a.w = q->d_w;
     a.F = F;
     a.T = q->T;
-    a.X = layered_x(ctx, q);
+    a.X = dh > 0 ? cv.nb - cv.mb : layered_x(ctx, q);
+    if (dh > 0) layered_stair_degree(q->code, a.stair_deg);
     a.fallback = fb_it > 0;
     a.state = ws.state;
     a.belief = ws.belief;
@@ -411,7 +433,10 @@ int layered_decode(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr, uint32_
     a.synd = d_synd;
     a.fb = d_fb;
     if (pf.on) GF2L_HIP(ctx, hipEventRecord(pf.lev[0], s));
-    GF2L_HIP(ctx, launch_layered_fwd(a, false, ctx->num_cus, s));
+    if (dh > 0)
+        GF2L_HIP(ctx, launch_layered_stair(a, (uint32_t)dh, ctx->num_cus, s));
+    else
+        GF2L_HIP(ctx, launch_layered_fwd(a, false, ctx->num_cus, s));
     if (pf.on) GF2L_HIP(ctx, hipEventRecord(pf.lev[1], s));
     // frames handed over, and the layered frame-iterations run
     GF2L_HIP(ctx, hipMemcpyAsync(ws.h_count, ws.counts, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -596,7 +621,7 @@ int gf2l_nms_create_layered(gf2l_ctx *ctx, const gf2l_code *code, uint32_t itera
 
 int gf2l_nms_layered_columns(gf2l_ctx *ctx, const gf2l_nms *q, uint32_t *columns) {
     if (!ctx || !q || !columns) return GF2L_ERR_INVALID;
-    *columns = q->layered ? gf2l::layered_x(ctx, q) : 0;
+    *columns = q->layered ? gf2l::layered_decode_x(ctx, q) : 0;
     return GF2L_OK;
 }
 

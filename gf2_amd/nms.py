@@ -65,6 +65,15 @@ class NmsDecoder:
         return self._h
 
     @property
+    def layered_columns(self) -> int:
+        """Block-columns of beliefs a layered decode keeps in LDS under the
+        context's current tuning (the information columns when the staircase
+        kernel runs); 0 for a flooding decoder."""
+        x = ctypes.c_uint32()
+        _lib.check(self.ctx.handle, _lib.load().gf2l_nms_layered_columns(self.ctx.handle, self._h, ctypes.byref(x)))
+        return x.value
+
+    @property
     def fallback(self) -> str:
         return self._fallback
 

@@ -113,14 +113,28 @@ int gf2l_set_path(gf2l_ctx *ctx, int path);
  *                        kernels takes per shared-counter fetch (1..64);
  *                        0: fixed-stride item assignment; -1 (default):
  *                        8 when every variable degree is at most 8, else 0
- *   GF2L_TUNE_LAYERED_COLUMNS  block-columns of beliefs the layered learned
- *                        decoder keeps in LDS per workgroup (the rest in
- *                        global memory); 0 (default): as many as fit */
+ *   GF2L_TUNE_LAYERED_COLUMNS  block-columns of beliefs the general layered
+ *                        kernel keeps in LDS per workgroup (the rest in
+ *                        global memory); 0 (default): as many as fit in a
+ *                        third of the LDS (three frames per compute unit).
+ *                        A value above 0 also selects the general kernel
+ *                        when GF2L_TUNE_LAYERED_KERNEL is 0.
+ *   GF2L_TUNE_LAYERED_KERNEL  kernel of the layered learned decoder:
+ *                        0 (default): the staircase kernel where the code
+ *                        allows it (QC-IRA code whose parity part is a block
+ *                        staircase of unshifted circulants, z <= 512, at
+ *                        least 10 block-rows, row degrees 5..7 in two adjacent
+ *                        values, the information beliefs of one frame in
+ *                        LDS), else the general kernel; 1: the general
+ *                        kernel; 2: the staircase kernel (decoding fails
+ *                        with GF2L_ERR_UNSUPPORTED where it does not apply).
+ *                        Training always runs the general kernel. */
 #define GF2L_TUNE_WG_PER_CU 1
 #define GF2L_TUNE_XCD_MAP 2
 #define GF2L_TUNE_GRAPH_ITERS 3
 #define GF2L_TUNE_CHUNK 4
 #define GF2L_TUNE_LAYERED_COLUMNS 5
+#define GF2L_TUNE_LAYERED_KERNEL 6
 int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value);
 
 /* ---- codes ------------------------------------------------------------ */

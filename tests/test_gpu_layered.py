@@ -46,9 +46,15 @@ def same(got, ref, what):
         assert np.array_equal(a, b), f"{what}: {name} differs at {np.nonzero((a != b).reshape(len(a), -1).any(1))[0][:8]}"
 
 
-@pytest.mark.parametrize("which,cols", [("n2304", 0), ("n2304", 5), ("n64800", 0), ("n64800", 37)])
+# kernel: the automatic choice (the staircase kernel on both codes: QC-IRA
+# staircases of unshifted circulants, mb = 12 and 90) or the general kernel,
+# with the automatic column count or a forced small one
+@pytest.mark.parametrize("which,cols,kernel", [("n2304", 0, _lib.LAYERED_AUTO), ("n2304", 0, _lib.LAYERED_GENERAL),
+                                               ("n2304", 5, _lib.LAYERED_AUTO), ("n64800", 0, _lib.LAYERED_AUTO),
+                                               ("n64800", 0, _lib.LAYERED_GENERAL),
+                                               ("n64800", 37, _lib.LAYERED_AUTO)])
 @pytest.mark.parametrize("fb_it", [0, 30])
-def test_layered_decode_matches_oracle(ctx, which, cols, fb_it):
+def test_layered_decode_matches_oracle(ctx, which, cols, kernel, fb_it):
     spec = codes.n2304_r12() if which == "n2304" else codes.n64800_r12()
     base = spec.base_matrix()
     nbe = int((base >= 0).sum())
@@ -59,16 +65,47 @@ def test_layered_decode_matches_oracle(ctx, which, cols, fb_it):
     F = 256 if which == "n2304" else 32
     _, _, llr = frames.random_frames(base, spec.z, F, 1.7 if which == "n2304" else 1.75, seed=52)
     ctx.set_tuning(_lib.TUNE_LAYERED_COLUMNS, cols)
-    say("layered", which, "lds columns", cols or "max", "fallback", fb_it)
+    ctx.set_tuning(_lib.TUNE_LAYERED_KERNEL, kernel)
     dec = NmsDecoder(LdpcCode(spec.qc(), ctx), T, weights=w, schedule="layered")
+    x = dec.layered_columns
+    kb = spec.n // spec.z - (spec.n - spec.k) // spec.z
+    stair = kernel == _lib.LAYERED_AUTO and cols == 0
+    assert (x == kb) if stair else x == (cols or x) > 0, x
+    say("layered", which, "lds columns", x, "kernel", "staircase" if stair else "general", "fallback", fb_it)
     got = dec.decode(llr, fallback_max_it=fb_it)
     ctx.set_tuning(_lib.TUNE_LAYERED_COLUMNS, 0)
+    ctx.set_tuning(_lib.TUNE_LAYERED_KERNEL, _lib.LAYERED_AUTO)
     ref = orc.layered_decode_batch(base, spec.z, llr, w, fb_it, threads=8)
     same(got, ref, f"{which} cols {cols} fb {fb_it}")
     conv, fb = ref[2], ref[4]
     assert (conv & ~fb).any(), "some frames converge in the layered phase"
     if fb_it:
         assert fb.any(), "some frames run the continuation"
+
+
+def test_staircase_kernel_where_it_does_not_apply(ctx):
+    """Asking for the staircase kernel on codes without its structure fails
+    with GF2L_ERR_UNSUPPORTED; the automatic choice runs the general kernel
+    there (bit-exact against the oracle)."""
+    two = {(s.n, s.k): s for s in codes.mixed_twelve()}
+    # 8 block-rows (no load-ahead distance divides them with two groups);
+    # 108 information columns (more than one workgroup's LDS)
+    for spec in (two[(16200, 13320)], two[(64800, 38880)]):
+        base = spec.base_matrix()
+        nbe = int((base >= 0).sum())
+        w = np.full((4, nbe), 0.8, dtype=np.float32)
+        _, _, llr = frames.random_frames(base, spec.z, 4, 3.5, seed=57)
+        dec = NmsDecoder(LdpcCode(spec.qc(), ctx), 4, weights=w, schedule="layered")
+        ctx.set_tuning(_lib.TUNE_LAYERED_KERNEL, _lib.LAYERED_STAIRCASE)
+        say("staircase kernel forced on", spec.name)
+        try:
+            with pytest.raises(_lib.Gf2lError) as e:
+                dec.decode(llr, fallback_max_it=5)
+            assert e.value.code == _lib.GF2L_ERR_UNSUPPORTED
+        finally:
+            ctx.set_tuning(_lib.TUNE_LAYERED_KERNEL, _lib.LAYERED_AUTO)
+        say("automatic kernel on", spec.name, "lds columns", dec.layered_columns)
+        same(dec.decode(llr, fallback_max_it=5), orc.layered_decode_batch(base, spec.z, llr, w, 5), spec.name)
 
 
 def test_layered_signed_zero_and_infinite_llrs(ctx):
