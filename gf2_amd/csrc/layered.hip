@@ -256,12 +256,13 @@ __device__ __forceinline__ void lds_st(uint32_t addr, float v) { *(lds_float *)(
 // x * (-1)^s for s in {0, 1}: a sign flip, exact
 __device__ __forceinline__ float flip(float x, uint32_t s) { return __uint_as_float(__float_as_uint(x) ^ (s << 31)); }
 
-// Layer br (edges e0..e0+D-1) of iteration t (t2: t >= 2).  (o, bo): this
-// layer's check states and belief of column kb+br, loaded ahead.  Code
-// tables and weights come from LDS: no scalar memory loads in the layer loop
-// (their counter would also wait for the LDS accesses).  In iteration 1 the
-// old message is rebuilt from a zero state with weight 0: x = b - (+0), which
-// is b exactly (also for -0 and infinities).
+// Layer br (edges e0..e0+D-1) of iteration t (t2: t >= 2; cin: br > 0, the
+// caller's constant where it knows it).  (o, bo): this layer's check states
+// and belief of column kb+br, loaded ahead.  Code tables and weights come
+// from LDS: no scalar memory loads in the layer loop (their counter would
+// also wait for the LDS accesses).  In iteration 1 the old message is rebuilt
+// from a zero state with weight 0 (the table's previous weights are zero
+// then): x = b - (+0), which is b exactly (also for -0 and infinities).
 //
 // The check rule is the general kernel's (CheckAcc) specialised to D <= 8:
 // with fewer than nine inputs every input of the reference's min-sum falls in
@@ -270,10 +271,9 @@ __device__ __forceinline__ float flip(float x, uint32_t s) { return __uint_as_fl
 // w * (+-m) is computed as +-(w * m) (IEEE multiplication: the same value).
 template <int D>
 __device__ __forceinline__ void stair_layer(const LayeredArgs &a, const StairFrame &S, uint32_t br, uint32_t e0,
-                                            bool t2, const uint3 o, const float bo, float &carry) {
+                                            bool t2, bool cin, const uint3 o, const float bo, float &carry) {
     static_assert(D >= 2 && D <= 8, "the staircase layer's sign rule needs 2 <= D <= 8");
     const CodeView &cv = a.cv;
-    const bool cin = br > 0;  // column kb+br-1 at position D-2
     uint32_t cb[D], sh[D];
     float wt[D], wp[D];
 #pragma unroll
@@ -282,8 +282,7 @@ __device__ __forceinline__ void stair_layer(const LayeredArgs &a, const StairFra
         cb[p] = r[0];
         sh[p] = r[1];
         wt[p] = __uint_as_float(r[2]);
-        const float wpv = __uint_as_float(r[3]);  // read unconditionally (stale in iteration 1)
-        wp[p] = t2 ? wpv : 0.0f;
+        wp[p] = __uint_as_float(r[3]);
     }
     const float m1o = t2 ? __uint_as_float(o.x) : 0.0f, m2o = t2 ? __uint_as_float(o.y) : 0.0f;
     const uint32_t mo = t2 ? o.z : 0u, io = mo & 31u;
@@ -326,7 +325,7 @@ __device__ __forceinline__ void stair_layer(const LayeredArgs &a, const StairFra
         const float v = x[p] + flip(wt[p] * mag, (sg >> p) & 1u);
         if (p == D - 1) {
             carry = v;  // column kb+br: to layer br+1, final now for the last block-row
-            S.gb[(S.kb + br) * cv.zp + S.i] = v;
+            stu(S.gb + (S.kb + br) * cv.zp, S.i, v);
             if (br == cv.mb - 1) {
                 const uint64_t m = __ballot(v < 0.0f);
                 if (lane == 0) S.hb[(cv.nb - 1) * cv.zw + wave] = m;
@@ -335,7 +334,7 @@ __device__ __forceinline__ void stair_layer(const LayeredArgs &a, const StairFra
             // column kb+br-1, final for this iteration (block-row 0: an
             // information column; the global row of column kb-1 takes the
             // store, unused until hand-over rewrites it)
-            S.gb[(S.kb + br - 1) * cv.zp + S.i] = v;
+            stu(S.gb + (S.kb + br - 1) * cv.zp, S.i, v);
             if (cin) {
                 const uint64_t m = __ballot(v < 0.0f);
                 if (lane == 0) S.hb[(S.kb + br - 1) * cv.zw + wave] = m;
@@ -346,7 +345,7 @@ __device__ __forceinline__ void stair_layer(const LayeredArgs &a, const StairFra
             lds_st(ad[p], v);
         }
     }
-    S.st[br * cv.zp + S.i] = make_uint3(__float_as_uint(m1), __float_as_uint(m2), meta);
+    stu(S.st + br * cv.zp, S.i, make_uint3(__float_as_uint(m1), __float_as_uint(m2), meta));
 }
 
 // Parity checks over the hard-decision words, the code's edge tables read
@@ -464,21 +463,24 @@ __global__ void __launch_bounds__(512) lnms_stair_kernel(LayeredArgs a) {
             const bool wrap = br + P == cv.mb;
             const uint32_t tp = t + (wrap ? 1u : 0u), bq = wrap ? 0u : br + P;
             const float *src = tp == 1 ? S.llr + (S.kb + bq) * cv.z : S.gb + (S.kb + bq) * cv.zp;
+            const uint32_t src_row = tp == 1 ? cv.z : cv.zp;
             const bool t2 = t >= 2;
 #pragma unroll
             for (int u = 0; u < P; ++u) {
                 const uint32_t r = br + u;
                 const uint64_t mw = r < 64 ? dm0 : r < 128 ? dm1 : r < 192 ? dm2 : dm3;
                 const bool hi = (mw >> (r & 63u)) & 1ull;
+                const bool cin = u > 0 || br > 0;  // a constant in all but the first copy
                 if (hi)
-                    stair_layer<DH>(a, S, r, e0, t2, sst[u], sbo[u], carry);
+                    stair_layer<DH>(a, S, r, e0, t2, cin, sst[u], sbo[u], carry);
                 else
-                    stair_layer<DH - 1>(a, S, r, e0, t2, sst[u], sbo[u], carry);
+                    stair_layer<DH - 1>(a, S, r, e0, t2, cin, sst[u], sbo[u], carry);
                 e0 += hi ? DH : DH - 1;
                 // loaded whether or not it will be used (past iteration T; the
-                // state in iteration 1): in bounds, one definition per slot
-                sbo[u] = src[u * (tp == 1 ? cv.z : cv.zp) + i];
-                sst[u] = S.st[(bq + u) * cv.zp + i];
+                // state in iteration 1): in bounds, one definition per slot;
+                // wave-uniform row bases, so the loads take the scalar-base form
+                sbo[u] = ldu(src + u * src_row, i);
+                sst[u] = ldu(S.st + (bq + u) * cv.zp, i);
                 lds_barrier();
             }
             br += P;
