@@ -141,6 +141,7 @@ struct LayeredArgs {
     uint32_t *tape_state;    // training: [T][F][mb][zp] uint4 (min1, min2, meta, idx2 | argmin input signs)
     float *tape_bel;         // training: [T][F][nb][zp] beliefs after each iteration
     uint64_t stair_deg[4];   // staircase kernel: bit r set when block-row r has the higher of its two degrees
+    uint64_t stair_sync[4];  // staircase kernel: bit r set when a barrier must follow block-row r
 };
 
 // Arguments of the layered backward kernel (layered.hip).
@@ -292,10 +293,10 @@ hipError_t launch_layered_fwd(const LayeredArgs &a, bool tape, int num_cus, hipS
 hipError_t launch_layered_bwd(const LayeredBwdArgs &a, int num_cus, hipStream_t s);
 // Staircase kernel (decoding only, X = kb information columns in LDS): its
 // LDS bytes, the row degree of its instantiation for a code (0: the code
-// lacks the structure; deg_mask, when given, receives the block-rows of that
-// degree as LayeredArgs::stair_deg) and its launcher.
+// lacks the structure; deg_mask and sync_mask, when given, receive
+// LayeredArgs::stair_deg and stair_sync) and its launcher.
 size_t layered_stair_bytes(const CodeView &cv);
-uint32_t layered_stair_degree(const gf2l_code *code, uint64_t *deg_mask = nullptr);
+uint32_t layered_stair_degree(const gf2l_code *code, uint64_t *deg_mask = nullptr, uint64_t *sync_mask = nullptr);
 hipError_t launch_layered_stair(const LayeredArgs &a, uint32_t dh, int num_cus, hipStream_t s);
 
 // Plain min-sum on a general sparse code (csr.hip).

@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <type_traits>
+#include <vector>
 
 #include "device_common.h"
 
@@ -423,9 +424,10 @@ __global__ void __launch_bounds__(512) lnms_stair_kernel(LayeredArgs a) {
     S.tab = s_tab;
     S.kb = kb;
     S.i = i;
-    // row-degree mask in scalar registers (an indexed kernel-argument read
-    // would be a scalar memory load inside the layer loop)
+    // row-degree and barrier masks in scalar registers (an indexed
+    // kernel-argument read would be a scalar memory load inside the layer loop)
     const uint64_t dm0 = a.stair_deg[0], dm1 = a.stair_deg[1], dm2 = a.stair_deg[2], dm3 = a.stair_deg[3];
+    const uint64_t bm0 = a.stair_sync[0], bm1 = a.stair_sync[1], bm2 = a.stair_sync[2], bm3 = a.stair_sync[3];
     for (;;) {
         __syncthreads();  // the previous frame is done with s_f and the LDS
         if (threadIdx.x == 0) s_f = atomicAdd(a.ctr, 1u);
@@ -469,6 +471,7 @@ __global__ void __launch_bounds__(512) lnms_stair_kernel(LayeredArgs a) {
             for (int u = 0; u < P; ++u) {
                 const uint32_t r = br + u;
                 const uint64_t mw = r < 64 ? dm0 : r < 128 ? dm1 : r < 192 ? dm2 : dm3;
+                const uint64_t sw = r < 64 ? bm0 : r < 128 ? bm1 : r < 192 ? bm2 : bm3;
                 const bool hi = (mw >> (r & 63u)) & 1ull;
                 const bool cin = u > 0 || br > 0;  // a constant in all but the first copy
                 if (hi)
@@ -481,7 +484,9 @@ __global__ void __launch_bounds__(512) lnms_stair_kernel(LayeredArgs a) {
                 // wave-uniform row bases, so the loads take the scalar-base form
                 sbo[u] = ldu(src + u * src_row, i);
                 sst[u] = ldu(S.st + (bq + u) * cv.zp, i);
-                lds_barrier();
+                // block-rows without a common information column need no
+                // barrier between them (parity beliefs and states are private)
+                if ((sw >> (r & 63u)) & 1ull) lds_barrier();
             }
             br += P;
             if (br == cv.mb) {  // end of iteration t
@@ -728,7 +733,7 @@ uint32_t stair_ahead(uint32_t mb) {
 // circulants as the last edges of each block-row, information edges before
 // them, row degrees dh-1 and dh only, z <= 512, mb <= 256 a multiple of a
 // load-ahead distance with two groups or more).
-uint32_t layered_stair_degree(const gf2l_code *code, uint64_t *deg_mask) {
+uint32_t layered_stair_degree(const gf2l_code *code, uint64_t *deg_mask, uint64_t *sync_mask) {
     if (code->is_csr) return 0;
     const CodeView &cv = code->view;
     if (cv.wrap_bc != 0xffffffffu || cv.z > 512 || cv.nb <= cv.mb || cv.mb > 256 || !stair_ahead(cv.mb)) return 0;
@@ -754,6 +759,30 @@ uint32_t layered_stair_degree(const gf2l_code *code, uint64_t *deg_mask) {
         for (int q = 0; q < 4; ++q) deg_mask[q] = 0;
         for (uint32_t br = 0; br < cv.mb; ++br)
             if (code->h_row_ptr[br + 1] - code->h_row_ptr[br] == dmax) deg_mask[br >> 6] |= 1ull << (br & 63u);
+    }
+    if (sync_mask) {
+        // a barrier after block-row r when block-row r+1 touches an
+        // information column (in LDS, read and written by other threads) that
+        // a block-row since the last barrier touched; always after the last
+        // block-row (the hard decisions read them all)
+        for (int q = 0; q < 4; ++q) sync_mask[q] = 0;
+        std::vector<uint8_t> since(kb, 0);  // columns touched since the last barrier
+        auto mark = [&](uint32_t br) {
+            for (uint32_t e = code->h_row_ptr[br]; e < code->h_row_ptr[br + 1]; ++e)
+                if (code->h_row_edge[e].x < kb) since[code->h_row_edge[e].x] = 1;
+        };
+        mark(0);
+        for (uint32_t br = 0; br < cv.mb; ++br) {
+            bool need = br + 1 == cv.mb;
+            if (!need)
+                for (uint32_t e = code->h_row_ptr[br + 1]; e < code->h_row_ptr[br + 2]; ++e)
+                    if (code->h_row_edge[e].x < kb && since[code->h_row_edge[e].x]) need = true;
+            if (need) {
+                sync_mask[br >> 6] |= 1ull << (br & 63u);
+                std::fill(since.begin(), since.end(), 0);
+            }
+            if (br + 1 < cv.mb) mark(br + 1);
+        }
     }
     return dmax;
 }

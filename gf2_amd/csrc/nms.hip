@@ -420,7 +420,7 @@ int layered_decode(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr, uint32_
     a.F = F;
     a.T = q->T;
     a.X = dh > 0 ? cv.nb - cv.mb : layered_x(ctx, q);
-    if (dh > 0) layered_stair_degree(q->code, a.stair_deg);
+    if (dh > 0) layered_stair_degree(q->code, a.stair_deg, a.stair_sync);
     a.fallback = fb_it > 0;
     a.state = ws.state;
     a.belief = ws.belief;
