@@ -52,8 +52,8 @@ LDS_PEAK_GBPS = 256 * 128 * 2.4
 # HBM bytes per frame-iteration measured with rocprofv3 FETCH_SIZE and
 # WRITE_SIZE on this workload (python bench.py --steps 1 --warmup 0 --no-cpu,
 # profiles/n64800_layered_bench.md), scaled by what the counters read for a
-# plain copy of known size (gf2_amd/hbm_probe: FETCH_SIZE counts half the
-# bytes read, WRITE_SIZE the bytes written).
+# plain copy of known size (FETCH_SIZE counts half the bytes read, WRITE_SIZE
+# the bytes written: the calibration in profiles/n64800_state_kernels_bench.md).
 MEASURED_TRAFFIC_PER_FI = {"layered": 1832398, "check": 1201488, "variable": 1477744}
 # CPU threads of one GPU's share of the box
 CPU_SHARE = 16
