@@ -54,7 +54,14 @@ LDS_PEAK_GBPS = 256 * 128 * 2.4
 # profiles/n64800_layered_bench.md), scaled by what the counters read for a
 # plain copy of known size (FETCH_SIZE counts half the bytes read, WRITE_SIZE
 # the bytes written: the calibration in profiles/n64800_state_kernels_bench.md).
-MEASURED_TRAFFIC_PER_FI = {"layered": 1832398, "check": 1201488, "variable": 1477744}
+MEASURED_TRAFFIC_PER_FI = {"layered_stair": 1065639, "layered_general": 1832398, "check": 1201488,
+                           "variable": 1477744}
+# Vector-ALU issue: 256 CUs x 4 SIMDs, one 64-lane FP32/INT32 instruction per
+# SIMD every 4 clocks (16 lanes per clock) at 2.4 GHz; wave-level VALU
+# instructions per frame-iteration measured with rocprofv3 SQ_INSTS_VALU on
+# this workload (profiles/n64800_layered_bench.md).
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4
+MEASURED_VALU_PER_FI = {"layered_stair": 112206}
 # CPU threads of one GPU's share of the box
 CPU_SHARE = 16
 DEFAULT_EBN0 = 1.62
@@ -200,29 +207,40 @@ def main():
     xc = ctypes.c_uint32()
     _lib.check(ctx.handle, lib.gf2l_nms_layered_columns(ctx.handle, learned.handle, ctypes.byref(xc)))
     X = xc.value
+    kb = nb - M // z
+    # the staircase kernel runs when every information column is in LDS and no
+    # column count was forced (GF2L_TUNE_LAYERED_KERNEL automatic)
+    stair = X == kb and not a.layered_columns
     edges_lds = int((base[:, :X] >= 0).sum()) * z
     E = int((base >= 0).sum()) * z
     # Algorithmic bytes per frame-iteration of each kernel (unique data it must move):
     #   check     read the frame's beliefs once (4N), read and write the 12-byte state of every check (24M)
     #   variable  read every check state once (12M), read L (4N), write beliefs (4N) and hard bits (N/8)
     #   syndrome  read the hard bits (N/8)
-    #   layered   HBM: read and write the 12-byte state of every check (24M), read and write the
-    #             beliefs of block-columns outside LDS once per edge (8 E_glob) and read them once
-    #             for the hard decisions (4 z (nb - X)); LDS: the same for the X columns in LDS
+    #   layered, staircase kernel: HBM: read and write the 12-byte state of every check (24M), read
+    #             and write each parity belief once (8 z (nb - kb)); LDS: read and write the
+    #             information beliefs once per edge (8 E_info) and read them once for the hard
+    #             decisions (4 z kb)
+    #   layered, general kernel: HBM: 24M, read and write the beliefs of block-columns outside LDS
+    #             once per edge (8 E_glob) and read them once for the hard decisions (4 z (nb - X));
+    #             LDS: the same for the X columns in LDS
     per_fi = {"check": 4 * cv_n + 24 * M, "variable": 12 * M + 8 * cv_n + cv_n // 8, "syndrome": cv_n // 8,
-              "layered": 24 * M + 8 * (E - edges_lds) + 4 * z * (nb - X)}
+              "layered": (24 * M + 8 * z * (nb - kb)) if stair else
+                         (24 * M + 8 * (E - edges_lds) + 4 * z * (nb - X))}
     lds_per_fi = 8 * edges_lds + 4 * z * X
     kernels = {"check": "cn_state_kernel", "variable": "vn_state_kernel", "syndrome": "syn_words_kernel",
-               "layered": "lnms_fwd_kernel"}
+               "layered": "lnms_stair_kernel" if stair else "lnms_fwd_kernel"}
+    traffic_key = {"layered": "layered_stair" if stair else "layered_general"}
 
     def roofline(prof):
         dom = max(prof, key=lambda k: prof[k][0])
         ms, launches, units = prof[dom]
         achieved = per_fi[dom] * units / (ms * 1e-3) / 1e9
+        tk = traffic_key.get(dom, dom)
         r = {"bound": "hbm", "kernel": kernels[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
              "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-             "traffic": (round(MEASURED_TRAFFIC_PER_FI[dom] * units / launches)
-                         if dom in MEASURED_TRAFFIC_PER_FI and launches else None),
+             "traffic": (round(MEASURED_TRAFFIC_PER_FI[tk] * units / launches)
+                         if tk in MEASURED_TRAFFIC_PER_FI and launches else None),
              "traffic_unit": "HBM bytes per launch (rocprofv3 bytes per frame-iteration on this workload x "
                              "frame-iterations per launch)",
              "avg_launch_us": round(ms / launches * 1e3, 2), "launches": int(launches),
@@ -233,6 +251,13 @@ def main():
             r["lds"] = {"achieved": round(lds, 1), "peak": round(LDS_PEAK_GBPS, 1), "unit": "GB/s",
                         "frac": round(lds / LDS_PEAK_GBPS, 4), "bytes_per_frame_iteration": lds_per_fi,
                         "lds_columns": X}
+            if tk in MEASURED_VALU_PER_FI:
+                # the issue rate that bounds the staircase kernel: measured
+                # instructions per frame-iteration x frame-iterations / time
+                v = MEASURED_VALU_PER_FI[tk] * units / (ms * 1e-3) / 1e9
+                r["valu"] = {"achieved": round(v, 1), "peak": round(VALU_PEAK_GINST, 1), "unit": "G wave-inst/s",
+                             "frac": round(v / VALU_PEAK_GINST, 4),
+                             "inst_per_frame_iteration": MEASURED_VALU_PER_FI[tk]}
         if dom == "variable":
             # the whole iteration against the uncompressed message model 16E + 4N
             fi = prof["variable"][2]

@@ -72,7 +72,10 @@ def count_errors(msg: np.ndarray, cw: np.ndarray, conv: np.ndarray, synd: np.nda
     diff = (msg[:, :kw] ^ cw[:, :kw]).copy()
     if k % 64:
         diff[:, -1] &= np.uint64((1 << (k % 64)) - 1)
-    wrong = np.unpackbits(diff.view(np.uint8), axis=1).sum(axis=1)
+    bad = diff.any(axis=1)
+    wrong = np.zeros(len(msg), dtype=np.int64)
+    if bad.any():  # bit counts only where a word differs
+        wrong[bad] = np.unpackbits(diff[bad].view(np.uint8), axis=1).sum(axis=1)
     err = (~conv) | (~synd) | (wrong > 0)
     return int(err.sum()), int(wrong.sum())
 
