@@ -24,6 +24,9 @@ PATH_LDS = 1
 PATH_STREAM = 2
 PATH_STATE = 3
 
+RULE_MINSUM = 0
+RULE_SPA = 1
+
 TUNE_WG_PER_CU = 1
 TUNE_XCD_MAP = 2
 TUNE_GRAPH_ITERS = 3
@@ -55,6 +58,7 @@ SIGNATURES = {
     "gf2l_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "gf2l_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
     "gf2l_set_path": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "gf2l_set_check_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "gf2l_device_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _c_void_pp]),
     "gf2l_device_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "gf2l_copy_to_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,

@@ -128,12 +128,21 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
     if (code->is_csr) {
         if (d_w || h_trace_c2v)
             return fail(ctx, GF2L_ERR_UNSUPPORTED, "weights and traces need a quasi-cyclic code");
+        if (ctx->rule != GF2L_RULE_MINSUM)
+            return fail(ctx, GF2L_ERR_UNSUPPORTED, "the sum-product rule needs a quasi-cyclic code");
         if (d_fb) GF2L_HIP(ctx, hipMemsetAsync(d_fb, 0, F, s));
         return decode_csr(ctx, code, d_llr, F, max_it, d_msg, d_iters, d_conv, d_synd);
     }
     if (cv.dvb_q && (d_w || h_trace_c2v))
         return fail(ctx, GF2L_ERR_UNSUPPORTED, "weights and traces are not supported for DVB codes");
-    const int path = resolve_path(ctx, cv);
+    int path = resolve_path(ctx, cv);
+    if (ctx->rule == GF2L_RULE_SPA) {  // the sum-product rule runs on the STREAM path only
+        if (d_w || cv.dvb_q)
+            return fail(ctx, GF2L_ERR_UNSUPPORTED, "the sum-product rule takes no weights and no DVB code");
+        if (ctx->path != gf2l::kAuto && ctx->path != gf2l::kStream)
+            return fail(ctx, GF2L_ERR_UNSUPPORTED, "the sum-product rule needs the STREAM path");
+        path = gf2l::kStream;
+    }
     if (path < 0) return fail(ctx, GF2L_ERR_UNSUPPORTED, "the forced decoder path cannot run this code");
     if (path == gf2l::kLds) {
         float *d_tc = nullptr, *d_tb = nullptr;
@@ -230,6 +239,7 @@ int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const fl
             a.w_prev = (w && it >= 2) ? w + (size_t)(it - 2) * cv.nbe : nullptr;
             a.w_cur = w ? w + (size_t)(it - 1) * cv.nbe : nullptr;
             if (cont && it == 1) a.w_prev = d_w + (size_t)(max_it - 1) * cv.nbe;
+            a.spa = ctx->rule == GF2L_RULE_SPA && path == gf2l::kStream && !w;
             a.msg = d_msg;
             a.iters = d_iters;
             a.conv = d_conv;
@@ -524,6 +534,15 @@ int gf2l_set_path(gf2l_ctx *ctx, int path) {
         if (path < GF2L_PATH_AUTO || path > GF2L_PATH_STATE)
             return fail(ctx, GF2L_ERR_INVALID, "unknown decoder path");
         ctx->path = path;
+        return GF2L_OK;
+    });
+}
+
+int gf2l_set_check_rule(gf2l_ctx *ctx, int rule) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    return gf2l::guard(ctx, [&]() -> int {
+        if (rule != GF2L_RULE_MINSUM && rule != GF2L_RULE_SPA) return fail(ctx, GF2L_ERR_INVALID, "unknown check rule");
+        ctx->rule = rule;
         return GF2L_OK;
     });
 }

@@ -126,7 +126,11 @@ vn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *
 // ---------------------------------------------------------------------------
 // STREAM path (one stored message per edge).
 
-template <int MAXD, bool NMS>
+// SPA: the sum-product rule of check_node_update_spa (core.rs:762-787) in
+// place of min-sum: the message to position p is Llr::boxplus_n of the other
+// inputs (llr.rs:210-215), 2 atanh(prod tanh(x_q / 2)) with the product
+// folded from 1.0 in ascending position order; a lone input gets +0.0.
+template <int MAXD, bool NMS, bool SPA = false>
 __global__ void __launch_bounds__(512)
 cn_stream_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
                  float *__restrict__ c2v, const float *__restrict__ belief,
@@ -156,15 +160,29 @@ cn_stream_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t 
                     bl[p] = fb[re[p].x * cv.z + wrap_add(i, re[p].y, cv.z)];
                     co[p] = fc[(size_t)p * cv.z + i];
                 }
-                CheckAcc acc;
-                acc.init();
+                if constexpr (SPA) {
+                    float th[DA];
 #pragma unroll
-                for (int p = 0; p < D; ++p) acc.add(p, bl[p] - co[p]);
+                    for (int p = 0; p < D; ++p) th[p] = tanhf((bl[p] - co[p]) / 2.0f);
 #pragma unroll
-                for (int p = 0; p < D; ++p) {
-                    float o = acc.out(D, p);
-                    if constexpr (NMS) o = wc[p] * o;
-                    fc[(size_t)p * cv.z + i] = o;
+                    for (int p = 0; p < D; ++p) {
+                        float prod = 1.0f;
+#pragma unroll
+                        for (int q = 0; q < D; ++q)
+                            if (q != p) prod = prod * th[q];
+                        fc[(size_t)p * cv.z + i] = D > 1 ? 2.0f * atanhf(prod) : 0.0f;
+                    }
+                } else {
+                    CheckAcc acc;
+                    acc.init();
+#pragma unroll
+                    for (int p = 0; p < D; ++p) acc.add(p, bl[p] - co[p]);
+#pragma unroll
+                    for (int p = 0; p < D; ++p) {
+                        float o = acc.out(D, p);
+                        if constexpr (NMS) o = wc[p] * o;
+                        fc[(size_t)p * cv.z + i] = o;
+                    }
                 }
             }
         });
@@ -648,13 +666,15 @@ hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, con
 #undef GF2L_VN
 #undef GF2L_VN_DYN
     } else {
-#define GF2L_CN(MD, NM)                                                                        \
-    hipLaunchKernelGGL((cn_stream_kernel<MD, NM>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
+#define GF2L_CN(MD, NM, SP)                                                                        \
+    hipLaunchKernelGGL((cn_stream_kernel<MD, NM, SP>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
                        ws.c2v, ws.belief, a.w_cur, xcd)
-        if (nms_cur) {
-            if (wide_c) GF2L_CN(32, true); else GF2L_CN(16, true);
+        if (a.spa) {
+            if (wide_c) GF2L_CN(32, false, true); else GF2L_CN(16, false, true);
+        } else if (nms_cur) {
+            if (wide_c) GF2L_CN(32, true, false); else GF2L_CN(16, true, false);
         } else {
-            if (wide_c) GF2L_CN(32, false); else GF2L_CN(16, false);
+            if (wide_c) GF2L_CN(32, false, false); else GF2L_CN(16, false, false);
         }
 #undef GF2L_CN
         if (ev) hipEventRecord(ev[1], s);

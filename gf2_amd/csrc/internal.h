@@ -118,6 +118,7 @@ struct IterArgs {
     int last;                        // retire frames that still fail the syndrome
     int fb_phase;                    // value written to fb[] for retiring frames
     const float *w_prev, *w_cur;     // per-block-edge weights (NULL = plain min-sum)
+    int spa;                         // sum-product check rule (STREAM path, no weights)
     uint64_t *msg;
     uint32_t *iters;
     uint8_t *conv, *synd, *fb;
@@ -182,6 +183,7 @@ struct gf2l_ctx {
     hipStream_t stream = nullptr;
     int num_cus = 0;
     int path = 0;
+    int rule = 0;                // GF2L_RULE_*: check update of the plain decoder
     gf2l::Tuning tune;
     size_t lds_limit = 0;        // bytes of dynamic LDS one workgroup may use
     std::string err;

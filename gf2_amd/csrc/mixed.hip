@@ -810,6 +810,8 @@ int gf2l_decode_mixed_device(gf2l_ctx *ctx, gf2l_codeset *cs, const uint16_t *co
                              const float *d_llr, uint32_t max_it, uint64_t *d_msg_words,
                              uint32_t *d_iterations, uint8_t *d_converged, uint8_t *d_syndrome_ok) {
     return gf2l::guard(ctx, [&]() -> int {
+        if (ctx->rule != GF2L_RULE_MINSUM)
+            return fail(ctx, GF2L_ERR_UNSUPPORTED, "the mixed-code decoder runs the min-sum rule only");
         return gf2l::decode_mixed(ctx, cs, nullptr, code_ids, frames, d_llr, max_it, d_msg_words, d_iterations,
                                   d_converged, d_syndrome_ok, nullptr);
     });
@@ -822,6 +824,8 @@ int gf2l_decode_mixed_nms_device(gf2l_ctx *ctx, gf2l_codeset *cs, const gf2l_nms
     if (!ctx) return GF2L_ERR_INVALID;
     return gf2l::guard(ctx, [&]() -> int {
         if (!decoders) return fail(ctx, GF2L_ERR_INVALID, "null decoder list");
+        if (ctx->rule != GF2L_RULE_MINSUM)
+            return fail(ctx, GF2L_ERR_UNSUPPORTED, "the learned decoders run the min-sum rule only");
         return gf2l::decode_mixed(ctx, cs, decoders, code_ids, frames, d_llr, fallback_max_it, d_msg_words,
                                   d_iterations, d_converged, d_syndrome_ok, d_fallback);
     });

@@ -752,6 +752,8 @@ int gf2l_nms_decode_device(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr,
         if (frames == 0) return GF2L_OK;
         if (!d_llr || !d_msg_words || !d_iterations || !d_converged || !d_syndrome_ok)
             return fail(ctx, GF2L_ERR_INVALID, "null buffer");
+        if (ctx->rule != GF2L_RULE_MINSUM)
+            return fail(ctx, GF2L_ERR_UNSUPPORTED, "the learned decoders run the min-sum rule only");
         GF2L_HIP(ctx, hipSetDevice(ctx->device));
         if (q->layered) {
             if (frames > 0xffffffffull) return fail(ctx, GF2L_ERR_INVALID, "too many frames in one batch");

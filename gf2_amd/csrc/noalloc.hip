@@ -95,6 +95,8 @@ extern "C" int gf2l_decode_batch_device_noalloc(gf2l_ctx *ctx, const gf2l_code *
         if (!d_llr || !d_msg_words || !d_iterations || !d_converged || !d_syndrome_ok)
             return fail(ctx, GF2L_ERR_INVALID, "null buffer");
         if (frames > 0xffffffffull) return fail(ctx, GF2L_ERR_INVALID, "too many frames in one batch");
+        if (ctx->rule != GF2L_RULE_MINSUM)
+            return fail(ctx, GF2L_ERR_UNSUPPORTED, "the no-allocation decoder runs the min-sum rule only");
         const gf2l::CodeView &cv = code->view;
         if (code->is_csr || cv.max_dc > gf2l::kMaxStateDegree)
             return fail(ctx, GF2L_ERR_UNSUPPORTED,

@@ -114,6 +114,11 @@ class Context:
     def set_path(self, path: int) -> None:
         _lib.check(self._h, _lib.load().gf2l_set_path(self._h, path))
 
+    def set_check_rule(self, rule: int) -> None:
+        """_lib.RULE_MINSUM (default, the reference decoder's rule) or
+        _lib.RULE_SPA (check_node_update_spa, core.rs:762-787)."""
+        _lib.check(self._h, _lib.load().gf2l_set_check_rule(self._h, rule))
+
     def set_tuning(self, key: int, value: int) -> None:
         _lib.check(self._h, _lib.load().gf2l_set_tuning(self._h, key, int(value)))
 

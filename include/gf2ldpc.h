@@ -98,6 +98,25 @@ int gf2l_profile_read_kernel(gf2l_ctx *ctx, int kernel, double *ms, uint64_t *la
 #define GF2L_PATH_STATE 3
 int gf2l_set_path(gf2l_ctx *ctx, int path);
 
+/* Check-node rule of the plain decoder (gf2l_decode_batch[_device],
+ * gf2l_decode_trace):
+ *   GF2L_RULE_MINSUM  (default) LdpcDecoder::check_node_update_minsum
+ *                     (core.rs:790-814), bit-exact with the reference
+ *   GF2L_RULE_SPA     LdpcDecoder::check_node_update_spa (core.rs:762-787,
+ *                     compiled but never called by the reference decoder):
+ *                     the message to an edge is Llr::boxplus_n of the others
+ *                     (llr.rs:210-215), 2 atanh(prod tanh(L/2)) in binary32.
+ *                     The device's tanh/atanh differ from the host C
+ *                     library's in the last bits, so results match the
+ *                     reference within a tolerance, not bit for bit.  Runs on
+ *                     the STREAM path of quasi-cyclic codes only (a forced
+ *                     LDS or STATE path, a DVB or general sparse code, the
+ *                     learned, mixed-code and no-allocation decoders return
+ *                     GF2L_ERR_UNSUPPORTED under it). */
+#define GF2L_RULE_MINSUM 0
+#define GF2L_RULE_SPA 1
+int gf2l_set_check_rule(gf2l_ctx *ctx, int rule);
+
 /* Launch-shape tunables of the HBM paths (results never depend on them).
  *   GF2L_TUNE_WG_PER_CU  persistent workgroups per compute unit (1..32); 0 (default):
  *                        5 with counter-fed items (GF2L_TUNE_CHUNK), else 3

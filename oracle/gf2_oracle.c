@@ -197,13 +197,24 @@ static void hard_bits(const float *belief, uint32_t n, uint64_t *words) {
         if (belief[v] < 0.0f) words[v >> 6] |= (uint64_t)1 << (v & 63);
 }
 
+/* Llr::boxplus_n, crates/gf2-coding/src/llr.rs:210-215:
+ * 2 * atanh(prod tanh(L_i / 2)), the product folded from 1.0 in input order
+ * (Iterator::product), every step a binary32 operation (tanhf / atanhf of
+ * the C library, as Rust's f32::tanh / f32::atanh call). */
+float or_boxplus_n(const float *x, size_t n) {
+    float prod = 1.0f;
+    for (size_t i = 0; i < n; ++i) prod = prod * tanhf(x[i] / 2.0f);
+    return 2.0f * atanhf(prod);
+}
+
 /* decode_iterative on a freshly constructed decoder (core.rs:653-697,
  * 889-938).  Flat arrays replace the nested Vec<Vec<Llr>>; the CSR/CSC
  * position maps replace the reference's linear neighbour scans
- * (core.rs:841-856) and return the same positions. */
-void or_decode_iterative_ex(const or_code *c, const float *llr, uint32_t max_it,
-                            const float *edge_w, uint64_t *msg_words, or_result *res,
-                            float *trace_c2v, float *trace_bel, int no_early_stop) {
+ * (core.rs:841-856) and return the same positions.  spa != 0 replaces the
+ * min-sum check update by check_node_update_spa (core.rs:762-787). */
+static void decode_core(const or_code *c, const float *llr, uint32_t max_it, const float *edge_w, int spa,
+                        uint64_t *msg_words, or_result *res, float *trace_c2v, float *trace_bel,
+                        int no_early_stop) {
     const uint32_t m = c->m, n = c->n;
     const uint64_t nnz = c->nnz;
     float *c2v = (float *)calloc(nnz ? nnz : 1, sizeof(float)); /* [check][pos] */
@@ -228,14 +239,15 @@ void or_decode_iterative_ex(const or_code *c, const float *llr, uint32_t max_it,
     int converged = 0;
     for (uint32_t it = 0; it < max_it; ++it) {
         iterations = it + 1;
-        /* check_node_update_minsum, core.rs:790-814 */
+        /* check_node_update_minsum, core.rs:790-814 (check_node_update_spa,
+         * core.rs:762-787, when spa) */
         for (uint32_t r = 0; r < m; ++r) {
             uint64_t b = c->row_ptr[r], d = c->row_ptr[r + 1] - b;
             for (uint64_t p = 0; p < d; ++p) {
                 size_t t = 0;
                 for (uint64_t q = 0; q < d; ++q)
                     if (q != p) temp[t++] = v2c[csr_to_csc[b + q]];
-                float msg = (t == 0) ? 0.0f : or_minsum_avx2(temp, t);
+                float msg = (t == 0) ? 0.0f : spa ? or_boxplus_n(temp, t) : or_minsum_avx2(temp, t);
                 if (edge_w) msg = edge_w[(size_t)it * nnz + b + p] * msg; /* llr.rs:352-355 */
                 c2v[b + p] = msg;
             }
@@ -272,10 +284,21 @@ void or_decode_iterative_ex(const or_code *c, const float *llr, uint32_t max_it,
     free(c2v); free(v2c); free(belief); free(csr_to_csc); free(temp); free(hard);
 }
 
+void or_decode_iterative_ex(const or_code *c, const float *llr, uint32_t max_it,
+                            const float *edge_w, uint64_t *msg_words, or_result *res,
+                            float *trace_c2v, float *trace_bel, int no_early_stop) {
+    decode_core(c, llr, max_it, edge_w, 0, msg_words, res, trace_c2v, trace_bel, no_early_stop);
+}
+
 void or_decode_iterative(const or_code *c, const float *llr, uint32_t max_it,
                          const float *edge_w, uint64_t *msg_words, or_result *res,
                          float *trace_c2v, float *trace_bel) {
-    or_decode_iterative_ex(c, llr, max_it, edge_w, msg_words, res, trace_c2v, trace_bel, 0);
+    decode_core(c, llr, max_it, edge_w, 0, msg_words, res, trace_c2v, trace_bel, 0);
+}
+
+void or_decode_spa(const or_code *c, const float *llr, uint32_t max_it, uint64_t *msg_words,
+                   or_result *res, float *trace_c2v, float *trace_bel) {
+    decode_core(c, llr, max_it, NULL, 1, msg_words, res, trace_c2v, trace_bel, 0);
 }
 
 /* ------------------------------------------------------------------ */
@@ -284,7 +307,7 @@ void or_decode_iterative(const or_code *c, const float *llr, uint32_t max_it,
  * input order as rayon's indexed collect does. */
 typedef struct {
     const or_code *c; const float *llr; size_t f0, f1; uint32_t max_it;
-    const float *edge_w; uint64_t *msg; or_result *res;
+    const float *edge_w; uint64_t *msg; or_result *res; int spa;
 } batch_job;
 
 static void *batch_worker(void *arg) {
@@ -292,13 +315,13 @@ static void *batch_worker(void *arg) {
     uint32_t k = (j->c->n > j->c->m) ? j->c->n - j->c->m : 0;
     size_t kw = ((size_t)k + 63) / 64;
     for (size_t f = j->f0; f < j->f1; ++f)
-        or_decode_iterative(j->c, j->llr + f * j->c->n, j->max_it, j->edge_w,
-                            j->msg + f * kw, j->res + f, NULL, NULL);
+        decode_core(j->c, j->llr + f * j->c->n, j->max_it, j->edge_w, j->spa, j->msg + f * kw, j->res + f,
+                    NULL, NULL, 0);
     return NULL;
 }
 
-void or_decode_batch(const or_code *c, const float *llr, size_t frames, uint32_t max_it,
-                     const float *edge_w, uint64_t *msg_words, or_result *res, int threads) {
+static void run_batch(const or_code *c, const float *llr, size_t frames, uint32_t max_it,
+                      const float *edge_w, int spa, uint64_t *msg_words, or_result *res, int threads) {
     if (frames == 0) return;
     if (threads < 1) threads = 1;
     if ((size_t)threads > frames) threads = (int)frames;
@@ -306,7 +329,7 @@ void or_decode_batch(const or_code *c, const float *llr, size_t frames, uint32_t
     batch_job *jobs = (batch_job *)malloc(sizeof(batch_job) * threads);
     for (int t = 0; t < threads; ++t) {
         jobs[t] = (batch_job){c, llr, frames * t / threads, frames * (t + 1) / threads,
-                              max_it, edge_w, msg_words, res};
+                              max_it, edge_w, msg_words, res, spa};
         if (threads == 1) batch_worker(&jobs[t]);
         else pthread_create(&tid[t], NULL, batch_worker, &jobs[t]);
     }
@@ -314,6 +337,16 @@ void or_decode_batch(const or_code *c, const float *llr, size_t frames, uint32_t
         for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
     free(tid);
     free(jobs);
+}
+
+void or_decode_batch(const or_code *c, const float *llr, size_t frames, uint32_t max_it,
+                     const float *edge_w, uint64_t *msg_words, or_result *res, int threads) {
+    run_batch(c, llr, frames, max_it, edge_w, 0, msg_words, res, threads);
+}
+
+void or_decode_batch_spa(const or_code *c, const float *llr, size_t frames, uint32_t max_it,
+                         uint64_t *msg_words, or_result *res, int threads) {
+    run_batch(c, llr, frames, max_it, NULL, 1, msg_words, res, threads);
 }
 
 /* ------------------------------------------------------------------ */

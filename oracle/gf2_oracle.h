@@ -95,6 +95,15 @@ void or_decode_iterative_ex(const or_code *c, const float *llr, uint32_t max_it,
 void or_decode_batch(const or_code *c, const float *llr, size_t frames, uint32_t max_it,
                      const float *edge_w, uint64_t *msg_words, or_result *res, int threads);
 
+/* Sum-product check rule: Llr::boxplus_n (llr.rs:210-215) for n >= 1 inputs,
+ * and the decoder with check_node_update_spa (core.rs:762-787) in place of
+ * the min-sum update (dead code in the reference; same loop otherwise). */
+float or_boxplus_n(const float *x, size_t n);
+void or_decode_spa(const or_code *c, const float *llr, uint32_t max_it, uint64_t *msg_words,
+                   or_result *res, float *trace_c2v, float *trace_bel);
+void or_decode_batch_spa(const or_code *c, const float *llr, size_t frames, uint32_t max_it,
+                         uint64_t *msg_words, or_result *res, int threads);
+
 /* ---- normalized min-sum training oracle (double precision) ------------
  * Forward pass of the weighted min-sum decoder with T unrolled iterations
  * (no early stop), and the gradient of

@@ -63,6 +63,14 @@ def load():
         lib.or_decode_batch.restype = None
         lib.or_decode_batch.argtypes = [ctypes.POINTER(OrCode), vp, ctypes.c_size_t,
                                         ctypes.c_uint32, vp, vp, vp, ctypes.c_int]
+        lib.or_boxplus_n.restype = ctypes.c_float
+        lib.or_boxplus_n.argtypes = [vp, ctypes.c_size_t]
+        lib.or_decode_spa.restype = None
+        lib.or_decode_spa.argtypes = [ctypes.POINTER(OrCode), vp, ctypes.c_uint32, vp,
+                                      ctypes.POINTER(OrResult), vp, vp]
+        lib.or_decode_batch_spa.restype = None
+        lib.or_decode_batch_spa.argtypes = [ctypes.POINTER(OrCode), vp, ctypes.c_size_t, ctypes.c_uint32, vp, vp,
+                                            ctypes.c_int]
         lib.or_nms_loss_grad.restype = ctypes.c_double
         lib.or_nms_loss_grad.argtypes = [ctypes.POINTER(OrCode), vp, vp, ctypes.c_uint32, vp, vp]
         lib.or_philox4x32.restype = None
@@ -188,6 +196,29 @@ class Code:
                                       _p(tc), _p(tb), int(no_early_stop))
         return tc, tb, msg, (res.iterations, bool(res.converged), bool(res.syndrome_ok))
 
+    def decode_batch_spa(self, llr: np.ndarray, max_it: int, threads: int = 1):
+        """decode_batch with the sum-product check rule (core.rs:762-787)."""
+        llr = np.ascontiguousarray(llr, dtype=np.float32).reshape(-1, self.n)
+        F = llr.shape[0]
+        kw = (self.k + 63) // 64
+        msg = np.zeros((F, kw), dtype=np.uint64)
+        res = (OrResult * max(F, 1))()
+        load().or_decode_batch_spa(self._p, _p(llr), F, max_it, _p(msg), res, threads)
+        it = np.array([res[i].iterations for i in range(F)], dtype=np.uint32)
+        conv = np.array([res[i].converged for i in range(F)], dtype=bool)
+        synd = np.array([res[i].syndrome_ok for i in range(F)], dtype=bool)
+        return msg, it, conv, synd
+
+    def decode_trace_spa(self, llr: np.ndarray, max_it: int):
+        """One frame with the sum-product rule: (c2v [max_it, nnz], belief [max_it, n], msg, result)."""
+        llr = np.ascontiguousarray(llr, dtype=np.float32).reshape(self.n)
+        tc = np.full((max(max_it, 1), self.nnz), np.nan, dtype=np.float32)
+        tb = np.full((max(max_it, 1), self.n), np.nan, dtype=np.float32)
+        msg = np.zeros((self.k + 63) // 64, dtype=np.uint64)
+        res = OrResult()
+        load().or_decode_spa(self._p, _p(llr), max_it, _p(msg), ctypes.byref(res), _p(tc), _p(tb))
+        return tc, tb, msg, (res.iterations, bool(res.converged), bool(res.syndrome_ok))
+
     def port_decode_batch(self, llr: np.ndarray, max_it: int, threads: int):
         """The structural restatement of the rayon path (ref_port.cpp)."""
         if self._port is None:
@@ -293,6 +324,12 @@ def framegen_zero(n: int, k: int, seed: int, frame_id: int, ebn0_lo: float, ebn0
 def minsum_avx2(x) -> np.float32:
     a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
     return np.float32(load().or_minsum_avx2(_p(a), a.size))
+
+
+def boxplus_n(x) -> np.float32:
+    """Llr::boxplus_n (llr.rs:210-215) in binary32."""
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    return np.float32(load().or_boxplus_n(_p(a), a.size))
 
 
 def minsum_scalar(x) -> np.float32:
