@@ -33,6 +33,7 @@ TUNE_GRAPH_ITERS = 3
 TUNE_CHUNK = 4
 TUNE_LAYERED_COLUMNS = 5
 TUNE_LAYERED_KERNEL = 6
+TUNE_LDS_GROUPS = 7
 LAYERED_AUTO, LAYERED_GENERAL, LAYERED_STAIRCASE = 0, 1, 2
 
 KERNEL_CHECK = 0

@@ -151,7 +151,7 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
             GF2L_HIP(ctx, hipMalloc(&d_tb, (size_t)trace_iters * F * cv.n * sizeof(float)));
         }
         GF2L_HIP(ctx, gf2l::launch_lds_decode(cv, d_llr, F, max_it, d_w, fb_it, fb_continue, d_msg, d_iters,
-                                              d_conv, d_synd, d_fb, d_tc, d_tb, trace_iters, s));
+                                              d_conv, d_synd, d_fb, d_tc, d_tb, trace_iters, ctx->tune.lds_groups, s));
         if (d_tc) {
             GF2L_HIP(ctx, hipMemcpyAsync(h_trace_c2v, d_tc, (size_t)trace_iters * F * E * sizeof(float),
                                          hipMemcpyDeviceToHost, s));
@@ -661,6 +661,10 @@ int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value) {
             case GF2L_TUNE_LAYERED_KERNEL:
                 if (value < 0 || value > 2) return fail(ctx, GF2L_ERR_INVALID, "layered kernel out of [0,2]");
                 ctx->tune.layered_kernel = value;
+                return GF2L_OK;
+            case GF2L_TUNE_LDS_GROUPS:
+                if (value < 0 || value > 512) return fail(ctx, GF2L_ERR_INVALID, "lds groups out of [0,512]");
+                ctx->tune.lds_groups = value;
                 return GF2L_OK;
             default:
                 return fail(ctx, GF2L_ERR_INVALID, "unknown tuning key");

@@ -126,11 +126,7 @@ vn_state_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *
 // ---------------------------------------------------------------------------
 // STREAM path (one stored message per edge).
 
-// SPA: the sum-product rule of check_node_update_spa (core.rs:762-787) in
-// place of min-sum: the message to position p is Llr::boxplus_n of the other
-// inputs (llr.rs:210-215), 2 atanh(prod tanh(x_q / 2)) with the product
-// folded from 1.0 in ascending position order; a lone input gets +0.0.
-template <int MAXD, bool NMS, bool SPA = false>
+template <int MAXD, bool NMS>
 __global__ void __launch_bounds__(512)
 cn_stream_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
                  float *__restrict__ c2v, const float *__restrict__ belief,
@@ -160,32 +156,62 @@ cn_stream_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t 
                     bl[p] = fb[re[p].x * cv.z + wrap_add(i, re[p].y, cv.z)];
                     co[p] = fc[(size_t)p * cv.z + i];
                 }
-                if constexpr (SPA) {
-                    float th[DA];
+                CheckAcc acc;
+                acc.init();
 #pragma unroll
-                    for (int p = 0; p < D; ++p) th[p] = tanhf((bl[p] - co[p]) / 2.0f);
+                for (int p = 0; p < D; ++p) acc.add(p, bl[p] - co[p]);
 #pragma unroll
-                    for (int p = 0; p < D; ++p) {
-                        float prod = 1.0f;
-#pragma unroll
-                        for (int q = 0; q < D; ++q)
-                            if (q != p) prod = prod * th[q];
-                        fc[(size_t)p * cv.z + i] = D > 1 ? 2.0f * atanhf(prod) : 0.0f;
-                    }
-                } else {
-                    CheckAcc acc;
-                    acc.init();
-#pragma unroll
-                    for (int p = 0; p < D; ++p) acc.add(p, bl[p] - co[p]);
-#pragma unroll
-                    for (int p = 0; p < D; ++p) {
-                        float o = acc.out(D, p);
-                        if constexpr (NMS) o = wc[p] * o;
-                        fc[(size_t)p * cv.z + i] = o;
-                    }
+                for (int p = 0; p < D; ++p) {
+                    float o = acc.out(D, p);
+                    if constexpr (NMS) o = wc[p] * o;
+                    fc[(size_t)p * cv.z + i] = o;
                 }
             }
         });
+    }
+}
+
+// The sum-product rule of check_node_update_spa (core.rs:762-787) in place
+// of min-sum: the message to position p is Llr::boxplus_n of the other
+// inputs (llr.rs:210-215), 2 atanh(prod tanh(x_q / 2)) with the product
+// folded from 1.0 in ascending position order; a lone input gets +0.0.
+// Loops run to MAXD under a runtime-degree guard (one copy of the
+// transcendental code per class rather than one per degree).
+template <int MAXD>
+__global__ void __launch_bounds__(512)
+cn_stream_spa_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
+                     float *__restrict__ c2v, const float *__restrict__ belief, int xcd) {
+    const XcdSplit xs(*count_ptr, cv.mb, xcd);
+    const size_t E = (size_t)cv.nbe * cv.z;
+    for (uint32_t item = xs.q; item < xs.items; item += xs.Q) {
+        const uint32_t f = list[xs.list_index(item, cv.mb)];
+        const uint32_t br = item % cv.mb;
+        const uint32_t e0 = cv.row_ptr[br];
+        const uint32_t d = cv.row_ptr[br + 1] - e0;
+        float *fc = c2v + (size_t)f * E + (size_t)e0 * cv.z;
+        const float *fb = belief + (size_t)f * cv.n;
+        for (uint32_t i = threadIdx.x; i < cv.z; i += blockDim.x) {
+            float th[MAXD];
+#pragma unroll
+            for (int p = 0; p < MAXD; ++p) {
+                th[p] = 1.0f;
+                if (p < (int)d) {
+                    const uint2 re = cv.row_edge[e0 + p];
+                    const float x = fb[re.x * cv.z + wrap_add(i, re.y, cv.z)] - fc[(size_t)p * cv.z + i];
+                    th[p] = tanhf(x / 2.0f);
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < MAXD; ++p) {
+                if (p < (int)d) {
+                    float prod = 1.0f;
+#pragma unroll
+                    for (int q = 0; q < MAXD; ++q)
+                        if (q != p && q < (int)d) prod = prod * th[q];
+                    fc[(size_t)p * cv.z + i] = d > 1 ? 2.0f * atanhf(prod) : 0.0f;
+                }
+            }
+        }
     }
 }
 
@@ -320,94 +346,148 @@ __global__ void state_expand_kernel(CodeView cv, const uint32_t *__restrict__ st
     }
 }
 
+// Dynamic LDS of the LDS path: messages, beliefs, LLRs, then the code
+// tables (row and column edges, row and column pointers).
+size_t lds_decode_bytes(const CodeView &cv) {
+    return (size_t)lds_table_offset(cv) * sizeof(float) + (size_t)cv.nbe * 2 * sizeof(uint2) +
+           (size_t)(cv.mb + 1 + cv.nb + 1) * sizeof(uint32_t);
+}
+
 // ---------------------------------------------------------------------------
-// LDS path: one workgroup owns one frame for all its iterations; messages,
-// beliefs and LLRs live in LDS.  Used when (E + 2n) floats fit.
-// Phase 0 runs max_it iterations (weighted by w[it-1][e] when w is given);
+// LDS path: one workgroup owns one frame for all its iterations.  Messages,
+// beliefs and LLRs live in LDS, and the code tables are copied there once
+// per workgroup (a table read is an LDS read, not a dependent global load).
+// The threads form G groups of ZT lanes: group g updates block-rows g, g + G, ... and
+// block-columns g, g + G, ..., lane i the lifted checks / variables i,
+// i + ZT, ... of each, so no thread divides an index inside the loops.
+// An iteration is a check pass and a variable pass.  The check pass also
+// takes the parity of the hard decisions b < 0 of every check's inputs,
+// which is the syndrome of the previous iteration's beliefs: a frame whose
+// previous iteration satisfied every check stops there (the messages that
+// pass computed are dropped; the beliefs, the output, are untouched).  The
+// last iteration of a phase is checked by a pass of its own.
+// Phase 0 runs max_it iterations (weighted by w[it-1][e]: NMS, w given);
 // a frame that still fails the syndrome is, when fb_it > 0, decoded again
 // from its channel LLRs by fb_it plain iterations (phase 1), or with
 // fb_continue by fb_it plain iterations continued from the phase-0 beliefs
-// and (weighted) messages.
-template <int DMAX, int DVMAX>
+// and (weighted) messages.  The plain instantiation (NMS false) carries no
+// weight code, which keeps it within 80 VGPRs (6 waves per SIMD).
+template <int DMAX, int DVMAX, bool NMS>
 __global__ void __launch_bounds__(512)
-lds_decode_kernel(CodeView cv, const float *__restrict__ llr, uint32_t frames, uint32_t max_it,
+lds_decode_kernel(CodeView cv, LdsShape shp, const float *__restrict__ llr, uint32_t frames, uint32_t max_it,
                   const float *__restrict__ w, uint32_t fb_it, int fb_continue, uint64_t *__restrict__ msg,
                   uint32_t *__restrict__ iters, uint8_t *__restrict__ conv,
                   uint8_t *__restrict__ synd, uint8_t *__restrict__ fb,
                   float *__restrict__ trace_c2v, float *__restrict__ trace_bel,
                   uint32_t trace_iters) {
     extern __shared__ float sm[];
-    const uint32_t E = cv.nbe * cv.z, n = cv.n, z = cv.z;
+    const uint32_t E = cv.nbe * cv.z, n = cv.n, z = cv.z, mb = cv.mb, nb = cv.nb;
     float *s_c2v = sm;
     float *s_bel = sm + E;
     float *s_llr = s_bel + n;
+    uint2 *s_re = reinterpret_cast<uint2 *>(sm + lds_table_offset(cv));
+    uint2 *s_ce = s_re + cv.nbe;
+    uint32_t *s_rp = reinterpret_cast<uint32_t *>(s_ce + cv.nbe);
+    uint32_t *s_cp = s_rp + mb + 1;
+    const uint32_t G = shp.groups, ZT = shp.lanes;
+    const uint32_t g = threadIdx.x / ZT, lane = threadIdx.x - g * ZT;
+    const bool act = g < G;
+    for (uint32_t t = threadIdx.x; t < cv.nbe; t += blockDim.x) {
+        s_re[t] = cv.row_edge[t];
+        s_ce[t] = cv.col_edge[t];
+    }
+    for (uint32_t t = threadIdx.x; t <= mb; t += blockDim.x) s_rp[t] = cv.row_ptr[t];
+    for (uint32_t t = threadIdx.x; t <= nb; t += blockDim.x) s_cp[t] = cv.col_ptr[t];
+    // (the barrier after the first frame's reset orders the tables)
     const uint32_t kw = (cv.k + 63) / 64;
-    for (uint32_t f = blockIdx.x; f < frames; f += gridDim.x) {
-        for (uint32_t v = threadIdx.x; v < n; v += blockDim.x) s_llr[v] = llr[(size_t)f * n + v];
-        uint32_t it_done = 0;
-        int converged = 0, fell_back = 0;
-        for (int phase = 0; phase < 2; ++phase) {
-            const uint32_t n_it = phase == 0 ? max_it : fb_it;
-            if (phase == 1) {
-                if (converged || fb_it == 0) break;
-                fell_back = 1;
-            }
-            if (phase == 0 || !fb_continue) {
-                for (uint32_t v = threadIdx.x; v < n; v += blockDim.x) s_bel[v] = s_llr[v];
-                for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) s_c2v[e] = 0.0f;
-            }
-            __syncthreads();
-            for (uint32_t it = 1; it <= n_it; ++it) {
-                const float *w_it = (phase == 0 && w) ? w + (size_t)(it - 1) * cv.nbe : nullptr;
-                {   // check update
-                    uint32_t br = threadIdx.x / z, i = threadIdx.x % z;
-                    for (uint32_t c = threadIdx.x; c < cv.m; c += blockDim.x) {
-                        const uint32_t e0 = cv.row_ptr[br], d = cv.row_ptr[br + 1] - e0;
-                        float x[DMAX];
+    const uint32_t wave = threadIdx.x >> 6, wlane = threadIdx.x & 63u, nwaves = blockDim.x >> 6;
+
+    // messages of one iteration from the current beliefs (update), and
+    // whether one of this thread's checks is unsatisfied by their signs
+    auto check_pass = [&](const float *w_it, bool update) -> int {
+        int unsat = 0;
+        if (!act) return 0;
+        for (uint32_t i = lane; i < z; i += ZT) {
+            for (uint32_t br = g; br < mb; br += G) {
+                const uint32_t e0 = s_rp[br], d = s_rp[br + 1] - e0;
+                float x[DMAX];
+                uint32_t par = 0;
 #pragma unroll
-                        for (int p = 0; p < DMAX; ++p) {
-                            if (p < (int)d) {
-                                const uint2 re = cv.row_edge[e0 + p];
-                                x[p] = s_bel[re.x * z + wrap_add(i, re.y, z)] - s_c2v[(e0 + p) * z + i];
-                            }
-                        }
-                        CheckAcc acc;
-                        acc.init();
-#pragma unroll
-                        for (int p = 0; p < DMAX; ++p)
-                            if (p < (int)d) acc.add(p, x[p]);
-#pragma unroll
-                        for (int p = 0; p < DMAX; ++p) {
-                            if (p < (int)d) {
-                                float o = acc.out(d, p);
-                                if (w_it) o = w_it[e0 + p] * o;
-                                s_c2v[(e0 + p) * z + i] = o;
-                            }
-                        }
-                        i += blockDim.x;
-                        while (i >= z) { i -= z; ++br; }
+                for (int p = 0; p < DMAX; ++p) {
+                    if (p < (int)d) {
+                        const uint2 re = s_re[e0 + p];
+                        const float b = s_bel[re.x * z + wrap_add(i, re.y, z)];
+                        par ^= (uint32_t)(b < 0.0f);
+                        x[p] = b - s_c2v[(e0 + p) * z + i];
                     }
                 }
-                __syncthreads();
-                {   // variable update
-                    uint32_t bc = threadIdx.x / z, j = threadIdx.x % z;
-                    for (uint32_t v = threadIdx.x; v < n; v += blockDim.x) {
-                        const uint32_t c0 = cv.col_ptr[bc], dv = cv.col_ptr[bc + 1] - c0;
-                        float c[DVMAX];
+                unsat |= (int)par;
+                if (!update) continue;
+                CheckAcc acc;
+                acc.init();
 #pragma unroll
-                        for (int q = 0; q < DVMAX; ++q) {
-                            if (q < (int)dv) {
-                                const uint2 ce = cv.col_edge[c0 + q];
-                                c[q] = s_c2v[ce.x * z + wrap_sub(j, ce.y, z)];
-                            }
+                for (int p = 0; p < DMAX; ++p)
+                    if (p < (int)d) acc.add(p, x[p]);
+#pragma unroll
+                for (int p = 0; p < DMAX; ++p) {
+                    if (p < (int)d) {
+                        float o = acc.out(d, p);
+                        if constexpr (NMS) {  // phase 1 (the plain fallback) has no weights
+                            if (w_it) o = w_it[e0 + p] * o;
                         }
-                        float bel = s_llr[v];
+                        s_c2v[(e0 + p) * z + i] = o;
+                    }
+                }
+            }
+        }
+        return unsat;
+    };
+
+    for (uint32_t f = blockIdx.x; f < frames; f += gridDim.x) {
+        const float *fl = llr + (size_t)f * n;
+        for (uint32_t v = threadIdx.x; v < n; v += blockDim.x) s_llr[v] = fl[v];
+        // beliefs := LLRs, messages := 0 (each thread reads what it wrote above)
+        auto reset = [&]() {
+            for (uint32_t v = threadIdx.x; v < n; v += blockDim.x) s_bel[v] = s_llr[v];
+            for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) s_c2v[e] = 0.0f;
+        };
+        uint32_t it_done = 0;
+        int converged = 0, fell_back = 0;
+        for (int phase = 0; phase < 2 && !converged; ++phase) {
+            const uint32_t n_it = phase == 0 ? max_it : fb_it;
+            if (phase == 1) {
+                if (fb_it == 0) break;
+                fell_back = 1;
+            }
+            if (phase == 0 || !fb_continue) reset();
+            __syncthreads();
+            const uint32_t base = phase == 0 ? 0 : max_it;
+            uint32_t it = 1;
+            for (; it <= n_it; ++it) {
+                const float *w_it = (phase == 0 && w) ? w + (size_t)(it - 1) * cv.nbe : nullptr;
+                const int unsat = __syncthreads_or(check_pass(w_it, true));
+                if (it > 1 && !unsat) {  // iteration it - 1 satisfied every check
+                    converged = 1;
+                    break;
+                }
+                if (act) {  // beliefs: L + the messages in ascending check order
+                    for (uint32_t j = lane; j < z; j += ZT) {
+                        for (uint32_t bc = g; bc < nb; bc += G) {
+                            const uint32_t c0 = s_cp[bc], dv = s_cp[bc + 1] - c0;
+                            float c[DVMAX];
 #pragma unroll
-                        for (int q = 0; q < DVMAX; ++q)
-                            if (q < (int)dv) bel = bel + c[q];
-                        s_bel[v] = bel;
-                        j += blockDim.x;
-                        while (j >= z) { j -= z; ++bc; }
+                            for (int q = 0; q < DVMAX; ++q) {
+                                if (q < (int)dv) {
+                                    const uint2 ce = s_ce[c0 + q];
+                                    c[q] = s_c2v[ce.x * z + wrap_sub(j, ce.y, z)];
+                                }
+                            }
+                            float bel = s_llr[bc * z + j];
+#pragma unroll
+                            for (int q = 0; q < DVMAX; ++q)
+                                if (q < (int)dv) bel = bel + c[q];
+                            s_bel[bc * z + j] = bel;
+                        }
                     }
                 }
                 __syncthreads();
@@ -417,43 +497,33 @@ lds_decode_kernel(CodeView cv, const float *__restrict__ llr, uint32_t frames, u
                     for (uint32_t v = threadIdx.x; v < n; v += blockDim.x)
                         trace_bel[((size_t)(it - 1) * frames + f) * n + v] = s_bel[v];
                 }
-                int unsat = 0;
-                {   // syndrome of the hard decisions
-                    uint32_t br = threadIdx.x / z, i = threadIdx.x % z;
-                    for (uint32_t c = threadIdx.x; c < cv.m; c += blockDim.x) {
-                        const uint32_t e0 = cv.row_ptr[br], e1 = cv.row_ptr[br + 1];
-                        uint32_t par = 0;
-                        for (uint32_t e = e0; e < e1; ++e) {
-                            const uint2 re = cv.row_edge[e];
-                            par ^= (uint32_t)(s_bel[re.x * z + wrap_add(i, re.y, z)] < 0.0f);
-                        }
-                        unsat |= (int)par;
-                        i += blockDim.x;
-                        while (i >= z) { i -= z; ++br; }
-                    }
+            }
+            if (converged) {
+                it_done = base + it - 1;
+            } else if (n_it > 0) {  // the phase's last iteration
+                it_done = base + n_it;
+                if (!__syncthreads_or(check_pass(nullptr, false))) converged = 1;
+            }
+            // frames that stop keep their final state in the later trace slots
+            // (copied from the slot this thread wrote: the messages in LDS may
+            // already be those of the iteration after)
+            if (phase == 0 && trace_c2v && converged && it_done >= 1 && it_done < trace_iters) {
+                const size_t src = (size_t)(it_done - 1) * frames + f;
+                for (uint32_t t = it_done + 1; t <= trace_iters; ++t) {
+                    const size_t dst = (size_t)(t - 1) * frames + f;
+                    for (uint32_t e = threadIdx.x; e < E; e += blockDim.x)
+                        trace_c2v[dst * E + e] = trace_c2v[src * E + e];
+                    for (uint32_t v = threadIdx.x; v < n; v += blockDim.x)
+                        trace_bel[dst * n + v] = trace_bel[src * n + v];
                 }
-                unsat = __syncthreads_or(unsat);
-                it_done = (phase == 0 ? 0 : max_it) + it;
-                if (phase == 0 && trace_c2v && !unsat) {  // frames that stop keep their final state
-                    for (uint32_t t = it + 1; t <= trace_iters; ++t) {
-                        for (uint32_t e = threadIdx.x; e < E; e += blockDim.x)
-                            trace_c2v[((size_t)(t - 1) * frames + f) * E + e] = s_c2v[e];
-                        for (uint32_t v = threadIdx.x; v < n; v += blockDim.x)
-                            trace_bel[((size_t)(t - 1) * frames + f) * n + v] = s_bel[v];
-                    }
-                }
-                if (!unsat) { converged = 1; break; }
             }
         }
-        // max_it == 0: a fresh decoder's beliefs are all zero (core.rs:680-693)
-        for (uint32_t wd = threadIdx.x; wd < kw; wd += blockDim.x) {
-            uint64_t word = 0;
-            if (max_it > 0)
-                for (uint32_t b = 0; b < 64; ++b) {
-                    const uint32_t v = wd * 64 + b;
-                    if (v < cv.k && s_bel[v] < 0.0f) word |= 1ull << b;
-                }
-            msg[(size_t)f * kw + wd] = word;
+        // message words: the first k hard decisions, one ballot per wave and word
+        // (max_it == 0: a fresh decoder's beliefs are all zero, core.rs:680-693)
+        for (uint32_t wd = wave; wd < kw; wd += nwaves) {
+            const uint32_t v = wd * 64 + wlane;
+            const uint64_t word = __ballot(max_it > 0 && v < cv.k && s_bel[v] < 0.0f);
+            if (wlane == 0) msg[(size_t)f * kw + wd] = word;
         }
         if (threadIdx.x == 0) {
             iters[f] = it_done;
@@ -513,17 +583,18 @@ uint32_t hbm_grid(uint32_t items, int num_cus, int wg_per_cu) {
             LAUNCH(32, A, B);              \
     } while (0)
 
-template <int DMAX, int DVMAX>
-hipError_t launch_lds_t(const CodeView &cv, const float *llr, uint32_t frames, uint32_t max_it,
+template <int DMAX, int DVMAX, bool NMS>
+hipError_t launch_lds_t(const CodeView &cv, LdsShape shp, const float *llr, uint32_t frames, uint32_t max_it,
                         const float *w, uint32_t fb_it, bool fb_continue, uint64_t *msg, uint32_t *iters,
                         uint8_t *conv, uint8_t *synd, uint8_t *fb, float *tc, float *tb, uint32_t ti,
                         hipStream_t s) {
-    const size_t bytes = lds_path_bytes(cv);
-    auto k = lds_decode_kernel<DMAX, DVMAX>;
+    const size_t bytes = lds_decode_bytes(cv);
+    auto k = lds_decode_kernel<DMAX, DVMAX, NMS>;
     hipError_t err = set_lds(k, bytes);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k, dim3(frames), dim3(512), bytes, s, cv, llr, frames, max_it, w, fb_it, (int)fb_continue,
-                       msg, iters, conv, synd, fb, tc, tb, ti);
+    const uint32_t block = std::max<uint32_t>(64u, (shp.groups * shp.lanes + 63u) & ~63u);
+    hipLaunchKernelGGL(k, dim3(frames), dim3(block), bytes, s, cv, shp, llr, frames, max_it, w, fb_it,
+                       (int)fb_continue, msg, iters, conv, synd, fb, tc, tb, ti);
     return hipGetLastError();
 }
 
@@ -554,20 +625,44 @@ hipError_t launch_dvb_permute(const CodeView &cv, const float *llr, float *out, 
     return hipGetLastError();
 }
 
-size_t lds_path_bytes(const CodeView &cv) {
-    return ((size_t)cv.nbe * cv.z + 2 * (size_t)cv.n) * sizeof(float);
+size_t lds_path_bytes(const CodeView &cv) { return lds_decode_bytes(cv); }
+
+// Groups of the LDS path: lanes = min(z, 512); automatic groups: the count up
+// to 512 / lanes (and at most max(mb, nb)) with the fewest check plus
+// variable passes, the smaller count on a tie.
+LdsShape lds_shape(const CodeView &cv, int groups) {
+    LdsShape shp;
+    shp.lanes = std::min<uint32_t>(cv.z, 512u);
+    const uint32_t gmax = std::max<uint32_t>(1u, std::min<uint32_t>(512u / shp.lanes, std::max(cv.mb, cv.nb)));
+    if (groups > 0) {
+        shp.groups = std::min<uint32_t>((uint32_t)groups, gmax);
+        return shp;
+    }
+    uint32_t best = 1, best_passes = ~0u;
+    for (uint32_t g = 1; g <= gmax; ++g) {
+        const uint32_t passes = (cv.mb + g - 1) / g + (cv.nb + g - 1) / g;
+        if (passes < best_passes) {
+            best = g;
+            best_passes = passes;
+        }
+    }
+    shp.groups = best;
+    return shp;
 }
 
 hipError_t launch_lds_decode(const CodeView &cv, const float *llr, uint32_t frames, uint32_t max_it,
                              const float *w, uint32_t fb_it, bool fb_continue, uint64_t *msg, uint32_t *iters,
                              uint8_t *conv, uint8_t *synd, uint8_t *fb, float *trace_c2v,
-                             float *trace_bel, uint32_t trace_iters, hipStream_t s) {
+                             float *trace_bel, uint32_t trace_iters, int groups, hipStream_t s) {
     if (frames == 0) return hipSuccess;
+    const LdsShape shp = lds_shape(cv, groups);
     const int dm = pick_dmax(cv.max_dc), dvm = pick_dvmax(cv.max_dv);
-#define GF2L_LDS(DM, DV)                                                                                     \
-    if (dm == DM && dvm == DV)                                                                               \
-        return launch_lds_t<DM, DV>(cv, llr, frames, max_it, w, fb_it, fb_continue, msg, iters, conv, synd, \
-                                    fb, trace_c2v, trace_bel, trace_iters, s);
+#define GF2L_LDS(DM, DV)                                                                                       \
+    if (dm == DM && dvm == DV)                                                                                 \
+        return w ? launch_lds_t<DM, DV, true>(cv, shp, llr, frames, max_it, w, fb_it, fb_continue, msg, iters,  \
+                                              conv, synd, fb, trace_c2v, trace_bel, trace_iters, s)             \
+                 : launch_lds_t<DM, DV, false>(cv, shp, llr, frames, max_it, w, fb_it, fb_continue, msg, iters, \
+                                               conv, synd, fb, trace_c2v, trace_bel, trace_iters, s);
     GF2L_LDS(8, 4) GF2L_LDS(8, 8) GF2L_LDS(8, 16) GF2L_LDS(8, 32)
     GF2L_LDS(16, 4) GF2L_LDS(16, 8) GF2L_LDS(16, 16) GF2L_LDS(16, 32)
     GF2L_LDS(32, 4) GF2L_LDS(32, 8) GF2L_LDS(32, 16) GF2L_LDS(32, 32)
@@ -666,16 +761,25 @@ hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, con
 #undef GF2L_VN
 #undef GF2L_VN_DYN
     } else {
-#define GF2L_CN(MD, NM, SP)                                                                        \
-    hipLaunchKernelGGL((cn_stream_kernel<MD, NM, SP>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
+#define GF2L_CN(MD, NM)                                                                        \
+    hipLaunchKernelGGL((cn_stream_kernel<MD, NM>), dim3(g_cn), dim3(block), 0, s, cv, list, count, \
                        ws.c2v, ws.belief, a.w_cur, xcd)
+#define GF2L_CN_SPA(MD)                                                                                \
+    hipLaunchKernelGGL((cn_stream_spa_kernel<MD>), dim3(g_cn), dim3(block), 0, s, cv, list, count, ws.c2v, \
+                       ws.belief, xcd)
         if (a.spa) {
-            if (wide_c) GF2L_CN(32, false, true); else GF2L_CN(16, false, true);
+            if (wide_c)
+                GF2L_CN_SPA(32);
+            else if (cv.max_dc > 8)
+                GF2L_CN_SPA(16);
+            else
+                GF2L_CN_SPA(8);
         } else if (nms_cur) {
-            if (wide_c) GF2L_CN(32, true, false); else GF2L_CN(16, true, false);
+            if (wide_c) GF2L_CN(32, true); else GF2L_CN(16, true);
         } else {
-            if (wide_c) GF2L_CN(32, false, false); else GF2L_CN(16, false, false);
+            if (wide_c) GF2L_CN(32, false); else GF2L_CN(16, false);
         }
+#undef GF2L_CN_SPA
 #undef GF2L_CN
         if (ev) hipEventRecord(ev[1], s);
         if (wide_v)

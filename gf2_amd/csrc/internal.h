@@ -101,6 +101,7 @@ struct Tuning {
     int chunk = -1;      // items per counter fetch of the check-state kernels (0: fixed stride, -1: automatic)
     int layered_cols = 0; // block-columns of beliefs the general layered kernel keeps in LDS (0: automatic)
     int layered_kernel = 0; // 0: staircase kernel where it applies, 1: general kernel, 2: staircase kernel
+    int lds_groups = 0;  // thread groups per workgroup of the LDS path (0: automatic)
 };
 
 // Word offset of the item counters in Workspace::counts and its size (two
@@ -305,15 +306,29 @@ hipError_t launch_layered_stair(const LayeredArgs &a, uint32_t dh, int num_cus, 
 int decode_csr(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, uint32_t F, uint32_t max_it,
                uint64_t *d_msg, uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd);
 
+// Thread layout of the LDS path's workgroup: `groups` groups of `lanes`
+// threads; group g updates block-rows / block-columns g, g + groups, ...
+struct LdsShape {
+    uint32_t groups, lanes;
+};
+// Float offset of the code tables in the LDS path's dynamic LDS: after the
+// messages, the beliefs and the LLRs, 8-byte aligned.
+__host__ __device__ inline uint32_t lds_table_offset(const CodeView &cv) {
+    const uint32_t f = cv.nbe * cv.z + 2u * cv.n;
+    return (f + 1u) & ~1u;
+}
+
 // Kernel launchers (gf2ldpc_kernels.hip).  All enqueue on `s`.
+// Dynamic LDS of the LDS path (frame state and code tables).
 size_t lds_path_bytes(const CodeView &cv);
 // Phase 0: max_it iterations, weighted by w[it-1][e] if w != NULL; then, for
 // frames still failing the syndrome, fb_it plain iterations from scratch or,
-// with fb_continue, from the phase-0 state.
+// with fb_continue, from the phase-0 state.  groups: thread groups per
+// workgroup (0: automatic, else at most 512 / min(z, 512)).
 hipError_t launch_lds_decode(const CodeView &cv, const float *llr, uint32_t frames, uint32_t max_it,
                              const float *w, uint32_t fb_it, bool fb_continue, uint64_t *msg, uint32_t *iters,
                              uint8_t *conv, uint8_t *synd, uint8_t *fb, float *trace_c2v,
-                             float *trace_bel, uint32_t trace_iters, hipStream_t s);
+                             float *trace_bel, uint32_t trace_iters, int groups, hipStream_t s);
 hipError_t launch_hbm_init(const CodeView &cv, Workspace &ws, int path, const float *llr,
                            uint32_t frames, hipStream_t s);
 hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, const IterArgs &a,

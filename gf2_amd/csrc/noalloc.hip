@@ -115,7 +115,8 @@ extern "C" int gf2l_decode_batch_device_noalloc(gf2l_ctx *ctx, const gf2l_code *
         const int path = gf2l::resolve_path(ctx, cv);
         if (path == gf2l::kLds) {  // the whole frame state lives in LDS: no scratch at all
             GF2L_HIP(ctx, gf2l::launch_lds_decode(cv, d_llr, F, max_it, nullptr, 0, false, d_msg_words, d_iterations,
-                                                  d_converged, d_syndrome_ok, nullptr, nullptr, nullptr, 0, s));
+                                                  d_converged, d_syndrome_ok, nullptr, nullptr, nullptr, 0,
+                                                  ctx->tune.lds_groups, s));
             return GF2L_OK;
         }
         if (path != gf2l::kState)

@@ -141,19 +141,28 @@ int gf2l_set_check_rule(gf2l_ctx *ctx, int rule);
  *   GF2L_TUNE_LAYERED_KERNEL  kernel of the layered learned decoder:
  *                        0 (default): the staircase kernel where the code
  *                        allows it (QC-IRA code whose parity part is a block
- *                        staircase of unshifted circulants, z <= 512, at
- *                        least 10 block-rows, row degrees 5..7 in two adjacent
- *                        values, the information beliefs of one frame in
- *                        LDS), else the general kernel; 1: the general
+ *                        staircase of unshifted circulants, z <= 512, a
+ *                        block-row count of at most 256 that is a multiple
+ *                        of 10 (at least 20) or of 6 (at least 12), row
+ *                        degrees dh - 1 and dh with dh 6 or 7, the
+ *                        information beliefs of one frame in LDS), else the
+ *                        general kernel; 1: the general
  *                        kernel; 2: the staircase kernel (decoding fails
  *                        with GF2L_ERR_UNSUPPORTED where it does not apply).
- *                        Training always runs the general kernel. */
+ *                        Training always runs the general kernel.
+ *   GF2L_TUNE_LDS_GROUPS thread groups per workgroup of the LDS path (one
+ *                        frame per workgroup): group g updates block-rows
+ *                        and block-columns g, g + groups, ... with min(z, 512)
+ *                        lanes each; at most 512 / min(z, 512) (larger
+ *                        values are clamped).  0 (default): the count with
+ *                        the fewest check plus variable passes. */
 #define GF2L_TUNE_WG_PER_CU 1
 #define GF2L_TUNE_XCD_MAP 2
 #define GF2L_TUNE_GRAPH_ITERS 3
 #define GF2L_TUNE_CHUNK 4
 #define GF2L_TUNE_LAYERED_COLUMNS 5
 #define GF2L_TUNE_LAYERED_KERNEL 6
+#define GF2L_TUNE_LDS_GROUPS 7
 int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value);
 
 /* ---- codes ------------------------------------------------------------ */
