@@ -427,11 +427,14 @@ lds_decode_kernel(CodeView cv, LdsShape shp, const float *__restrict__ llr, uint
                 acc.init();
 #pragma unroll
                 for (int p = 0; p < DMAX; ++p)
-                    if (p < (int)d) acc.add(p, x[p]);
+                    if (p < (int)d) acc.add_sel(p, x[p]);
+                // output signs of all positions at once (acc.out's rule)
+                const uint32_t sg = acc.signs<DMAX>(d);
 #pragma unroll
                 for (int p = 0; p < DMAX; ++p) {
                     if (p < (int)d) {
-                        float o = acc.out(d, p);
+                        const float mag = (uint32_t)p == acc.idx ? acc.min2 : acc.min1;
+                        float o = d == 1 ? 0.0f : ((sg >> p) & 1u) ? -mag : mag;
                         if constexpr (NMS) {  // phase 1 (the plain fallback) has no weights
                             if (w_it) o = w_it[e0 + p] * o;
                         }
