@@ -43,6 +43,7 @@ pub const GF2L_PATH_AUTO: c_int = 0;
 pub const GF2L_PATH_LDS: c_int = 1;
 pub const GF2L_PATH_STREAM: c_int = 2;
 pub const GF2L_PATH_STATE: c_int = 3;
+pub const GF2L_PATH_CSR: c_int = 4;
 
 pub const GF2L_RULE_MINSUM: c_int = 0;
 pub const GF2L_RULE_SPA: c_int = 1;
@@ -77,6 +78,7 @@ extern "C" {
         frame_iterations: *mut u64,
     ) -> c_int;
     pub fn gf2l_set_path(ctx: *mut gf2l_ctx, path: c_int) -> c_int;
+    pub fn gf2l_code_path(ctx: *mut gf2l_ctx, code: *const gf2l_code, path: *mut c_int) -> c_int;
     pub fn gf2l_set_check_rule(ctx: *mut gf2l_ctx, rule: c_int) -> c_int;
     pub fn gf2l_set_tuning(ctx: *mut gf2l_ctx, key: c_int, value: c_int) -> c_int;
 

@@ -23,6 +23,7 @@ PATH_AUTO = 0
 PATH_LDS = 1
 PATH_STREAM = 2
 PATH_STATE = 3
+PATH_CSR = 4
 
 RULE_MINSUM = 0
 RULE_SPA = 1
@@ -59,6 +60,7 @@ SIGNATURES = {
     "gf2l_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "gf2l_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
     "gf2l_set_path": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "gf2l_code_path": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "gf2l_set_check_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "gf2l_device_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _c_void_pp]),
     "gf2l_device_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

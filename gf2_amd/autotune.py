@@ -19,6 +19,13 @@ the context tunables of gf2l_set_tuning / gf2l_set_path:
   path         automatic (whole frame in LDS when it fits), compressed check
                state, or one stored message per edge (single-code decoder);
                a path that cannot run the code is skipped
+  lds_groups   thread groups per workgroup of the LDS path (one frame per
+               workgroup; group g takes block-rows and block-columns g,
+               g + groups, ...): the block-row tiling of small codes, which
+               trades threads per frame (latency hiding) against passes per
+               iteration and lanes left idle by z; 0 is the library's choice
+By default a code whose automatic path is the LDS path tunes lds_groups, any
+other code path, wg_per_cu and chunk.
 
 A tuning run decodes a batch of frames generated on the device at a given
 Eb/N0 (the iteration profile of the operating point matters) once per
@@ -47,9 +54,11 @@ CANDIDATES: Dict[str, List[int]] = {
     "graph_iters": [2, 4, 8],
     "chunk": [0, 4, 8, 16],
     "path": [_lib.PATH_AUTO, _lib.PATH_STATE, _lib.PATH_STREAM],
+    "lds_groups": [0, 2, 3, 4, 5, 8],
 }
 _TUNE_KEYS = {"wg_per_cu": _lib.TUNE_WG_PER_CU, "xcd_map": _lib.TUNE_XCD_MAP,
-              "graph_iters": _lib.TUNE_GRAPH_ITERS, "chunk": _lib.TUNE_CHUNK}
+              "graph_iters": _lib.TUNE_GRAPH_ITERS, "chunk": _lib.TUNE_CHUNK,
+              "lds_groups": _lib.TUNE_LDS_GROUPS}
 
 
 def default_cache_path() -> str:
@@ -129,14 +138,27 @@ def _time(fn, repeats: int) -> float:
     return (time.perf_counter() - t0) / repeats * 1e3
 
 
+def default_knobs(code: LdpcCode) -> tuple:
+    """Knobs worth tuning for a code: the LDS path's thread groups when the
+    automatic path keeps the frame in LDS, else the HBM paths' knobs."""
+    ctx = code.context
+    ctx.set_path(_lib.PATH_AUTO)
+    if ctx.code_path(code) == _lib.PATH_LDS:
+        return ("lds_groups",)
+    return ("path", "wg_per_cu", "chunk")
+
+
 def tune_code(code: LdpcCode, frames: int, ebn0_db: float, max_it: int = 50, repeats: int = 2,
-              knobs: Sequence[str] = ("path", "wg_per_cu", "chunk"), cache: Optional[TuningCache] = None,
+              knobs: Optional[Sequence[str]] = None, cache: Optional[TuningCache] = None,
               seed: int = 11) -> dict:
-    """Tunes the single-code decoder for batches of about `frames` frames.
-    Returns {"config", "ms", "results": [(config, ms), ...]} and applies the
-    best config to the code's context."""
+    """Tunes the single-code decoder for batches of about `frames` frames
+    (knobs: default_knobs(code) unless given).  Returns {"config", "ms",
+    "results": [(config, ms), ...]} and applies the best config to the
+    code's context."""
     ctx = code.context
     lib = _lib.load()
+    if knobs is None:
+        knobs = default_knobs(code)
     F = frames
     kw = (code.k() + 63) // 64
     try:

@@ -538,6 +538,28 @@ int gf2l_set_path(gf2l_ctx *ctx, int path) {
     });
 }
 
+int gf2l_code_path(gf2l_ctx *ctx, const gf2l_code *code, int *path) {
+    if (!ctx) return GF2L_ERR_INVALID;
+    return gf2l::guard(ctx, [&]() -> int {
+        if (!code || !path) return fail(ctx, GF2L_ERR_INVALID, "null argument");
+        if (code->is_csr) {
+            if (ctx->rule != GF2L_RULE_MINSUM)
+                return fail(ctx, GF2L_ERR_UNSUPPORTED, "the sum-product rule needs a quasi-cyclic code");
+            *path = GF2L_PATH_CSR;
+            return GF2L_OK;
+        }
+        int p = gf2l::resolve_path(ctx, code->view);
+        if (ctx->rule == GF2L_RULE_SPA) {  // as decode_device: the STREAM path only
+            if (code->view.dvb_q || (ctx->path != gf2l::kAuto && ctx->path != gf2l::kStream))
+                return fail(ctx, GF2L_ERR_UNSUPPORTED, "the sum-product rule needs the STREAM path");
+            p = gf2l::kStream;
+        }
+        if (p < 0) return fail(ctx, GF2L_ERR_UNSUPPORTED, "the forced decoder path cannot run this code");
+        *path = p;
+        return GF2L_OK;
+    });
+}
+
 int gf2l_set_check_rule(gf2l_ctx *ctx, int rule) {
     if (!ctx) return GF2L_ERR_INVALID;
     return gf2l::guard(ctx, [&]() -> int {

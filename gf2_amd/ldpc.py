@@ -114,6 +114,13 @@ class Context:
     def set_path(self, path: int) -> None:
         _lib.check(self._h, _lib.load().gf2l_set_path(self._h, path))
 
+    def code_path(self, code: "LdpcCode") -> int:
+        """The path (PATH_LDS / _STREAM / _STATE / _CSR) a decode of `code`
+        takes under this context's current settings."""
+        p = ctypes.c_int()
+        _lib.check(self.handle, _lib.load().gf2l_code_path(self.handle, code.handle, ctypes.byref(p)))
+        return p.value
+
     def set_check_rule(self, rule: int) -> None:
         """_lib.RULE_MINSUM (default, the reference decoder's rule) or
         _lib.RULE_SPA (check_node_update_spa, core.rs:762-787)."""

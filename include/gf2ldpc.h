@@ -97,6 +97,14 @@ int gf2l_profile_read_kernel(gf2l_ctx *ctx, int kernel, double *ms, uint64_t *la
 #define GF2L_PATH_STREAM 2
 #define GF2L_PATH_STATE 3
 int gf2l_set_path(gf2l_ctx *ctx, int path);
+/* The path a decode of `code` on this context takes under its current path
+ * setting and check rule: GF2L_PATH_LDS, _STREAM or _STATE for a
+ * quasi-cyclic or DVB code, GF2L_PATH_CSR (reported here only) for a general
+ * sparse code.  GF2L_ERR_UNSUPPORTED when the forced path or rule cannot run
+ * the code (the error a decode call would return).  No reference
+ * counterpart: the reference has one CPU path. */
+#define GF2L_PATH_CSR 4
+int gf2l_code_path(gf2l_ctx *ctx, const gf2l_code *code, int *path);
 
 /* Check-node rule of the plain decoder (gf2l_decode_batch[_device],
  * gf2l_decode_trace):

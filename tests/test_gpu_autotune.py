@@ -15,7 +15,8 @@ pytestmark = pytest.mark.gpu
 def ctx():
     c = Context(0)
     yield c
-    autotune.apply(c, {"path": _lib.PATH_AUTO, "wg_per_cu": 3, "xcd_map": 1, "graph_iters": 4, "chunk": -1})
+    autotune.apply(c, {"path": _lib.PATH_AUTO, "wg_per_cu": 3, "xcd_map": 1, "graph_iters": 4, "chunk": -1,
+                       "lds_groups": 0})
 
 
 def test_tune_single_code(ctx, tmp_path):
@@ -63,3 +64,19 @@ def test_tune_dvb_code(ctx, tmp_path):
     assert paths == {_lib.PATH_AUTO, _lib.PATH_STATE}
     assert len(r["results"]) == 2 * len(autotune.CANDIDATES["chunk"])
     assert r["key"].startswith("single:sparse-n16200")
+
+
+def test_tune_lds_groups(ctx, tmp_path):
+    """The N=2304 code runs on the LDS path, so the default knobs are the
+    LDS path's thread groups (its block-row tiling); every group count gives
+    the same results and the best one is applied."""
+    code = LdpcCode(codes.n2304_r12().qc(), ctx)
+    assert autotune.default_knobs(code) == ("lds_groups",)
+    cache = autotune.TuningCache(str(tmp_path / "g.json"))
+    r = autotune.tune_code(code, 512, 2.0, max_it=20, repeats=1, cache=cache)
+    assert [cfg["lds_groups"] for cfg, _ in r["results"]] == autotune.CANDIDATES["lds_groups"]
+    assert cache.get(r["key"]) == r["config"]
+    ctx.set_tuning(_lib.TUNE_LDS_GROUPS, 0)
+    big = LdpcCode(codes.n64800_r12().qc(), ctx)
+    assert autotune.default_knobs(big) == ("path", "wg_per_cu", "chunk")
+    assert ctx.code_path(big) == _lib.PATH_STATE
