@@ -330,3 +330,38 @@ def test_counter_fed_items_match_oracle(ctx, chunk):
         ctx.set_path(_lib.PATH_AUTO)
     ref = orc.Code.from_qc(base, spec.z).decode_batch(llr, 30, threads=8)
     assert_same(gpu, ref, f"chunk {chunk}")
+
+
+# --------------------------------------------------- LDS path thread layouts
+@pytest.mark.parametrize("groups", [1, 3, 8])
+def test_lds_groups_match_oracle(ctx, groups):
+    """The LDS path with a forced thread-group count (group g updates
+    block-rows and block-columns g, g + groups, ...): the oracle's results."""
+    spec = codes.n2304_r12()
+    base = spec.base_matrix()
+    _, _, llr = frames.random_frames(base, spec.z, 96, 1.8, seed=40 + groups)
+    say("lds groups", groups)
+    ctx.set_tuning(_lib.TUNE_LDS_GROUPS, groups)
+    try:
+        gpu = decode_batch_arrays(gpu_code(ctx, base, spec.z, "lds"), llr, 50)
+    finally:
+        ctx.set_tuning(_lib.TUNE_LDS_GROUPS, 0)
+    assert_same(gpu, orc.Code.from_qc(base, spec.z).decode_batch(llr, 50, threads=8), f"groups {groups}")
+
+
+@pytest.mark.parametrize("z", [7, 1000])
+def test_lds_path_lift_edge_cases(ctx, z):
+    """z = 7: an odd number of message and belief floats ahead of the
+    8-byte-aligned code tables in LDS; z = 1000: more lifted checks per
+    block-row than lanes per group (lane i takes checks i and i + 512)."""
+    rng = np.random.default_rng(z)
+    base = rng.integers(-1, z, size=(3, 6)).astype(np.int32)
+    if z % 2 and (int((base >= 0).sum()) * z + 2 * 6 * z) % 2 == 0:  # make the float count odd
+        base[0, 1] = -1 if base[0, 1] >= 0 else 3
+    llr = rng.normal(1.0, 1.6, size=(24, 6 * z)).astype(np.float32)
+    say("lds lift", z)
+    code = gpu_code(ctx, base, z, "lds")
+    assert ctx.code_path(code) == _lib.PATH_LDS
+    assert_same(decode_batch_arrays(code, llr, 30), orc.Code.from_qc(base, z).decode_batch(llr, 30, threads=8),
+                f"lds z={z}")
+    ctx.set_path(_lib.PATH_AUTO)
