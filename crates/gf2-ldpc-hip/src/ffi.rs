@@ -55,6 +55,7 @@ pub const GF2L_TUNE_CHUNK: c_int = 4;
 pub const GF2L_TUNE_LAYERED_COLUMNS: c_int = 5;
 pub const GF2L_TUNE_LAYERED_KERNEL: c_int = 6;
 pub const GF2L_TUNE_LDS_GROUPS: c_int = 7;
+pub const GF2L_TUNE_HBM_GRAPH: c_int = 8;
 
 pub const GF2L_NOALLOC_SCRATCH_BYTES: u32 = 64 << 20;
 

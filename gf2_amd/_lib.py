@@ -35,6 +35,7 @@ TUNE_CHUNK = 4
 TUNE_LAYERED_COLUMNS = 5
 TUNE_LAYERED_KERNEL = 6
 TUNE_LDS_GROUPS = 7
+TUNE_HBM_GRAPH = 8
 LAYERED_AUTO, LAYERED_GENERAL, LAYERED_STAIRCASE = 0, 1, 2
 
 KERNEL_CHECK = 0

@@ -180,18 +180,65 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
                    d_synd, d_fb, h_trace_c2v, h_trace_bel, trace_iters, d_tc, d_tb);
 }
 
+// Captures `unroll` (even) plain iterations of the final phase as a graph:
+// list parity starting at `parity`, iteration numbers from the device word
+// ws.counts[kItcWord].  Reuses the context's graph when it was captured for
+// the same code, sizes, buffers and launch shape.
+int hbm_graph(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const IterArgs &base, uint32_t unroll,
+              uint32_t parity) {
+    const Tuning &t = ctx->tune;
+    std::vector<uintptr_t> key = {
+        (uintptr_t)cv.row_edge, (uintptr_t)path, base.frames, (uintptr_t)base.llr, (uintptr_t)base.msg,
+        (uintptr_t)base.iters, (uintptr_t)base.conv, (uintptr_t)base.synd, (uintptr_t)base.fb,
+        (uintptr_t)ws.lists, (uintptr_t)ws.counts, (uintptr_t)ws.state, (uintptr_t)ws.c2v, (uintptr_t)ws.belief,
+        (uintptr_t)ws.hb, (uintptr_t)ws.cap_frames, base.itc_last, base.itc_fb_from, (uintptr_t)base.spa, unroll,
+        parity, (uintptr_t)(intptr_t)t.wg_per_cu, (uintptr_t)(intptr_t)t.chunk, (uintptr_t)(intptr_t)t.xcd_map,
+        (uintptr_t)ctx->num_cus};
+    if (ctx->hbm_exec && ctx->hbm_key == key) return GF2L_OK;
+    if (ctx->hbm_exec) hipGraphExecDestroy(ctx->hbm_exec);
+    ctx->hbm_exec = nullptr;
+    ctx->hbm_key.clear();
+    hipStream_t s = ctx->stream;
+    hipGraph_t graph = nullptr;
+    GF2L_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t launch = hipSuccess;
+    for (uint32_t u = 0; u < unroll && launch == hipSuccess; ++u) {
+        IterArgs a = base;
+        a.cur = (parity + u) & 1u;
+        launch = launch_hbm_iteration(cv, ws, path, a, ctx->num_cus, ctx->tune, s, nullptr);
+    }
+    const hipError_t end = hipStreamEndCapture(s, &graph);
+    if (launch != hipSuccess || end != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        return fail(ctx, GF2L_ERR_HIP,
+                    std::string("graph capture: ") + hipGetErrorString(launch != hipSuccess ? launch : end));
+    }
+    const hipError_t inst = hipGraphInstantiate(&ctx->hbm_exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (inst != hipSuccess) {
+        ctx->hbm_exec = nullptr;
+        return fail(ctx, GF2L_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(inst));
+    }
+    ctx->hbm_key = std::move(key);
+    return GF2L_OK;
+}
+
 // The iteration loop of the HBM paths over a prepared workspace: LLR
 // permutation of DVB codes, then phase 0 (max_it iterations, weighted by d_w
 // when given) and, with fb_it > 0, phase 1 (fb_it plain iterations for
-// frames still failing, restarted from the LLRs or continued).  Iterations are enqueued in
-// batches of GF2L_TUNE_GRAPH_ITERS; the host reads the running count once per
-// batch (every iteration when tracing) and stops once no frame runs.
-// Iterations enqueued after the last frame retired find an empty list and
-// do nothing.  Allocates nothing: ws.h_count may point at pageable memory.
+// frames still failing, restarted from the LLRs or continued).  Iterations
+// are enqueued in batches of GF2L_TUNE_GRAPH_ITERS (rounded down to an even
+// count, at least 2); the host reads the running count once per batch (every
+// iteration when tracing) and stops once no frame runs.  Iterations enqueued
+// after the last frame retired find an empty list and do nothing.  The plain
+// iterations of the final phase after its first are replayed from a
+// captured graph (graphs: not when tracing or profiling); the rest are
+// launched one by one.  Allocates nothing when graphs is false: ws.h_count
+// may then point at pageable memory.
 int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const float *d_llr, uint32_t F,
             uint32_t max_it, const float *d_w, uint32_t fb_it, bool fb_continue, uint64_t *d_msg,
             uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb, float *h_trace_c2v,
-            float *h_trace_bel, uint32_t trace_iters, float *d_tc, float *d_tb) {
+            float *h_trace_bel, uint32_t trace_iters, float *d_tc, float *d_tb, bool graphs) {
     hipStream_t s = ctx->stream;
     const size_t E = (size_t)cv.nbe * cv.z;
     gf2l::Profile &pf = ctx->prof;
@@ -201,6 +248,8 @@ int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const fl
         d_llr = ws.llr_perm;
     }
     const uint32_t batch = h_trace_c2v ? 1u : (uint32_t)std::max(1, std::min(ctx->tune.graph_iters, kProfileSlots));
+    const uint32_t unroll = std::max(2u, (uint32_t)std::min(ctx->tune.graph_iters, kProfileSlots) & ~1u);
+    const bool use_graph = graphs && ctx->tune.hbm_graph && !h_trace_c2v && !pf.on;
     uint32_t running = F;  // frames entering the first iteration of the current batch
     uint32_t queued = 0;   // iterations enqueued since the last readback
     // per-kernel times of the iterations of one batch (stream synchronized)
@@ -215,6 +264,7 @@ int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const fl
         return GF2L_OK;
     };
     const uint32_t n_phase = fb_it ? 2 : 1;
+    const uint32_t last_it = max_it + (fb_it ? fb_it : 0);
     uint32_t g = 0;  // iterations launched so far: parity of the running list
     for (uint32_t phase = 0; phase < n_phase; ++phase) {
         const uint32_t n_it = phase == 0 ? max_it : fb_it;
@@ -227,14 +277,16 @@ int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const fl
         // a continued fallback never runs the first-iteration kernels, and its
         // first check update subtracts the last weighted messages
         const bool cont = phase == 1 && fb_continue;
-        for (uint32_t it = 1; it <= n_it; ++it, ++g) {
+        const bool final_phase = phase + 1 == n_phase;
+        uint32_t it = 1;
+        while (it <= n_it) {
             gf2l::IterArgs a{};
             a.llr = d_llr;
             a.frames = F;
             a.it = cont ? max_it + it : it;
             a.cur = g & 1u;
             a.it_report = (phase == 0 ? 0 : max_it) + it;
-            a.last = it == n_it && phase + 1 == n_phase;
+            a.last = it == n_it && final_phase;
             a.fb_phase = (int)phase;
             a.w_prev = (w && it >= 2) ? w + (size_t)(it - 2) * cv.nbe : nullptr;
             a.w_cur = w ? w + (size_t)(it - 1) * cv.nbe : nullptr;
@@ -245,6 +297,25 @@ int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const fl
             a.conv = d_conv;
             a.synd = d_synd;
             a.fb = d_fb;
+            a.itc = ws.counts + kItcWord;
+            a.itc_last = last_it;
+            a.itc_fb_from = fb_it ? max_it : last_it;
+            if (use_graph && final_phase && it >= 2 && !w && queued == 0) {
+                // unroll iterations replayed from the graph (those past the
+                // phase's end find no frame running: the last one retired all)
+                a.itc_graph = 1;
+                int rc = hbm_graph(ctx, cv, path, ws, a, unroll, a.cur);
+                if (rc != GF2L_OK) return rc;
+                GF2L_HIP(ctx, hipGraphLaunch(ctx->hbm_exec, s));
+                g += unroll;
+                it += unroll;
+                GF2L_HIP(ctx, hipMemcpyAsync(ws.h_count, ws.counts + (g & 1u), sizeof(uint32_t),
+                                             hipMemcpyDeviceToHost, s));
+                GF2L_HIP(ctx, hipStreamSynchronize(s));
+                running = ws.h_count[0];
+                if (running == 0) return GF2L_OK;
+                continue;
+            }
             GF2L_HIP(ctx, gf2l::launch_hbm_iteration(cv, ws, path, a, ctx->num_cus, ctx->tune, s,
                                                      pf.on ? pf.ev + 4 * queued : nullptr));
             if (phase == 0 && h_trace_c2v && it <= trace_iters) {
@@ -261,12 +332,17 @@ int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const fl
             }
             // count of frames still running after this iteration: every
             // iteration's when profiling (frames processed per launch), else
-            // the batch's last one, read back to stop launching early
-            const bool readback = queued + 1 >= batch || (it == n_it && phase + 1 == n_phase);
+            // the batch's last one, read back to stop launching early; the
+            // first iteration of a phase that continues in a graph is read
+            // back too (the graph starts a batch)
+            const bool to_graph = use_graph && final_phase && !w && it + 1 <= n_it;
+            const bool readback = queued + 1 >= batch || (it == n_it && final_phase) || to_graph;
             if (pf.on || readback)
                 GF2L_HIP(ctx, hipMemcpyAsync(ws.h_count + (pf.on ? queued : 0), ws.counts + ((g + 1) & 1u),
                                              sizeof(uint32_t), hipMemcpyDeviceToHost, s));
             ++queued;
+            ++g;
+            ++it;
             if (!readback) continue;
             GF2L_HIP(ctx, hipStreamSynchronize(s));
             if (pf.on) {
@@ -279,12 +355,13 @@ int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const fl
             running = ws.h_count[pf.on ? queued - 1 : 0];
             queued = 0;
             if (running == 0) {
+                const uint32_t done = it - 1;  // the iteration that retired the last frame
                 if (phase == 0 && h_trace_c2v)
-                    for (uint32_t t = it + 1; t <= trace_iters; ++t) {
+                    for (uint32_t t = done + 1; t <= trace_iters; ++t) {
                         std::memcpy(h_trace_c2v + (size_t)(t - 1) * F * E,
-                                    h_trace_c2v + (size_t)(it - 1) * F * E, (size_t)F * E * sizeof(float));
+                                    h_trace_c2v + (size_t)(done - 1) * F * E, (size_t)F * E * sizeof(float));
                         std::memcpy(h_trace_bel + (size_t)(t - 1) * F * cv.n,
-                                    h_trace_bel + (size_t)(it - 1) * F * cv.n,
+                                    h_trace_bel + (size_t)(done - 1) * F * cv.n,
                                     (size_t)F * cv.n * sizeof(float));
                     }
                 return GF2L_OK;
@@ -465,6 +542,7 @@ void gf2l_destroy(gf2l_ctx *ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->hbm_exec) hipGraphExecDestroy(ctx->hbm_exec);
     free_workspace(ctx->ws);
     for (auto &e : ctx->prof.ev)
         if (e) hipEventDestroy(e);
@@ -683,6 +761,10 @@ int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value) {
             case GF2L_TUNE_LAYERED_KERNEL:
                 if (value < 0 || value > 2) return fail(ctx, GF2L_ERR_INVALID, "layered kernel out of [0,2]");
                 ctx->tune.layered_kernel = value;
+                return GF2L_OK;
+            case GF2L_TUNE_HBM_GRAPH:
+                if (value < 0 || value > 1) return fail(ctx, GF2L_ERR_INVALID, "hbm graph out of [0,1]");
+                ctx->tune.hbm_graph = value;
                 return GF2L_OK;
             case GF2L_TUNE_LDS_GROUPS:
                 if (value < 0 || value > 512) return fail(ctx, GF2L_ERR_INVALID, "lds groups out of [0,512]");

@@ -271,10 +271,16 @@ __global__ void __launch_bounds__(kSynThreads)
 syn_words_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t *__restrict__ count_ptr,
                  uint32_t *__restrict__ next_list, uint32_t *__restrict__ next_count,
                  const uint64_t *__restrict__ hb, uint32_t it_report, int last, int fb_phase,
+                 const uint32_t *__restrict__ itc, uint32_t itc_last, uint32_t itc_fb_from,
                  uint64_t *__restrict__ msg, uint32_t *__restrict__ iters,
                  uint8_t *__restrict__ conv, uint8_t *__restrict__ synd, uint8_t *__restrict__ fb) {
     extern __shared__ uint64_t sh[];
     const uint32_t count = *count_ptr;
+    if (itc) {  // replayed from a graph: the iteration number is on the device
+        it_report = *itc;
+        last = it_report >= itc_last;
+        fb_phase = it_report > itc_fb_from;
+    }
     const uint32_t words = cv.nb * cv.zw;
     const uint32_t kw = (cv.k + 63) / 64;
     for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
@@ -298,13 +304,18 @@ syn_words_kernel(CodeView cv, const uint32_t *__restrict__ list, const uint32_t 
 
 // Closes the holes the syndrome pass left in next_list (survivors stay in
 // list order) and stores the survivor count.  One workgroup: the list holds
-// at most one slot per frame.
+// at most one slot per frame.  itc (may be null): the device iteration
+// word, set to itc_next (a direct launch) or, with itc_next 0, advanced by
+// one (an iteration replayed from a graph).
 __global__ void __launch_bounds__(1024)
 list_compact_kernel(const uint32_t *__restrict__ count_ptr, uint32_t *__restrict__ next_list,
-                    uint32_t *__restrict__ next_count) {
+                    uint32_t *__restrict__ next_count, uint32_t *__restrict__ itc, uint32_t itc_next) {
     __shared__ uint32_t wave_kept[16];
     const uint32_t kept = compact_list(next_list, *count_ptr, wave_kept);
-    if (threadIdx.x == 0) *next_count = kept;
+    if (threadIdx.x == 0) {
+        *next_count = kept;
+        if (itc) *itc = itc_next ? itc_next : *itc + 1u;
+    }
 }
 
 // counts: [0..1] running-list counts; from kSchedOffset the check-kernel
@@ -795,9 +806,11 @@ hipError_t launch_hbm_iteration(const CodeView &cv, Workspace &ws, int path, con
     if (ev) hipEventRecord(ev[2], s);
     const size_t sh = (size_t)cv.nb * cv.zw * sizeof(uint64_t);
     hipLaunchKernelGGL(syn_words_kernel, dim3(g_syn), dim3(kSynThreads), sh, s, cv, list, count, next_list,
-                       next_count, ws.hb, a.it_report, a.last, a.fb_phase, a.msg, a.iters, a.conv,
-                       a.synd, a.fb);
-    hipLaunchKernelGGL(list_compact_kernel, dim3(1), dim3(1024), 0, s, count, next_list, next_count);
+                       next_count, ws.hb, a.it_report, a.last, a.fb_phase,
+                       a.itc_graph ? (const uint32_t *)a.itc : nullptr, a.itc_last, a.itc_fb_from, a.msg, a.iters,
+                       a.conv, a.synd, a.fb);
+    hipLaunchKernelGGL(list_compact_kernel, dim3(1), dim3(1024), 0, s, count, next_list, next_count, a.itc,
+                       a.itc_graph ? 0u : a.it_report + 1u);
     if (ev) hipEventRecord(ev[3], s);
     return hipGetLastError();
 }

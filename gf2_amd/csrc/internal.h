@@ -102,11 +102,15 @@ struct Tuning {
     int layered_cols = 0; // block-columns of beliefs the general layered kernel keeps in LDS (0: automatic)
     int layered_kernel = 0; // 0: staircase kernel where it applies, 1: general kernel, 2: staircase kernel
     int lds_groups = 0;  // thread groups per workgroup of the LDS path (0: automatic)
+    int hbm_graph = 0;   // 1: plain iterations of the HBM paths replayed from a captured graph
 };
 
 // Word offset of the item counters in Workspace::counts and its size (two
 // kernels x 8 XCDs, 64 words apart).
 constexpr uint32_t kSchedOffset = 64;
+// Word of Workspace::counts holding the iteration number of the next
+// syndrome pass (IterArgs::itc).
+constexpr uint32_t kItcWord = 2;
 constexpr uint32_t kCountsWords = kSchedOffset + 16 * 64;
 
 // Arguments of one iteration of an HBM path.
@@ -120,6 +124,13 @@ struct IterArgs {
     int fb_phase;                    // value written to fb[] for retiring frames
     const float *w_prev, *w_cur;     // per-block-edge weights (NULL = plain min-sum)
     int spa;                         // sum-product check rule (STREAM path, no weights)
+    // device iteration word (may be NULL): a direct launch sets it to
+    // it_report + 1 for the next iteration; an iteration replayed from a
+    // captured graph (itc_graph) takes its number from it and advances it,
+    // with last = (number >= itc_last), fb_phase = (number > itc_fb_from)
+    uint32_t *itc;
+    int itc_graph;
+    uint32_t itc_last, itc_fb_from;
     uint64_t *msg;
     uint32_t *iters;
     uint8_t *conv, *synd, *fb;
@@ -192,6 +203,10 @@ struct gf2l_ctx {
     // staging for the host-pointer entry point
     void *d_io = nullptr;
     size_t d_io_bytes = 0;
+    // plain iterations of the HBM paths captured as a graph, and what they
+    // were captured for (code, sizes, buffers, launch shape)
+    hipGraphExec_t hbm_exec = nullptr;
+    std::vector<uintptr_t> hbm_key;
 };
 
 namespace gf2l {
@@ -263,7 +278,7 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
 int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const float *d_llr, uint32_t F,
             uint32_t max_it, const float *d_w, uint32_t fb_it, bool fb_continue, uint64_t *d_msg,
             uint32_t *d_iters, uint8_t *d_conv, uint8_t *d_synd, uint8_t *d_fb, float *h_trace_c2v,
-            float *h_trace_bel, uint32_t trace_iters, float *d_tc, float *d_tb);
+            float *h_trace_bel, uint32_t trace_iters, float *d_tc, float *d_tb, bool graphs = true);
 // Decoder path (GF2L_PATH_*) the context would run a quasi-cyclic code on,
 // or -1 when a forced path cannot run it.
 int resolve_path(const gf2l_ctx *ctx, const CodeView &cv);

@@ -154,6 +154,7 @@ extern "C" int gf2l_decode_batch_device_noalloc(gf2l_ctx *ctx, const gf2l_code *
         ws.llr_perm = cv.dvb_q ? reinterpret_cast<float *>(base + c.llr) : nullptr;
         ws.h_count = h_counts;
         return gf2l::run_hbm(ctx, cv, path, ws, d_llr, F, max_it, nullptr, 0, false, d_msg_words, d_iterations,
-                             d_converged, d_syndrome_ok, nullptr, nullptr, nullptr, 0, nullptr, nullptr);
+                             d_converged, d_syndrome_ok, nullptr, nullptr, nullptr, 0, nullptr, nullptr,
+                             /*graphs=*/false);  // instantiating a graph allocates
     });
 }

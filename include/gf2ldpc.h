@@ -163,7 +163,15 @@ int gf2l_set_check_rule(gf2l_ctx *ctx, int rule);
  *                        and block-columns g, g + groups, ... with min(z, 512)
  *                        lanes each; at most 512 / min(z, 512) (larger
  *                        values are clamped).  0 (default): the count with
- *                        the fewest check plus variable passes. */
+ *                        the fewest check plus variable passes.
+ *   GF2L_TUNE_HBM_GRAPH  1: the plain iterations of the single-code HBM paths
+ *                        after the first of the final phase are replayed from
+ *                        a graph of GF2L_TUNE_GRAPH_ITERS iterations (rounded
+ *                        down to an even count), captured once per code,
+ *                        batch size and buffers and kept by the context;
+ *                        0 (default): launched one by one.  Never while
+ *                        profiling or tracing, never by the no-allocation
+ *                        entry point. */
 #define GF2L_TUNE_WG_PER_CU 1
 #define GF2L_TUNE_XCD_MAP 2
 #define GF2L_TUNE_GRAPH_ITERS 3
@@ -171,6 +179,7 @@ int gf2l_set_check_rule(gf2l_ctx *ctx, int rule);
 #define GF2L_TUNE_LAYERED_COLUMNS 5
 #define GF2L_TUNE_LAYERED_KERNEL 6
 #define GF2L_TUNE_LDS_GROUPS 7
+#define GF2L_TUNE_HBM_GRAPH 8
 int gf2l_set_tuning(gf2l_ctx *ctx, int key, int value);
 
 /* ---- codes ------------------------------------------------------------ */

@@ -365,3 +365,45 @@ def test_lds_path_lift_edge_cases(ctx, z):
     assert_same(decode_batch_arrays(code, llr, 30), orc.Code.from_qc(base, z).decode_batch(llr, 30, threads=8),
                 f"lds z={z}")
     ctx.set_path(_lib.PATH_AUTO)
+
+
+# ------------------------------------------- graph-replayed plain iterations
+@pytest.mark.parametrize("path,unroll", [("state", 4), ("state", 6), ("stream", 2)])
+def test_graph_replays_match_oracle(ctx, path, unroll):
+    """With GF2L_TUNE_HBM_GRAPH, plain iterations after the first run as
+    replays of a captured graph (cached on the context while the code, sizes
+    and buffers stay the same).
+    Several calls on the same device buffers, each replaying the graph, with
+    an eager decode (profiling on: iterations launched one by one) between
+    them; every result equals the oracle's."""
+    from gf2_amd.device import DeviceBuffer
+    spec = codes.n2304_r12()
+    base = spec.base_matrix()
+    F, kw = 64, (spec.k + 63) // 64
+    code = gpu_code(ctx, base, spec.z, path)
+    oc = orc.Code.from_qc(base, spec.z)
+    lib = _lib.load()
+    ctx.set_tuning(_lib.TUNE_GRAPH_ITERS, unroll)
+    ctx.set_tuning(_lib.TUNE_HBM_GRAPH, 1)
+    d_llr = DeviceBuffer(ctx, F * spec.n * 4)
+    outs = [DeviceBuffer(ctx, F * kw * 8), DeviceBuffer(ctx, F * 4), DeviceBuffer(ctx, F), DeviceBuffer(ctx, F)]
+    try:
+        for call, ebn0 in enumerate([1.2, 2.0, 1.6, 2.4]):
+            _, _, llr = frames.random_frames(base, spec.z, F, ebn0, seed=70 + call)
+            d_llr.upload(llr)
+            eager = call % 2 == 1
+            say("graph replays", path, unroll, call, "eager" if eager else "graph")
+            if eager:
+                _lib.check(ctx.handle, lib.gf2l_profile_enable(ctx.handle, 1))
+            _lib.check(ctx.handle, lib.gf2l_decode_batch_device(ctx.handle, code.handle, d_llr.ptr, F, 50,
+                                                                *[o.ptr for o in outs]))
+            ctx.synchronize()
+            if eager:
+                _lib.check(ctx.handle, lib.gf2l_profile_enable(ctx.handle, 0))
+            gpu = (outs[0].download(np.uint64, (F, kw)), outs[1].download(np.uint32, F),
+                   outs[2].download(np.uint8, F).astype(bool), outs[3].download(np.uint8, F).astype(bool))
+            assert_same(gpu, oc.decode_batch(llr, 50, threads=8), f"{path} call {call}")
+    finally:
+        ctx.set_tuning(_lib.TUNE_GRAPH_ITERS, 4)
+        ctx.set_tuning(_lib.TUNE_HBM_GRAPH, 0)
+        ctx.set_path(_lib.PATH_AUTO)
