@@ -28,7 +28,11 @@ The JSON line carries
   cpu_baseline    the structural C++ restatement of the reference's rayon
                   decode_batch (oracle/ref_port.cpp) on a bounded sample of
                   the same frames, on this host's CPU share and on one thread;
-                  beside it the same arithmetic on flat arrays
+                  beside it the same arithmetic on flat arrays, and the port
+                  on the reference benchmark's own workload
+  reference_workload  that workload (DVB-T2 normal r3/5, 202 error-free
+                  frames, benches/ldpc_throughput.rs:97-124) on the GPU,
+                  beside the reference's published CPU figure
 
     python bench.py [--steps K] [--warmup W] [--frames F] [--ebn0 X]
 """
@@ -65,6 +69,10 @@ MEASURED_VALU_PER_FI = {"layered_stair": 112206}
 # CPU threads of one GPU's share of the box
 CPU_SHARE = 16
 DEFAULT_EBN0 = 1.62
+# The reference's own decode benchmark (crates/gf2-coding/benches/ldpc_throughput.rs:97-124):
+# DVB-T2 normal rate-3/5 codewords sent as error-free +-10 LLRs, 202 blocks, published at
+# 8.29 Mbit/s on a 24-core CPU (crates/gf2-coding/docs/PARALLELIZATION.md:46-47).
+REF_FRAMES, REF_PUBLISHED_MBPS, REF_PUBLISHED_CORES = 202, 8.29, 24
 
 
 def parse():
@@ -85,8 +93,52 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(spec, base, llr, max_it, seconds):
-    """Time the structural restatement of the rayon path on a bounded sample."""
+def reference_frames():
+    """The reference benchmark's input: DVB-T2 normal r3/5 codewords of random
+    messages as error-free LLRs (+10 for a 0 bit, -10 for a 1 bit)."""
+    from gf2_amd import codes
+    from gf2_amd.channel import encode_dvb
+    s = codes.dvb_t2(1, 1)
+    info = np.random.default_rng(97).integers(0, 2, size=(REF_FRAMES, s.k), dtype=np.uint8)
+    cw = encode_dvb(s.table, s.n, s.k, s.q, info)
+    return s, info, np.where(cw == 1, -10.0, 10.0).astype(np.float32)
+
+
+def reference_workload_gpu(ctx, s, info, llr, max_it):
+    """The reference benchmark's workload decoded on the GPU (device entry
+    point, LLRs resident in HBM)."""
+    from gf2_amd import _lib, channel
+    from gf2_amd.device import DeviceBuffer
+    from gf2_amd.ldpc import CodeRate, LdpcCode
+    lib = _lib.load()
+    code = LdpcCode.dvb_t2_normal(CodeRate.RATE_3_5, ctx)
+    F, kw = llr.shape[0], (s.k + 63) // 64
+    d_llr = DeviceBuffer.from_array(ctx, llr)
+    outs = [DeviceBuffer(ctx, F * kw * 8), DeviceBuffer(ctx, F * 4), DeviceBuffer(ctx, F), DeviceBuffer(ctx, F)]
+
+    def run():
+        _lib.check(ctx.handle, lib.gf2l_decode_batch_device(ctx.handle, code.handle, d_llr.ptr, F, max_it,
+                                                            *[o.ptr for o in outs]))
+        ctx.synchronize()
+
+    run()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+    dt = (time.perf_counter() - t0) / reps
+    bits = channel.unpack_words(outs[0].download(np.uint64, (F, kw)), s.k)
+    return {"workload": f"DVB-T2 normal r3/5 (n={s.n}, k={s.k}), {F} codewords as error-free +-10 LLRs, "
+                        f"max {max_it} iterations (crates/gf2-coding/benches/ldpc_throughput.rs:97-124)",
+            "gpu_mbps": round(F * s.k / dt / 1e6, 1), "us_per_batch": round(dt * 1e6, 1),
+            "mean_iterations": round(float(outs[1].download(np.uint32, F).mean()), 3),
+            "decoded_correctly": int((bits == info).all(axis=1).sum()),
+            "published_cpu_mbps": REF_PUBLISHED_MBPS, "published_cpu_cores": REF_PUBLISHED_CORES}
+
+
+def cpu_baseline(spec, base, llr, max_it, seconds, ref=None):
+    """Time the structural restatement of the rayon path on a bounded sample
+    (and on the reference benchmark's own workload when ref is given)."""
     from oracle import oracle as orc
     nproc = os.cpu_count() or 1
     threads = max(1, min(CPU_SHARE, nproc))
@@ -121,6 +173,22 @@ def cpu_baseline(spec, base, llr, max_it, seconds):
     if n2:
         out["flat_array_c"] = {"value": n2 * spec.k / dt2 / 1e9, "cores": threads,
                                "sample": f"first {n2} frames, {dt2:.1f} s, oracle/gf2_oracle.c decode_batch"}
+    if ref is not None:  # the reference benchmark's workload on the port, all of it and one thread
+        s, rllr = ref
+        rc = orc.Code(s.m, s.n, orc.dvb_edges(s.table, s.n, s.k, s.q))
+        t0 = time.perf_counter()
+        rc.port_decode_batch(rllr, max_it, threads)
+        tr = time.perf_counter() - t0
+        n1 = 8
+        t0 = time.perf_counter()
+        rc.port_decode_batch(rllr[:n1], max_it, 1)
+        t1r = time.perf_counter() - t0
+        out["reference_workload"] = {
+            "value_mbps": round(rllr.shape[0] * s.k / tr / 1e6, 2), "cores": threads,
+            "single_thread_mbps": round(n1 * s.k / t1r / 1e6, 2),
+            "published_mbps": REF_PUBLISHED_MBPS, "published_cores": REF_PUBLISHED_CORES,
+            "sample": f"all {rllr.shape[0]} frames of the reference benchmark's workload on oracle/ref_port.cpp "
+                      f"with {threads} threads ({tr:.2f} s); one thread: first {n1} frames ({t1r:.2f} s)"}
     return out
 
 
@@ -298,10 +366,12 @@ def main():
         "roofline_plain": roofline({k: v for k, v in prof_p.items() if v[1]}),
         "cpu_baseline": None,
     }
+    rs, rinfo, rllr = reference_frames()
+    out["reference_workload"] = reference_workload_gpu(ctx, rs, rinfo, rllr, a.max_it)
     if not a.no_cpu:
         n_sample = min(F, 1024)
         llr = d_llr.download(np.float32, (n_sample, spec.n))
-        out["cpu_baseline"] = cpu_baseline(spec, base, llr, a.max_it, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(spec, base, llr, a.max_it, a.cpu_seconds, ref=(rs, rllr))
     print(json.dumps(out), flush=True)
     return 0
 
