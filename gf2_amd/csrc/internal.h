@@ -299,7 +299,8 @@ int ensure_buffers(gf2l_ctx *ctx, size_t frames, size_t c2v, size_t state, size_
 // Weights [iterations][nbe] (device), weighted iterations and code of a
 // learned decoder, and whether it is a flooding decoder whose fallback
 // restarts from the LLRs (nms.hip).
-const float *nms_weights(const gf2l_nms *q, uint32_t *iterations, const gf2l_code **code, bool *flooding_restart);
+const float *nms_weights(const gf2l_nms *q, uint32_t *iterations, const gf2l_code **code, bool *layered,
+                         bool *fb_continue);
 
 // Layered learned decoder (layered.hip): LDS bytes of a workgroup holding
 // X block-columns of beliefs, the largest X that fits lds_limit, and the

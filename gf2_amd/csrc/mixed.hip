@@ -18,8 +18,8 @@
 // iteration number of every running frame.  With learned decoders (one weight
 // set per code) the first T iterations are the weighted rule, and frames
 // still failing the syndrome after T restart from their LLRs together at
-// iteration T + 1 with plain min-sum (the single-code learned decoder's
-// fallback, gf2l_nms_decode_device).  An iteration is a check-update
+// iteration T + 1 with plain min-sum, or continue from the weighted state
+// (the single-code learned decoder's two fallbacks, gf2l_nms_decode_device).  An iteration is a check-update
 // and a variable-update launch per degree class present in the set
 // (registers sized per class) and one syndrome/retirement launch, each over
 // the running frames of every code; the running lists (one per code,
@@ -317,7 +317,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kVarWa
 // One 256-thread workgroup per running frame: syndrome, then retire the frame
 // (converged, or at the iteration cap) or keep its slot in its code's next list.
 // A frame failing at the end of the weighted iterations (it == T < cap)
-// stays on the list: the next iteration restarts it from its LLRs.
+// stays on the list: the next iteration restarts it from its LLRs or
+// continues from its state.
 // The last workgroup to finish advances the iteration counter and flips the
 // list parity.
 __global__ void __launch_bounds__(256) syn_mix_kernel(MixShared m) {
@@ -543,6 +544,21 @@ void enqueue_iteration(const gf2l_codeset *cs, const Shape &sh, hipStream_t s, u
     hipLaunchKernelGGL(mix_compact_kernel, dim3(m.ncodes), dim3(1024), 0, s, m);
 }
 
+// The continuing fallback's first iteration (t = T + 1): the check update
+// subtracts iteration T's weighted messages from the weighted phase's
+// beliefs; the variable update is plain.
+void enqueue_continue(const gf2l_codeset *cs, const Shape &sh, hipStream_t s, uint32_t t) {
+    const MixShared m = cs->m;
+    launch_cn<8, false, true>(m, cs->cn_mask[0], sh, s, t, 0u);
+    launch_cn<16, false, true>(m, cs->cn_mask[1], sh, s, t, 1u);
+    launch_cn<27, false, true>(m, cs->cn_mask[2], sh, s, t, 2u);
+    launch_vn<8, false>(m, cs->vn_mask[0], sh, s, t, 3u);
+    launch_vn<16, false>(m, cs->vn_mask[1], sh, s, t, 4u);
+    launch_vn<32, false>(m, cs->vn_mask[2], sh, s, t, 5u);
+    hipLaunchKernelGGL(syn_mix_kernel, dim3(sh.g_syn), dim3(256), sh.sh, s, m);
+    hipLaunchKernelGGL(mix_compact_kernel, dim3(m.ncodes), dim3(1024), 0, s, m);
+}
+
 // Captures `unroll` non-first iterations as a graph over the code set's
 // current workspace.
 int capture(gf2l_ctx *ctx, gf2l_codeset *cs, const Shape &sh, uint32_t unroll) {
@@ -613,22 +629,24 @@ int decode_mixed(gf2l_ctx *ctx, gf2l_codeset *cs, const gf2l_nms *const *decoder
     const uint32_t F = (uint32_t)frames, K = (uint32_t)cs->codes.size();
     // learned decoders: one per code of the set, same weighted iteration count
     uint32_t T = 0;
+    bool cont = false;  // the fallback continues from the weighted state (else restarts from the LLRs)
     const float *w[kMaxCodes] = {nullptr};
     if (decoders) {
         for (uint32_t c = 0; c < K; ++c) {
             if (!decoders[c]) return fail(ctx, GF2L_ERR_INVALID, "null learned decoder for code " + std::to_string(c));
             uint32_t Tc = 0;
             const gf2l_code *code = nullptr;
-            bool restart = false;
-            w[c] = nms_weights(decoders[c], &Tc, &code, &restart);
+            bool layered = false, cc = false;
+            w[c] = nms_weights(decoders[c], &Tc, &code, &layered, &cc);
             if (code != cs->codes[c])
                 return fail(ctx, GF2L_ERR_INVALID, "learned decoder " + std::to_string(c) + " is for another code");
-            if (!restart)
-                return fail(ctx, GF2L_ERR_UNSUPPORTED,
-                            "mixed batches take flooding learned decoders with the restarting fallback");
+            if (layered) return fail(ctx, GF2L_ERR_UNSUPPORTED, "mixed batches take flooding learned decoders");
             if (c > 0 && Tc != T)
                 return fail(ctx, GF2L_ERR_INVALID, "learned decoders differ in their iteration count");
+            if (c > 0 && cc != cont)
+                return fail(ctx, GF2L_ERR_INVALID, "learned decoders differ in their fallback mode");
             T = Tc;
+            cont = cc;
         }
     }
     const uint32_t max_total = T + tail_it;
@@ -723,7 +741,12 @@ int decode_mixed(gf2l_ctx *ctx, gf2l_codeset *cs, const gf2l_nms *const *decoder
         rc = running_frames(ctx, cs, &running);
         if (rc != GF2L_OK) return rc;
     }
-    if (running) enqueue_iteration<true, false>(cs, sh, s, T + 1);
+    if (running) {
+        if (cont && T)
+            enqueue_continue(cs, sh, s, T + 1);
+        else
+            enqueue_iteration<true, false>(cs, sh, s, T + 1);
+    }
     GF2L_HIP(ctx, hipGetLastError());
     if (running && tail_it > 1) {
         rc = capture(ctx, cs, sh, (uint32_t)ctx->tune.graph_iters);

@@ -769,10 +769,12 @@ int gf2l_nms_decode_device(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr,
 
 namespace gf2l {
 
-const float *nms_weights(const gf2l_nms *q, uint32_t *iterations, const gf2l_code **code, bool *flooding_restart) {
+const float *nms_weights(const gf2l_nms *q, uint32_t *iterations, const gf2l_code **code, bool *layered,
+                         bool *fb_continue) {
     *iterations = q->T;
     *code = q->code;
-    *flooding_restart = !q->layered && !q->fb_continue;
+    *layered = q->layered;
+    *fb_continue = q->fb_continue;
     return q->d_w;
 }
 

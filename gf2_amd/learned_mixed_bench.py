@@ -6,14 +6,16 @@ GPU and decoded twice in one call each:
   learned  gf2l_decode_mixed_nms_device: one learned decoder per code
            (gf2_amd/weights/nms_n<N>-k<K>_t10.npz, trained by
            python -m gf2_amd.train_nms --all-mixed), 10 weighted iterations,
-           then plain 50-iteration min-sum from the LLRs for frames that fail
+           then plain min-sum up to 50 iterations in all for frames that fail,
+           restarted from the LLRs or (--mode continue) continued from the
+           weighted state
   plain    gf2l_decode_mixed_device: plain 50-iteration min-sum
 A frame is in error when it did not converge, its syndrome failed, or its
 message differs from the sent information bits (ldpc_awgn.rs:157 plus
 undetected errors).  One JSON line per Eb/N0: per-code FER of both decoders,
 fallback fraction, and info Gbit/s of both calls.
 
-    python -m gf2_amd.learned_mixed_bench [--frames 16384] [--ebn0 0.8:2.0:0.1]
+    python -m gf2_amd.learned_mixed_bench [--frames 16384] [--ebn0 0.8:2.0:0.1] [--mode restart|continue]
 """
 from __future__ import annotations
 
@@ -40,6 +42,8 @@ def main(argv=None) -> int:
     ap.add_argument("--ebn0", default="0.8:2.0:0.1")
     ap.add_argument("--fallback", type=int, default=50)
     ap.add_argument("--seed", type=int, default=17)
+    ap.add_argument("--mode", choices=["restart", "continue"], default="restart",
+                    help="the learned decoders' fallback: from the LLRs or from the weighted state")
     ap.add_argument("--json", default=None, help="also write all points to this file")
     a = ap.parse_args(argv)
     specs = codes.mixed_twelve()
@@ -47,6 +51,8 @@ def main(argv=None) -> int:
     lib = _lib.load()
     gcodes = [LdpcCode(s.qc(), ctx) for s in specs]
     decs = [NmsDecoder.load(g, default_path(mixed_name(s))) for g, s in zip(gcodes, specs)]
+    for d in decs:
+        d.fallback = a.mode
     cs = CodeSet(gcodes)
     F = a.frames
     rng = np.random.default_rng(a.seed)
@@ -105,7 +111,7 @@ def main(argv=None) -> int:
         run(False)  # warm-up of the set's workspace and graph
         dt_p, it_p, _, fer_p = run(False)
         dt_l, it_l, fb_l, fer_l = run(True)
-        pt = {"ebn0_db": e, "frames": F,
+        pt = {"ebn0_db": e, "frames": F, "fallback_mode": a.mode,
               "learned_gbps": round(info_bits / dt_l / 1e9, 4), "plain_gbps": round(info_bits / dt_p / 1e9, 4),
               "learned_over_plain": round(dt_p / dt_l, 4),
               "learned_mean_iterations": round(float(it_l.mean()), 3),

@@ -458,12 +458,13 @@ int gf2l_decode_mixed_device(gf2l_ctx *ctx, gf2l_codeset *set, const uint16_t *c
                              const float *d_llr, uint32_t max_it, uint64_t *d_msg_words,
                              uint32_t *d_iterations, uint8_t *d_converged, uint8_t *d_syndrome_ok);
 /* Learned decoding of a mixed batch: decoders[c] is a flooding learned
- * decoder with the restarting fallback (gf2l_nms_create) of code c of the
- * set, all with the same number T of
- * weighted iterations.  Every frame gets exactly the results of
- * gf2l_nms_decode_device with its code's decoder: T weighted iterations with
- * early stop, then, for frames still failing the syndrome, plain min-sum
- * restarted from the LLRs for up to fallback_max_it iterations;
+ * decoder (gf2l_nms_create) of code c of the set, all with the same number T
+ * of weighted iterations and the same fallback mode.  Every frame gets
+ * exactly the results of gf2l_nms_decode_device with its code's decoder: T
+ * weighted iterations with early stop, then, for frames still failing the
+ * syndrome, plain min-sum for up to fallback_max_it iterations, restarted
+ * from the LLRs or (gf2l_nms_set_fallback continue) continued from the
+ * weighted state;
  * d_iterations counts weighted plus fallback iterations and d_fallback (may
  * be NULL) marks re-decoded frames.  The weighted rule is the reference's
  * alpha * min-sum (llr.rs:352-355) with one alpha per base edge and

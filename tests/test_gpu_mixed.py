@@ -198,6 +198,42 @@ def test_learned_mixed_matches_oracle(ctx, fb_it):
     assert n[ids].sum() == packed.size
 
 
+@pytest.mark.parametrize("fb_it", [0, 20])
+def test_learned_mixed_continue_matches_oracle(ctx, fb_it):
+    """Learned decoders with the continuing fallback in one mixed call: every
+    frame equals the oracle's flooding decode of T + fb_it iterations with
+    the code's weights for the first T and 1.0 after (multiplying by 1.0 is
+    exact), as for a single code (gf2l_nms_decode_device)."""
+    specs = [codes.n2304_r12()] + [codes.mixed_twelve()[i] for i in (0, 6, 9)]
+    gcodes = [LdpcCode(QuasiCyclicLdpc(s.base_matrix(), s.z), ctx) for s in specs]
+    cs = CodeSet(gcodes)
+    rng = np.random.default_rng(47)
+    T = 4
+    weights = [rng.uniform(0.6, 0.95, size=(T, int((s.base_matrix() >= 0).sum()))).astype(np.float32)
+               for s in specs]
+    decs = [NmsDecoder(g, T, weights=w, fallback="continue") for g, w in zip(gcodes, weights)]
+    ids, blocks = build_batch(specs, 2, seed=49, offsets=(0.2, 1.5, 3.5))
+    say("learned mixed continue", len(ids), "frames, fallback", fb_it)
+    packed = np.concatenate(blocks)
+    msg, msg_off, it, conv, synd, fb = decode_mixed_arrays(cs, ids, packed, fb_it, decoders=decs)
+    for c, spec in enumerate(specs):
+        sel = np.nonzero(ids == c)[0]
+        base = spec.base_matrix()
+        oc = orc.Code.from_qc(base, spec.z)
+        nbe = int((base >= 0).sum())
+        wfull = np.concatenate([weights[c], np.ones((fb_it, nbe), np.float32)])
+        rm, ri, rc, rs = oc.decode_batch(np.stack([blocks[i] for i in sel]), T + fb_it,
+                                         edge_w=frames.edge_weights(base, spec.z, wfull, oc), threads=8)
+        rfb = ((ri > T) | ~rc) if fb_it else np.zeros(rc.shape, dtype=bool)
+        for j, f in enumerate(sel):
+            got = (msg[msg_off[f]:msg_off[f + 1]], it[f], conv[f], synd[f], fb[f])
+            assert np.array_equal(got[0], rm[j]) and tuple(got[1:]) == (ri[j], rc[j], rs[j], rfb[j]), \
+                f"code {c} frame {f}: got it {got[1]} conv {got[2]} fb {got[4]}, want {ri[j]} {rc[j]} {rfb[j]}"
+    assert (conv & ~fb & (it <= T)).any()  # some frames converge in the weighted phase
+    if fb_it:
+        assert fb.any() and (conv & fb).any()  # and some converge in the continuation
+
+
 def test_learned_mixed_rejects_mismatched_decoders(ctx):
     specs = codes.mixed_twelve()[:2]
     gcodes = [LdpcCode(QuasiCyclicLdpc(s.base_matrix(), s.z), ctx) for s in specs]
@@ -210,6 +246,13 @@ def test_learned_mixed_rejects_mismatched_decoders(ctx):
     uneven = [NmsDecoder(gcodes[0], 4, init=0.8), NmsDecoder(gcodes[1], 5, init=0.8)]
     with pytest.raises(_lib.Gf2lError, match="iteration count"):
         decode_mixed_arrays(cs, ids, packed, 10, decoders=uneven)
+    modes = [NmsDecoder(gcodes[0], 4, init=0.8), NmsDecoder(gcodes[1], 4, init=0.8, fallback="continue")]
+    with pytest.raises(_lib.Gf2lError, match="fallback mode"):
+        decode_mixed_arrays(cs, ids, packed, 10, decoders=modes)
+    layered = [NmsDecoder(g, 4, init=0.8, schedule="layered") for g in gcodes]
+    with pytest.raises(_lib.Gf2lError) as e:
+        decode_mixed_arrays(cs, ids, packed, 10, decoders=layered)
+    assert e.value.code == _lib.GF2L_ERR_UNSUPPORTED
 
 
 @pytest.mark.parametrize("chunk", [1, 4])
