@@ -382,9 +382,10 @@ size_t lds_decode_bytes(const CodeView &cv) {
 // from its channel LLRs by fb_it plain iterations (phase 1), or with
 // fb_continue by fb_it plain iterations continued from the phase-0 beliefs
 // and (weighted) messages.  The plain instantiation (NMS false) carries no
-// weight code, which keeps it within 80 VGPRs (6 waves per SIMD).
+// weight code; for degrees up to 8 it is built for 80 VGPRs (6 waves per
+// SIMD, no spills).
 template <int DMAX, int DVMAX, bool NMS>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(!NMS && DMAX <= 8 && DVMAX <= 8 ? 6 : 1)))
 lds_decode_kernel(CodeView cv, LdsShape shp, const float *__restrict__ llr, uint32_t frames, uint32_t max_it,
                   const float *__restrict__ w, uint32_t fb_it, int fb_continue, uint64_t *__restrict__ msg,
                   uint32_t *__restrict__ iters, uint8_t *__restrict__ conv,
