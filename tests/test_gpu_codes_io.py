@@ -221,3 +221,38 @@ def test_sdr_api_matches_oracle(ctx):
         assert lib.gf2_ldpc_decode_frames(dec, llr.ctypes.data, 0, outs.ctypes.data, None) == 0
     finally:
         lib.gf2_ldpc_decoder_destroy(dec)
+
+
+def test_code_path_reports_the_decode_path(ctx):
+    """gf2l_code_path: the path a decode takes under the context's settings,
+    and the error a decode would return when the forced path or rule cannot
+    run the code."""
+    from gf2_amd import codes
+    from gf2_amd.ldpc import CodeRate
+    dvb = LdpcCode.dvb_t2_short(CodeRate.RATE_1_2, ctx)
+    qc_small = LdpcCode(codes.n2304_r12().qc(), ctx)
+    qc_big = LdpcCode(codes.n64800_r12().qc(), ctx)
+    sparse = LdpcCode.from_edges(3, 6, [(0, 0), (0, 1), (0, 3), (1, 1), (1, 2), (1, 4), (2, 0), (2, 2), (2, 5)], ctx)
+    try:
+        ctx.set_path(_lib.PATH_AUTO)
+        assert ctx.code_path(dvb) == _lib.PATH_STATE
+        assert ctx.code_path(qc_small) == _lib.PATH_LDS
+        assert ctx.code_path(qc_big) == _lib.PATH_STATE
+        assert ctx.code_path(sparse) == _lib.PATH_CSR
+        ctx.set_path(_lib.PATH_STREAM)
+        assert ctx.code_path(qc_big) == _lib.PATH_STREAM
+        for code in (dvb, qc_big):  # DVB codes run on the state kernels only; N=64800 does not fit LDS
+            ctx.set_path(_lib.PATH_LDS)
+            with pytest.raises(_lib.Gf2lError) as e:
+                ctx.code_path(code)
+            assert e.value.code == _lib.GF2L_ERR_UNSUPPORTED
+        ctx.set_path(_lib.PATH_AUTO)
+        ctx.set_check_rule(_lib.RULE_SPA)
+        assert ctx.code_path(qc_small) == _lib.PATH_STREAM
+        for code in (dvb, sparse):
+            with pytest.raises(_lib.Gf2lError) as e:
+                ctx.code_path(code)
+            assert e.value.code == _lib.GF2L_ERR_UNSUPPORTED
+    finally:
+        ctx.set_check_rule(_lib.RULE_MINSUM)
+        ctx.set_path(_lib.PATH_AUTO)
