@@ -187,13 +187,17 @@ int decode_device(gf2l_ctx *ctx, const gf2l_code *code, const float *d_llr, size
 int hbm_graph(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const IterArgs &base, uint32_t unroll,
               uint32_t parity) {
     const Tuning &t = ctx->tune;
-    std::vector<uintptr_t> key = {
-        (uintptr_t)cv.row_edge, (uintptr_t)path, base.frames, (uintptr_t)base.llr, (uintptr_t)base.msg,
+    // the code view by value (sizes, degrees and table pointers are all baked
+    // into the captured kernel arguments), then everything else the graph uses
+    std::vector<uintptr_t> key((sizeof(CodeView) + sizeof(uintptr_t) - 1) / sizeof(uintptr_t), 0);
+    std::memcpy(key.data(), &cv, sizeof(CodeView));
+    key.insert(key.end(), {
+        (uintptr_t)path, base.frames, (uintptr_t)base.llr, (uintptr_t)base.msg,
         (uintptr_t)base.iters, (uintptr_t)base.conv, (uintptr_t)base.synd, (uintptr_t)base.fb,
         (uintptr_t)ws.lists, (uintptr_t)ws.counts, (uintptr_t)ws.state, (uintptr_t)ws.c2v, (uintptr_t)ws.belief,
         (uintptr_t)ws.hb, (uintptr_t)ws.cap_frames, base.itc_last, base.itc_fb_from, (uintptr_t)base.spa, unroll,
         parity, (uintptr_t)(intptr_t)t.wg_per_cu, (uintptr_t)(intptr_t)t.chunk, (uintptr_t)(intptr_t)t.xcd_map,
-        (uintptr_t)ctx->num_cus};
+        (uintptr_t)ctx->num_cus});
     if (ctx->hbm_exec && ctx->hbm_key == key) return GF2L_OK;
     if (ctx->hbm_exec) hipGraphExecDestroy(ctx->hbm_exec);
     ctx->hbm_exec = nullptr;
