@@ -140,7 +140,8 @@ def _time(fn, repeats: int) -> float:
 
 def default_knobs(code: LdpcCode) -> tuple:
     """Knobs worth tuning for a code: the LDS path's thread groups when the
-    automatic path keeps the frame in LDS, else the HBM paths' knobs."""
+    automatic path keeps the frame in LDS, else the HBM paths' knobs.  Sets
+    the code's context to the automatic path to find out."""
     ctx = code.context
     ctx.set_path(_lib.PATH_AUTO)
     if ctx.code_path(code) == _lib.PATH_LDS:
