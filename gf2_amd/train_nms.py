@@ -8,6 +8,9 @@
         --lr 3e-3 --lo 1.4 --hi 1.8
     python -m gf2_amd.train_nms --code n64800 --schedule layered
         [--out gf2_amd/weights/nms_n64800_layered_t10.npz]
+    python -m gf2_amd.train_nms --code n2304 --resume <checkpoint> --steps 512 [--out <new file>]
+        (continues a checkpoint: its weights, Adam moments and step, recipe
+        and seed; the frames continue the stream where it stopped)
 
 Recipes (defaults per code):
   n2304   N=2304 K=1152 Z=96 QC-IRA: 1024 steps x 4096 frames = 2^22 frames,
@@ -27,9 +30,12 @@ BPSK/AWGN); every step draws fresh frames.
 from __future__ import annotations
 
 import argparse
+import json
 import os
 import sys
 import time
+
+import numpy as np
 
 from . import codes
 from .ldpc import Context, LdpcCode
@@ -92,6 +98,34 @@ def train_one(name: str, iterations: int, ctx: Context, seed: int, out: str = No
     return out
 
 
+def resume_one(name: str, path: str, steps: int, ctx: Context, out: str = None) -> str:
+    """Continues a checkpoint's training for `steps` more Adam steps with its
+    own recipe and seed: the weights, Adam moments and step come from the
+    file, and the frames continue the seeded stream where the checkpoint's
+    steps left it (frame step * batch onwards), so a resumed run equals an
+    uninterrupted one bit for bit."""
+    spec = recipe(name)["spec"]()
+    code = LdpcCode(spec.qc(), ctx)
+    dec = NmsDecoder.load(code, path)
+    with np.load(path, allow_pickle=False) as z:
+        info = json.loads(str(z["meta"]))
+    r, seed = info["recipe"], int(info["seed"])
+    done = dec.adam_state()[2]
+    t0 = time.perf_counter()
+    losses = train(dec, steps, r["batch"], r["lr"], r["lo"], r["hi"], seed=seed, first_frame=done * r["batch"],
+                   log_every=max(1, steps // 4))
+    dt = time.perf_counter() - t0
+    out = out or path
+    info.update({"final_loss": float(losses[-10:].mean()), "train_seconds": float(info.get("train_seconds", 0.0)) + dt,
+                 "recipe": dict(r, steps=int(r.get("steps", done)) + steps)})
+    for k in ("iterations", "base_edges", "n", "k", "shifts", "z", "schedule", "fallback"):
+        info.pop(k, None)
+    dec.save(out, meta=info)
+    print(f"[nms-train] {spec.name}: resumed at step {done}, {steps} more steps in {dt:.1f}s, "
+          f"loss {losses[-10:].mean():.4f}; saved {out}", flush=True)
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--code", default="n2304")
@@ -106,10 +140,16 @@ def main(argv=None) -> int:
     ap.add_argument("--init", type=float, default=None,
                     help="initial value of every weight (default 1.0; the layered recipes' own value)")
     ap.add_argument("--schedule", choices=["flooding", "layered"], default="flooding")
+    ap.add_argument("--resume", help="checkpoint to continue for --steps more steps (its recipe and seed)")
     ap.add_argument("--out")
     ap.add_argument("--out-dir", help="directory for the checkpoints (default gf2_amd/weights)")
     a = ap.parse_args(argv)
     ctx = Context(0)
+    if a.resume:
+        if not a.steps:
+            ap.error("--resume needs --steps")
+        resume_one(a.code, a.resume, a.steps, ctx, a.out)
+        return 0
     names = [mixed_name(s) for s in codes.mixed_twelve()] if a.all_mixed else [a.code]
     for name in names:
         out = a.out if not a.all_mixed else None

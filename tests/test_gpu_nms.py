@@ -226,3 +226,32 @@ def test_shipped_n64800_t50_beats_plain_bp(ctx):
     # measured on 4096 frames: 0 vs 161 errors, 18.2 vs 30.8 iterations
     assert res["learned"][0] <= res["plain"][0], res
     assert res["learned"][1] < 0.75 * res["plain"][1], res
+
+
+def test_training_resumes_bit_for_bit(ctx, tmp_path):
+    """Checkpoint and resume: 3 Adam steps, save (weights, Adam moments,
+    step), load, 3 more steps on the frames that follow in the seeded stream
+    -- the same weights and Adam state, bit for bit, as 6 steps in one run
+    (the weight gradient is reduced in a fixed order)."""
+    from gf2_amd import train_nms
+    from gf2_amd.nms import NmsDecoder, train
+    spec = codes.n2304_r12()
+    code = LdpcCode(spec.qc(), ctx)
+    batch, seed = 256, 21
+    full = NmsDecoder(code, 10, init=0.9)
+    train(full, 6, batch, 1e-2, 1.0, 3.0, seed=seed, first_frame=0)
+    half = NmsDecoder(code, 10, init=0.9)
+    train(half, 3, batch, 1e-2, 1.0, 3.0, seed=seed, first_frame=0)
+    path = str(tmp_path / "half.npz")
+    half.save(path, meta={"recipe": {"steps": 3, "batch": batch, "lr": 1e-2, "lo": 1.0, "hi": 3.0}, "seed": seed})
+    again = NmsDecoder.load(code, path)
+    assert again.adam_state()[2] == 3
+    train(again, 3, batch, 1e-2, 1.0, 3.0, seed=seed, first_frame=3 * batch)
+    for a, b in zip(again.adam_state(), full.adam_state()):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    assert np.array_equal(again.weights, full.weights)
+    # the CLI path: --resume continues the checkpoint's own recipe and seed
+    out = str(tmp_path / "resumed.npz")
+    train_nms.resume_one("n2304", path, 3, ctx, out)
+    resumed = NmsDecoder.load(code, out)
+    assert np.array_equal(resumed.weights, full.weights) and resumed.adam_state()[2] == 6
