@@ -169,7 +169,7 @@ struct LayeredBwdArgs {
     float *H;                 // [F][nbe][z] dL/d(message) carried to the previous iteration
     float *gpart;             // [T][nbe][F] per-frame weight gradients
     float scale;              // 1 / (T n F)
-    float *loss;              // summed loss (float atomics; logging only)
+    float *lpart;             // [F] loss of each frame
     uint32_t *ctr;            // [0] next frame (zeroed by the host)
 };
 

@@ -622,10 +622,10 @@ template <int MAXD>
 __global__ void __launch_bounds__(512) lnms_bwd_kernel(LayeredBwdArgs a) {
     extern __shared__ float sg[];
     __shared__ float red[MAXD][16];
+    __shared__ float lred[16];
     __shared__ uint32_t s_f;
     const CodeView &cv = a.cv;
     const uint32_t nw = (cv.n + 63) / 64;
-    float lsum = 0.0f;
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) s_f = atomicAdd(a.ctr, 1u);
@@ -637,6 +637,7 @@ __global__ void __launch_bounds__(512) lnms_bwd_kernel(LayeredBwdArgs a) {
         const uint64_t *cw = a.cw + (size_t)f * nw;
         for (uint32_t c = 0; c < cv.nb; ++c)
             for (uint32_t j = threadIdx.x; j < cv.z; j += blockDim.x) G.st(c, j, 0.0f);
+        float lsum = 0.0f;
         for (uint32_t t = a.T; t >= 1; --t) {
             // BCE of the beliefs after iteration t
             const float *tb = a.tape_bel + ((size_t)(t - 1) * a.F + f) * cv.nb * cv.zp;
@@ -656,14 +657,15 @@ __global__ void __launch_bounds__(512) lnms_bwd_kernel(LayeredBwdArgs a) {
                 __syncthreads();
             }
         }
-    }
-    lsum = wave_sum(lsum);
-    if ((threadIdx.x & 63u) == 0) red[0][threadIdx.x >> 6] = lsum;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float s = 0.0f;
-        for (uint32_t w = 0; w < blockDim.x / 64; ++w) s += red[0][w];
-        atomicAdd(a.loss, s);
+        // the frame's loss, reduced in a fixed order whichever workgroup took it
+        lsum = wave_sum(lsum);
+        if ((threadIdx.x & 63u) == 0) lred[threadIdx.x >> 6] = lsum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float s = 0.0f;
+            for (uint32_t w = 0; w < blockDim.x / 64; ++w) s += lred[w];
+            a.lpart[f] = s;
+        }
     }
 }
 

@@ -45,6 +45,7 @@ struct gf2l_nms {
     float *gpart = nullptr;          // [T][nbe][F]
     float *grad = nullptr;           // [T][nbe]
     float *loss = nullptr;           // 1
+    float *lpart = nullptr;          // loss partials, summed in a fixed order
     uint32_t *list = nullptr;        // [F] identity list
     uint32_t *count = nullptr;       // 4 words: running count; layered frame counters
     uint32_t *l_state = nullptr;     // layered training forward: [F][mb][zp][3] check states
@@ -61,13 +62,14 @@ __device__ __forceinline__ float wave_sum(float v) {
 }
 
 // Variable pass of iteration t: dL/dbelief_t[v] = BCE'(belief_t[v]) * scale
-// + sum of dL/dv2c_t over v's edges (absent at t = T).  Also accumulates the
-// loss (logging only: float atomics).
+// + sum of dL/dv2c_t over v's edges (absent at t = T).  Also writes the loss
+// of the items this workgroup took (a fixed stride over the grid) to
+// lpart[blockIdx.x], for a fixed-order sum.
 template <int MAXD>
 __global__ void __launch_bounds__(512)
 vn_bwd_kernel(CodeView cv, uint32_t frames, const float *__restrict__ bel_t,
               const uint64_t *__restrict__ cw, const float *__restrict__ g_cur, int has_gcur,
-              float *__restrict__ g_bel, float scale, float *__restrict__ loss) {
+              float *__restrict__ g_bel, float scale, float *__restrict__ lpart) {
     __shared__ float red[8];
     const uint32_t items = frames * cv.nb;
     const uint32_t nw = (cv.n + 63) / 64;
@@ -110,7 +112,7 @@ vn_bwd_kernel(CodeView cv, uint32_t frames, const float *__restrict__ bel_t,
     if (threadIdx.x == 0) {
         float t = 0.0f;
         for (uint32_t w = 0; w < blockDim.x / 64; ++w) t += red[w];
-        atomicAdd(loss, t);
+        lpart[blockIdx.x] = t;
     }
 }
 
@@ -276,17 +278,24 @@ void free_train(gf2l_nms *q) {
     hipFree(q->gpart);
     hipFree(q->grad);
     hipFree(q->loss);
+    hipFree(q->lpart);
     hipFree(q->list);
     hipFree(q->count);
     hipFree(q->l_state);
     hipFree(q->l_bel);
     q->tape_state = nullptr;
     q->tape_bel = nullptr;
-    q->g_a = q->g_b = q->g_bel = q->gpart = q->grad = q->loss = nullptr;
+    q->g_a = q->g_b = q->g_bel = q->gpart = q->grad = q->loss = q->lpart = nullptr;
     q->list = q->count = nullptr;
     q->l_state = nullptr;
     q->l_bel = nullptr;
     q->cap = 0;
+}
+
+// Loss partials: one per frame (layered backward) or one per workgroup and
+// iteration (flooding variable pass, at most 4 workgroups per CU).
+size_t loss_parts(const gf2l_ctx *ctx, size_t T, size_t F) {
+    return std::max<size_t>(F, T * (size_t)ctx->num_cus * 4);
 }
 
 int ensure_train(gf2l_ctx *ctx, gf2l_nms *q, size_t F) {
@@ -303,6 +312,7 @@ int ensure_train(gf2l_ctx *ctx, gf2l_nms *q, size_t F) {
     GF2L_HIP(ctx, hipMalloc(&q->gpart, T * cv.nbe * F * sizeof(float)));
     GF2L_HIP(ctx, hipMalloc(&q->grad, T * cv.nbe * sizeof(float)));
     GF2L_HIP(ctx, hipMalloc(&q->loss, sizeof(float)));
+    GF2L_HIP(ctx, hipMalloc(&q->lpart, loss_parts(ctx, T, F) * sizeof(float)));
     GF2L_HIP(ctx, hipMalloc(&q->list, F * sizeof(uint32_t)));
     GF2L_HIP(ctx, hipMalloc(&q->count, 4 * sizeof(uint32_t)));
     if (q->layered) {
@@ -352,7 +362,6 @@ int layered_forward_backward(gf2l_ctx *ctx, gf2l_nms *q, const float *d_llr, con
     if (rc != GF2L_OK) return rc;
     hipStream_t s = ctx->stream;
     GF2L_HIP(ctx, hipMemsetAsync(q->count, 0, 4 * sizeof(uint32_t), s));
-    GF2L_HIP(ctx, hipMemsetAsync(q->loss, 0, sizeof(float), s));
     LayeredArgs a{};
     a.cv = cv;
     a.llr = d_llr;
@@ -382,10 +391,11 @@ int layered_forward_backward(gf2l_ctx *ctx, gf2l_nms *q, const float *d_llr, con
     b.H = q->g_a;
     b.gpart = q->gpart;
     b.scale = 1.0f / ((float)T * (float)cv.n * (float)F);
-    b.loss = q->loss;
+    b.lpart = q->lpart;
     b.ctr = q->count + 2;
     GF2L_HIP(ctx, launch_layered_bwd(b, ctx->num_cus, s));
     hipLaunchKernelGGL(grad_reduce_kernel, dim3(T * cv.nbe), dim3(256), 0, s, q->gpart, F, q->grad);
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3(1), dim3(256), 0, s, q->lpart, F, q->loss);
     GF2L_HIP(ctx, hipGetLastError());
     return GF2L_OK;
 }
@@ -521,7 +531,6 @@ int forward_backward(gf2l_ctx *ctx, gf2l_nms *q, const float *d_llr, const uint6
                                            t >= 2 ? q->d_w + (size_t)(t - 2) * cv.nbe : nullptr,
                                            q->d_w + (size_t)(t - 1) * cv.nbe, ctx->num_cus, ctx->tune, s));
     }
-    GF2L_HIP(ctx, hipMemsetAsync(q->loss, 0, sizeof(float), s));
     const uint32_t block = bwd_block(cv.z);
     const uint32_t gv = std::min<uint32_t>(F * cv.nb, (uint32_t)ctx->num_cus * 4u);
     const uint32_t gc = std::min<uint32_t>(F * cv.mb, (uint32_t)ctx->num_cus * 4u);
@@ -532,10 +541,10 @@ int forward_backward(gf2l_ctx *ctx, gf2l_nms *q, const float *d_llr, const uint6
         const float *bel_t = q->tape_bel + (t - 1) * bstride;
         if (cv.max_dv > 16)
             hipLaunchKernelGGL(vn_bwd_kernel<32>, dim3(gv), dim3(block), 0, s, cv, F, bel_t, d_cw, g_cur,
-                               has_gcur, q->g_bel, scale, q->loss);
+                               has_gcur, q->g_bel, scale, q->lpart + (size_t)(t - 1) * gv);
         else
             hipLaunchKernelGGL(vn_bwd_kernel<16>, dim3(gv), dim3(block), 0, s, cv, F, bel_t, d_cw, g_cur,
-                               has_gcur, q->g_bel, scale, q->loss);
+                               has_gcur, q->g_bel, scale, q->lpart + (size_t)(t - 1) * gv);
         GF2L_HIP(ctx, hipGetLastError());
         const uint32_t *st_prev = t >= 2 ? q->tape_state + (t - 2) * sstride : nullptr;
         const float *bel_prev = t >= 2 ? q->tape_bel + (t - 2) * bstride : nullptr;
@@ -552,6 +561,7 @@ int forward_backward(gf2l_ctx *ctx, gf2l_nms *q, const float *d_llr, const uint6
         std::swap(g_cur, g_prev);
     }
     hipLaunchKernelGGL(grad_reduce_kernel, dim3(T * cv.nbe), dim3(256), 0, s, q->gpart, F, q->grad);
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3(1), dim3(256), 0, s, q->lpart, T * gv, q->loss);
     GF2L_HIP(ctx, hipGetLastError());
     return GF2L_OK;
 }
