@@ -494,28 +494,39 @@ constexpr uint32_t kRetiredSlot = 0xffffffffu;
 // other slots in order, and returns how many remain.  Called by a whole
 // workgroup (blockDim.x a multiple of 64, at most 1024; wave_kept holds
 // blockDim.x / 64 entries): every write lands at or before the slot it came
-// from, and a chunk's writes start only after the whole chunk has been read.
+// from, so the slots of a batch of chunks can all be loaded before any of
+// the batch's writes (which never reach a later batch).  Loading eight chunks
+// at once pays one memory latency per batch instead of one per chunk.
 __device__ __forceinline__ uint32_t compact_list(uint32_t *__restrict__ list, uint32_t count,
                                                  uint32_t *wave_kept) {
+    constexpr int kBatch = 8;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t waves = blockDim.x >> 6;
     uint32_t base = 0;
-    for (uint32_t c0 = 0; c0 < count; c0 += blockDim.x) {
-        const uint32_t i = c0 + threadIdx.x;
-        const uint32_t v = i < count ? list[i] : kRetiredSlot;
-        const bool keep = v != kRetiredSlot;
-        const uint64_t kept = __ballot(keep);
-        if (lane == 0) wave_kept[wave] = (uint32_t)__popcll(kept);
-        __syncthreads();
-        uint32_t off = base, total = 0;
-        for (uint32_t w = 0; w < waves; ++w) {
-            const uint32_t n = wave_kept[w];
-            if (w < wave) off += n;
-            total += n;
+    for (uint64_t c0 = 0; c0 < count; c0 += kBatch * blockDim.x) {
+        uint32_t v[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            const uint64_t i = c0 + k * blockDim.x + threadIdx.x;
+            v[k] = i < count ? list[i] : kRetiredSlot;
         }
-        if (keep) list[off + (uint32_t)__popcll(kept & ((1ull << lane) - 1ull))] = v;
-        base += total;
-        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            if (c0 + k * blockDim.x >= count) break;  // uniform over the workgroup
+            const bool keep = v[k] != kRetiredSlot;
+            const uint64_t kept = __ballot(keep);
+            if (lane == 0) wave_kept[wave] = (uint32_t)__popcll(kept);
+            __syncthreads();
+            uint32_t off = base, total = 0;
+            for (uint32_t w = 0; w < waves; ++w) {
+                const uint32_t n = wave_kept[w];
+                if (w < wave) off += n;
+                total += n;
+            }
+            if (keep) list[off + (uint32_t)__popcll(kept & ((1ull << lane) - 1ull))] = v[k];
+            base += total;
+            __syncthreads();
+        }
     }
     return base;
 }
