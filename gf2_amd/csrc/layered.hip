@@ -309,12 +309,16 @@ __device__ __forceinline__ void stair_layer(const LayeredArgs &a, const StairFra
     float m1 = __builtin_inff(), m2 = __builtin_inff();
     uint32_t idx = 0, lt = 0;
 #pragma unroll
-    for (int p = 0; p < D; ++p) {  // strict comparisons: the first of equal magnitudes keeps idx
+    for (int p = 0; p < D; ++p) {
+        // The strict-comparison update (a < m1: m2 = m1, m1 = a; else a < m2:
+        // m2 = a) as min / median: with m1 <= m2 the new m2 is the median of
+        // (m1, a, m2) and the new m1 is min(m1, a), the same values for any
+        // non-NaN a (|x| carries no -0); idx keeps the strict comparison, so
+        // the first of equal magnitudes keeps it.
         const float av = fabsf(x[p]);
-        const bool l1 = av < m1, l2 = av < m2;
-        m2 = l1 ? m1 : (l2 ? av : m2);
-        m1 = l1 ? av : m1;
-        idx = l1 ? (uint32_t)p : idx;
+        idx = av < m1 ? (uint32_t)p : idx;
+        m2 = __builtin_amdgcn_fmed3f(m1, av, m2);
+        m1 = __builtin_fminf(m1, av);
         lt |= (x[p] < 0.0f ? 1u : 0u) << p;
     }
     const uint32_t sg = ((__popc(lt) & 1u) ? (1u << D) - 1u : 0u) ^ lt;  // output sign bits
