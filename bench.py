@@ -65,7 +65,7 @@ MEASURED_TRAFFIC_PER_FI = {"layered_stair": 1065639, "layered_general": 1832398,
 # instructions per frame-iteration measured with rocprofv3 SQ_INSTS_VALU on
 # this workload (profiles/n64800_layered_bench.md).
 VALU_PEAK_GINST = 256 * 4 * 2.4 / 4
-MEASURED_VALU_PER_FI = {"layered_stair": 112206}
+MEASURED_VALU_PER_FI = {"layered_stair": 105758}
 # CPU threads of one GPU's share of the box
 CPU_SHARE = 16
 DEFAULT_EBN0 = 1.62
