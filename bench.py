@@ -234,7 +234,7 @@ def main():
             step()
         ctx.synchronize()
         if profile:
-            _lib.check(ctx.handle, lib.gf2l_profile_enable(ctx.handle, 1))
+            _lib.check(ctx.handle, lib.gf2l_profile_enable(ctx.handle, profile))
         t0 = time.perf_counter()
         for _ in range(a.steps):
             step()
@@ -256,7 +256,7 @@ def main():
         return dt, it, fe, fb, prof
 
     gbps = lambda dt: a.steps * F * spec.k / dt / 1e9
-    dt_p, it_p, fe_p, _, prof_p = run(plain_step, True)
+    dt_p, it_p, fe_p, _, prof_p = run(plain_step, _lib.PROFILE_ALL)
     flooding = None
     if a.flooding_weights and os.path.exists(a.flooding_weights):
         fdec = NmsDecoder.load(code, a.flooding_weights)
@@ -268,7 +268,10 @@ def main():
                               "mean_iterations": round(float(it_f.mean()), 3),
                               "fallback_fraction": round(float(fb_f.mean()), 4),
                               "over_plain": round(dt_p / dt_f, 4)}
-    dt_l, it_l, fe_l, fb_l, prof_l = run(nms_step(learned), True)
+    # events around the layered kernel only: the plain continuation after it
+    # runs as in an unprofiled decode (per-iteration events and readbacks
+    # cost about 0.2 ms per step)
+    dt_l, it_l, fe_l, fb_l, prof_l = run(nms_step(learned), _lib.PROFILE_LAYERED)
 
     cv_n, M, z = spec.n, spec.n - spec.k, spec.z
     nb = spec.n // z

@@ -39,6 +39,9 @@ pub const GF2L_KERNEL_VARIABLE: c_int = 1;
 pub const GF2L_KERNEL_SYNDROME: c_int = 2;
 pub const GF2L_KERNEL_LAYERED: c_int = 3;
 
+pub const GF2L_PROFILE_ALL: c_int = 1;
+pub const GF2L_PROFILE_LAYERED: c_int = 2;
+
 pub const GF2L_PATH_AUTO: c_int = 0;
 pub const GF2L_PATH_LDS: c_int = 1;
 pub const GF2L_PATH_STREAM: c_int = 2;

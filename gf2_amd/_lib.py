@@ -42,6 +42,8 @@ KERNEL_CHECK = 0
 KERNEL_VARIABLE = 1
 KERNEL_SYNDROME = 2
 KERNEL_LAYERED = 3
+PROFILE_ALL = 1
+PROFILE_LAYERED = 2
 
 FALLBACK_RESTART = 0
 FALLBACK_CONTINUE = 1

@@ -253,7 +253,8 @@ int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const fl
     }
     const uint32_t batch = h_trace_c2v ? 1u : (uint32_t)std::max(1, std::min(ctx->tune.graph_iters, kProfileSlots));
     const uint32_t unroll = std::max(2u, (uint32_t)std::min(ctx->tune.graph_iters, kProfileSlots) & ~1u);
-    const bool use_graph = graphs && ctx->tune.hbm_graph && !h_trace_c2v && !pf.on;
+    const bool ph = pf.on && pf.all;  // time these launches
+    const bool use_graph = graphs && ctx->tune.hbm_graph && !h_trace_c2v && !ph;
     uint32_t running = F;  // frames entering the first iteration of the current batch
     uint32_t queued = 0;   // iterations enqueued since the last readback
     // per-kernel times of the iterations of one batch (stream synchronized)
@@ -321,7 +322,7 @@ int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const fl
                 continue;
             }
             GF2L_HIP(ctx, gf2l::launch_hbm_iteration(cv, ws, path, a, ctx->num_cus, ctx->tune, s,
-                                                     pf.on ? pf.ev + 4 * queued : nullptr));
+                                                     ph ? pf.ev + 4 * queued : nullptr));
             if (phase == 0 && h_trace_c2v && it <= trace_iters) {
                 const float *src_c = ws.c2v, *src_b = ws.belief;
                 if (path == gf2l::kState) {
@@ -341,22 +342,22 @@ int run_hbm(gf2l_ctx *ctx, const CodeView &cv, int path, Workspace &ws, const fl
             // back too (the graph starts a batch)
             const bool to_graph = use_graph && final_phase && !w && it + 1 <= n_it;
             const bool readback = queued + 1 >= batch || (it == n_it && final_phase) || to_graph;
-            if (pf.on || readback)
-                GF2L_HIP(ctx, hipMemcpyAsync(ws.h_count + (pf.on ? queued : 0), ws.counts + ((g + 1) & 1u),
+            if (ph || readback)
+                GF2L_HIP(ctx, hipMemcpyAsync(ws.h_count + (ph ? queued : 0), ws.counts + ((g + 1) & 1u),
                                              sizeof(uint32_t), hipMemcpyDeviceToHost, s));
             ++queued;
             ++g;
             ++it;
             if (!readback) continue;
             GF2L_HIP(ctx, hipStreamSynchronize(s));
-            if (pf.on) {
+            if (ph) {
                 const int rc = collect(queued);
                 if (rc != GF2L_OK) return rc;
                 // frames entering each iteration of the batch
                 pf.frame_iters += running;
                 for (uint32_t b = 0; b + 1 < queued; ++b) pf.frame_iters += ws.h_count[b];
             }
-            running = ws.h_count[pf.on ? queued - 1 : 0];
+            running = ws.h_count[ph ? queued - 1 : 0];
             queued = 0;
             if (running == 0) {
                 const uint32_t done = it - 1;  // the iteration that retired the last frame
@@ -586,6 +587,7 @@ int gf2l_profile_enable(gf2l_ctx *ctx, int on) {
         pf.frame_iters = 0;
         pf.layered_frame_iters = 0;
         pf.on = on != 0;
+        pf.all = on != GF2L_PROFILE_LAYERED;
         return GF2L_OK;
     });
 }

@@ -84,6 +84,7 @@ constexpr int kProfileSlots = 64;
 // iterations enqueued between two readbacks).
 struct Profile {
     bool on = false;
+    bool all = true;                  // every launch; false: the layered kernel only
     hipEvent_t ev[4 * kProfileSlots] = {};
     hipEvent_t lev[2] = {};           // around the layered kernel
     double ms[4] = {0, 0, 0, 0};      // check, variable, syndrome, layered kernels

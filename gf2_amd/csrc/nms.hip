@@ -464,6 +464,7 @@ int layered_decode(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr, uint32_
     // kernels: beliefs and check states come from the layered decoder, whose
     // last messages were weighted by w_T)
     const uint32_t batch = (uint32_t)std::max(1, std::min(ctx->tune.graph_iters, kProfileSlots));
+    const bool ph = pf.on && pf.all;  // time the continuation's launches
     uint32_t queued = 0;
     for (uint32_t u = 1; u <= fb_it; ++u) {
         IterArgs it{};
@@ -482,15 +483,15 @@ int layered_decode(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr, uint32_
         it.synd = d_synd;
         it.fb = d_fb;
         GF2L_HIP(ctx, launch_hbm_iteration(cv, ws, kState, it, ctx->num_cus, ctx->tune, s,
-                                           pf.on ? pf.ev + 4 * queued : nullptr));
+                                           ph ? pf.ev + 4 * queued : nullptr));
         const bool readback = queued + 1 >= batch || u == fb_it;
-        if (pf.on || readback)
-            GF2L_HIP(ctx, hipMemcpyAsync(ws.h_count + (pf.on ? queued : 0), ws.counts + (u & 1u), sizeof(uint32_t),
+        if (ph || readback)
+            GF2L_HIP(ctx, hipMemcpyAsync(ws.h_count + (ph ? queued : 0), ws.counts + (u & 1u), sizeof(uint32_t),
                                          hipMemcpyDeviceToHost, s));
         ++queued;
         if (!readback) continue;
         GF2L_HIP(ctx, hipStreamSynchronize(s));
-        if (pf.on) {
+        if (ph) {
             for (uint32_t b = 0; b < queued; ++b)
                 for (int k = 0; k < 3; ++k) {
                     float ms = 0.0f;
@@ -501,7 +502,7 @@ int layered_decode(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr, uint32_
             pf.frame_iters += running;
             for (uint32_t b = 0; b + 1 < queued; ++b) pf.frame_iters += ws.h_count[b];
         }
-        running = ws.h_count[pf.on ? queued - 1 : 0];
+        running = ws.h_count[ph ? queued - 1 : 0];
         queued = 0;
         if (!running) break;
     }

@@ -62,11 +62,18 @@ int gf2l_synchronize(gf2l_ctx *ctx);
 /* The context's hipStream_t, for callers that order their own work on it. */
 void *gf2l_stream(gf2l_ctx *ctx);
 
-/* Per-kernel timing of the streaming path: HIP events are recorded around
- * every check / variable / syndrome launch on the context's stream.
- * Enabling (or re-enabling) resets the counters.  No reference counterpart:
- * the reference has no in-code timing (profiling was done with perf,
+/* Per-kernel timing: HIP events are recorded around kernel launches on the
+ * context's stream.  on = GF2L_PROFILE_ALL (or any value other than 0 and
+ * GF2L_PROFILE_LAYERED): every check / variable / syndrome launch of the HBM
+ * paths and every layered learned-decoder launch; the HBM paths then read the
+ * running count back after every iteration (frames processed per launch).
+ * on = GF2L_PROFILE_LAYERED: the layered kernel only, so the plain
+ * continuation after it runs as it does unprofiled.  0: off.  Enabling (or
+ * re-enabling) resets the counters.  No reference counterpart: the reference
+ * has no in-code timing (profiling was done with perf,
  * crates/gf2-coding/docs/LDPC_PERFORMANCE.md:80-96). */
+#define GF2L_PROFILE_ALL 1
+#define GF2L_PROFILE_LAYERED 2
 int gf2l_profile_enable(gf2l_ctx *ctx, int on);
 /* ms[3] and launches[3] for the check, variable and syndrome kernels;
  * frame_iterations = frames processed summed over iterations. */

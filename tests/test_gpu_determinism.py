@@ -6,6 +6,7 @@ take work changes from run to run; the results must not.  Each case decodes
 (or differentiates, or generates) the same input twice on one context and
 compares every output byte.
 """
+import ctypes
 import os
 import sys
 
@@ -139,3 +140,46 @@ def test_gradient_and_frames_twice(ctx):
         g1, loss1 = dec.gradient(l1, c1, F)
         g2, loss2 = dec.gradient(l1, c1, F)
         assert np.array_equal(g1, g2) and loss1 == loss2, schedule
+
+
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_profiling_levels_leave_results_unchanged(ctx, level):
+    """Profiling records events (and, for every launch, per-iteration
+    readbacks) but decodes the same bytes; the layered-only level times the
+    layered kernel and none of the continuation's launches."""
+    spec = codes.n64800_r12()
+    code = LdpcCode(spec.qc(), ctx)
+    dec = NmsDecoder.load(code, os.path.join(WEIGHTS, "nms_n64800_layered_t10.npz"))
+    F = 512
+    d_llr, _ = generate_frames(code, 36, 0, F, 1.45, 1.45, with_codewords=False)
+    outs = outputs(ctx, F, spec.k, extra=1)
+    lib = _lib.load()
+    dec.decode_device(d_llr, F, 50, *outs)
+    ctx.synchronize()
+    ref = snapshot(outs)
+    say("profiling level", level)
+    _lib.check(ctx.handle, lib.gf2l_profile_enable(ctx.handle, level))
+    try:
+        dec.decode_device(d_llr, F, 50, *outs)
+        ctx.synchronize()
+        got = snapshot(outs)
+        counts = {}
+        for k in (_lib.KERNEL_CHECK, _lib.KERNEL_VARIABLE, _lib.KERNEL_LAYERED):
+            ms, la, fi = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
+            _lib.check(ctx.handle, lib.gf2l_profile_read_kernel(ctx.handle, k, ctypes.byref(ms), ctypes.byref(la),
+                                                                ctypes.byref(fi)))
+            counts[k] = (la.value, fi.value, ms.value)
+    finally:
+        _lib.check(ctx.handle, lib.gf2l_profile_enable(ctx.handle, 0))
+    assert all(np.array_equal(x, y) for x, y in zip(ref, got))
+    fb = outs[4].download(np.uint8, F)
+    assert fb.any()  # the continuation ran
+    layered, check = counts[_lib.KERNEL_LAYERED], counts[_lib.KERNEL_CHECK]
+    if level == 0:
+        assert layered[0] == 0 and check[0] == 0
+    else:
+        assert layered[0] == 1 and layered[2] > 0
+        # iterations run by the layered kernel, summed over frames
+        its = outs[1].download(np.uint32, F)
+        assert layered[1] == int(np.minimum(its, dec.iterations).sum())
+        assert (check[0] > 0) == (level == _lib.PROFILE_ALL)
