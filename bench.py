@@ -256,7 +256,11 @@ def main():
         return dt, it, fe, fb, prof
 
     gbps = lambda dt: a.steps * F * spec.k / dt / 1e9
-    dt_p, it_p, fe_p, _, prof_p = run(plain_step, _lib.PROFILE_ALL)
+    # plain BP timed unprofiled (its Gbit/s), then once more with events around
+    # every launch (its roofline; profiling reads the running count back after
+    # every iteration)
+    dt_p, it_p, fe_p, _, _ = run(plain_step, 0)
+    _, _, _, _, prof_p = run(plain_step, _lib.PROFILE_ALL)
     flooding = None
     if a.flooding_weights and os.path.exists(a.flooding_weights):
         fdec = NmsDecoder.load(code, a.flooding_weights)
