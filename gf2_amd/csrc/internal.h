@@ -146,6 +146,7 @@ struct LayeredArgs {
     uint32_t X;              // block-columns whose beliefs live in LDS (0..X-1)
     int fallback;            // frames failing after T: 1 -> continuation list, 0 -> retire
     uint32_t *state;         // [F][mb][zp][3] compressed check states
+    const uint3 *zero_state; // staircase kernel: [mb][zp] zero states, read in place of iteration 1's
     float *belief;           // [F][nb][zp] beliefs of block-columns >= X (all at hand-over)
     uint32_t *ctr;           // [0] next frame, [1] layered frame-iterations run (zeroed by the host)
     uint32_t *list, *count;  // running list of the continuation and its length

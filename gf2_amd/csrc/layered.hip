@@ -257,13 +257,14 @@ __device__ __forceinline__ void lds_st(uint32_t addr, float v) { *(lds_float *)(
 // x * (-1)^s for s in {0, 1}: a sign flip, exact
 __device__ __forceinline__ float flip(float x, uint32_t s) { return __uint_as_float(__float_as_uint(x) ^ (s << 31)); }
 
-// Layer br (edges e0..e0+D-1) of iteration t (t2: t >= 2; cin: br > 0, the
-// caller's constant where it knows it).  (o, bo): this layer's check states
-// and belief of column kb+br, loaded ahead.  Code tables and weights come
-// from LDS: no scalar memory loads in the layer loop (their counter would
-// also wait for the LDS accesses).  In iteration 1 the old message is rebuilt
-// from a zero state with weight 0 (the table's previous weights are zero
-// then): x = b - (+0), which is b exactly (also for -0 and infinities).
+// Layer br (edges e0..e0+D-1) of iteration t (cin: br > 0, the caller's
+// constant where it knows it).  (o, bo): this layer's check states and belief
+// of column kb+br, loaded ahead.  Code tables and weights come from LDS: no
+// scalar memory loads in the layer loop (their counter would also wait for
+// the LDS accesses).  In iteration 1 the old message is rebuilt from a zero
+// state (read from zero rows, not from the frame's stale ones) with weight 0
+// (the table's previous weights are zero then): x = b - (+0), which is b
+// exactly (also for -0 and infinities).
 //
 // The check rule is the general kernel's (CheckAcc) specialised to D <= 8:
 // with fewer than nine inputs every input of the reference's min-sum falls in
@@ -272,7 +273,7 @@ __device__ __forceinline__ float flip(float x, uint32_t s) { return __uint_as_fl
 // w * (+-m) is computed as +-(w * m) (IEEE multiplication: the same value).
 template <int D>
 __device__ __forceinline__ void stair_layer(const LayeredArgs &a, const StairFrame &S, uint32_t br, uint32_t e0,
-                                            bool t2, bool cin, const uint3 o, const float bo, float &carry) {
+                                            bool cin, const uint3 o, const float bo, float &carry) {
     static_assert(D >= 2 && D <= 8, "the staircase layer's sign rule needs 2 <= D <= 8");
     const CodeView &cv = a.cv;
     uint32_t cb[D], sh[D];
@@ -285,8 +286,8 @@ __device__ __forceinline__ void stair_layer(const LayeredArgs &a, const StairFra
         wt[p] = __uint_as_float(r[2]);
         wp[p] = __uint_as_float(r[3]);
     }
-    const float m1o = t2 ? __uint_as_float(o.x) : 0.0f, m2o = t2 ? __uint_as_float(o.y) : 0.0f;
-    const uint32_t mo = t2 ? o.z : 0u, io = mo & 31u;
+    const float m1o = __uint_as_float(o.x), m2o = __uint_as_float(o.y);
+    const uint32_t mo = o.z, io = mo & 31u;
     float x[D];
     uint32_t ad[D];
 #pragma unroll
@@ -470,7 +471,9 @@ __global__ void __launch_bounds__(512) lnms_stair_kernel(LayeredArgs a) {
             const uint32_t tp = t + (wrap ? 1u : 0u), bq = wrap ? 0u : br + P;
             const float *src = tp == 1 ? S.llr + (S.kb + bq) * cv.z : S.gb + (S.kb + bq) * cv.zp;
             const uint32_t src_row = tp == 1 ? cv.z : cv.zp;
-            const bool t2 = t >= 2;
+            // iteration 1 reads zero states (the frame's rows hold another
+            // frame's or nothing yet)
+            const uint3 *st_src = tp == 1 ? a.zero_state : S.st;
 #pragma unroll
             for (int u = 0; u < P; ++u) {
                 const uint32_t r = br + u;
@@ -479,15 +482,15 @@ __global__ void __launch_bounds__(512) lnms_stair_kernel(LayeredArgs a) {
                 const bool hi = (mw >> (r & 63u)) & 1ull;
                 const bool cin = u > 0 || br > 0;  // a constant in all but the first copy
                 if (hi)
-                    stair_layer<DH>(a, S, r, e0, t2, cin, sst[u], sbo[u], carry);
+                    stair_layer<DH>(a, S, r, e0, cin, sst[u], sbo[u], carry);
                 else
-                    stair_layer<DH - 1>(a, S, r, e0, t2, cin, sst[u], sbo[u], carry);
+                    stair_layer<DH - 1>(a, S, r, e0, cin, sst[u], sbo[u], carry);
                 e0 += hi ? DH : DH - 1;
-                // loaded whether or not it will be used (past iteration T; the
-                // state in iteration 1): in bounds, one definition per slot;
-                // wave-uniform row bases, so the loads take the scalar-base form
+                // loaded whether or not it will be used (past iteration T): in
+                // bounds, one definition per slot; wave-uniform row bases, so
+                // the loads take the scalar-base form
                 sbo[u] = ldu(src + u * src_row, i);
-                sst[u] = ldu(S.st + (bq + u) * cv.zp, i);
+                sst[u] = ldu(st_src + (bq + u) * cv.zp, i);
                 // block-rows without a common information column need no
                 // barrier between them (parity beliefs and states are private)
                 if ((sw >> (r & 63u)) & 1ull) lds_barrier();

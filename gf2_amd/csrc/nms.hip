@@ -33,6 +33,7 @@ struct gf2l_nms {
     bool fb_continue = false; // the plain fallback continues from the learned state (layered: always)
     uint32_t X = 0;           // layered: block-columns whose beliefs a workgroup keeps in LDS
     float *d_w = nullptr;     // [T][nbe]
+    uint32_t *d_zero_state = nullptr;  // layered: [mb][zp][3] zeros, iteration 1's check states
     float *d_m = nullptr;     // Adam first moment
     float *d_v = nullptr;     // Adam second moment
     uint64_t step = 0;
@@ -433,6 +434,7 @@ int layered_decode(gf2l_ctx *ctx, const gf2l_nms *q, const float *d_llr, uint32_
     if (dh > 0) layered_stair_degree(q->code, a.stair_deg, a.stair_sync);
     a.fallback = fb_it > 0;
     a.state = ws.state;
+    a.zero_state = reinterpret_cast<const uint3 *>(q->d_zero_state);
     a.belief = ws.belief;
     a.ctr = ws.counts + 2;
     a.list = ws.lists;  // list 0: read by the first continuation iteration
@@ -598,10 +600,14 @@ int create_nms(gf2l_ctx *ctx, const gf2l_code *code, uint32_t iterations, const 
     if (e == hipSuccess) e = hipMemcpy(q->d_w, w.data(), nw * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(q->d_m, 0, nw * sizeof(float));
     if (e == hipSuccess) e = hipMemset(q->d_v, 0, nw * sizeof(float));
+    const size_t zbytes = 3ull * code->view.mb * code->view.zp * sizeof(uint32_t);
+    if (e == hipSuccess && layered) e = hipMalloc(&q->d_zero_state, zbytes);
+    if (e == hipSuccess && layered) e = hipMemset(q->d_zero_state, 0, zbytes);
     if (e != hipSuccess) {
         hipFree(q->d_w);
         hipFree(q->d_m);
         hipFree(q->d_v);
+        hipFree(q->d_zero_state);
         delete q;
         return fail(ctx, GF2L_ERR_HIP, std::string("nms create: ") + hipGetErrorString(e));
     }
@@ -656,6 +662,7 @@ void gf2l_nms_destroy(gf2l_nms *q) {
     hipFree(q->d_w);
     hipFree(q->d_m);
     hipFree(q->d_v);
+    hipFree(q->d_zero_state);
     delete q;
 }
 
