@@ -497,11 +497,14 @@ __global__ void __launch_bounds__(512) lnms_stair_kernel(LayeredArgs a) {
             }
             br += P;
             if (br == cv.mb) {  // end of iteration t
+                // hard decisions of the information columns: the z threads
+                // are the zw waves, so wave w packs word w of every column
+                // from its own lanes (the last wave's missing lanes give the
+                // zero bits above z)
                 const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-                for (uint32_t s = wave; s < kb * cv.zw; s += cv.zw) {
-                    const uint32_t c = s / cv.zw, j = (s - c * cv.zw) * 64u + lane;
-                    const uint64_t m = __ballot(j < cv.z && s_bel[c * cv.z + (j < cv.z ? j : 0)] < 0.0f);
-                    if (lane == 0) s_hb[s] = m;
+                for (uint32_t c = 0; c < kb; ++c) {
+                    const uint64_t m = __ballot(s_bel[c * cv.z + i] < 0.0f);
+                    if (lane == 0) s_hb[c * cv.zw + wave] = m;
                 }
                 lds_barrier();
                 if (!stair_unsat(cv, s_hb, s_rp, s_re, s_bad)) {
