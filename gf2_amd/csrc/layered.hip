@@ -402,8 +402,19 @@ __device__ __forceinline__ uint32_t stair_unsat(const CodeView &cv, const lds_u6
     return uni(r);  // the same in every lane: keeps the iteration state scalar
 }
 
+// Bits b .. b+31 of a 256-bit mask held in four words (b < 256; bits past
+// 255 read as zero): one extraction per group of layers, so a layer tests a
+// constant bit.
+__device__ __forceinline__ uint32_t mask_slice(uint64_t m0, uint64_t m1, uint64_t m2, uint64_t m3, uint32_t b) {
+    const uint32_t w = b >> 6, o = b & 63u;
+    const uint64_t lo = w == 0 ? m0 : w == 1 ? m1 : w == 2 ? m2 : m3;
+    const uint64_t hi = w == 0 ? m1 : w == 1 ? m2 : w == 2 ? m3 : 0ull;
+    return (uint32_t)((lo >> o) | (o ? hi << (64u - o) : 0ull));
+}
+
 template <int DH, int P>
 __global__ void __launch_bounds__(512) lnms_stair_kernel(LayeredArgs a) {
+    static_assert(P >= 1 && P <= 32, "a group's mask bits fit one 32-bit slice");
     extern __shared__ __align__(16) uint64_t smem[];
     __shared__ uint32_t s_f;
     __shared__ uint32_t s_bad_arr[8];
@@ -474,12 +485,12 @@ __global__ void __launch_bounds__(512) lnms_stair_kernel(LayeredArgs a) {
             // iteration 1 reads zero states (the frame's rows hold another
             // frame's or nothing yet)
             const uint3 *st_src = tp == 1 ? a.zero_state : S.st;
+            // the group's degree and barrier bits (bit u: block-row br + u)
+            const uint32_t dsl = mask_slice(dm0, dm1, dm2, dm3, br), ssl = mask_slice(bm0, bm1, bm2, bm3, br);
 #pragma unroll
             for (int u = 0; u < P; ++u) {
                 const uint32_t r = br + u;
-                const uint64_t mw = r < 64 ? dm0 : r < 128 ? dm1 : r < 192 ? dm2 : dm3;
-                const uint64_t sw = r < 64 ? bm0 : r < 128 ? bm1 : r < 192 ? bm2 : bm3;
-                const bool hi = (mw >> (r & 63u)) & 1ull;
+                const bool hi = (dsl >> u) & 1u;
                 const bool cin = u > 0 || br > 0;  // a constant in all but the first copy
                 if (hi)
                     stair_layer<DH>(a, S, r, e0, cin, sst[u], sbo[u], carry);
@@ -493,7 +504,7 @@ __global__ void __launch_bounds__(512) lnms_stair_kernel(LayeredArgs a) {
                 sst[u] = ldu(st_src + (bq + u) * cv.zp, i);
                 // block-rows without a common information column need no
                 // barrier between them (parity beliefs and states are private)
-                if ((sw >> (r & 63u)) & 1ull) lds_barrier();
+                if ((ssl >> u) & 1u) lds_barrier();
             }
             br += P;
             if (br == cv.mb) {  // end of iteration t
