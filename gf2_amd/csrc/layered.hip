@@ -257,8 +257,8 @@ __device__ __forceinline__ void lds_st(uint32_t addr, float v) { *(lds_float *)(
 // x * (-1)^s for s in {0, 1}: a sign flip, exact
 __device__ __forceinline__ float flip(float x, uint32_t s) { return __uint_as_float(__float_as_uint(x) ^ (s << 31)); }
 
-// Layer br (edges e0..e0+D-1) of iteration t (cin: br > 0, the caller's
-// constant where it knows it).  (o, bo): this layer's check states and belief
+// Layer br (edges e0..e0+D-1) of iteration t (cin: br > 0; last: br is the
+// last block-row; both the caller's constants where it knows them).  (o, bo): this layer's check states and belief
 // of column kb+br, loaded ahead.  Code tables and weights come from LDS: no
 // scalar memory loads in the layer loop (their counter would also wait for
 // the LDS accesses).  In iteration 1 the old message is rebuilt from a zero
@@ -273,7 +273,7 @@ __device__ __forceinline__ float flip(float x, uint32_t s) { return __uint_as_fl
 // w * (+-m) is computed as +-(w * m) (IEEE multiplication: the same value).
 template <int D>
 __device__ __forceinline__ void stair_layer(const LayeredArgs &a, const StairFrame &S, uint32_t br, uint32_t e0,
-                                            bool cin, const uint3 o, const float bo, float &carry) {
+                                            bool cin, bool last, const uint3 o, const float bo, float &carry) {
     static_assert(D >= 2 && D <= 8, "the staircase layer's sign rule needs 2 <= D <= 8");
     const CodeView &cv = a.cv;
     uint32_t cb[D], sh[D];
@@ -332,7 +332,7 @@ __device__ __forceinline__ void stair_layer(const LayeredArgs &a, const StairFra
         if (p == D - 1) {
             carry = v;  // column kb+br: to layer br+1, final now for the last block-row
             stu(S.gb + (S.kb + br) * cv.zp, S.i, v);
-            if (br == cv.mb - 1) {
+            if (last) {
                 const uint64_t m = __ballot(v < 0.0f);
                 if (lane == 0) S.hb[(cv.nb - 1) * cv.zw + wave] = m;
             }
@@ -491,11 +491,12 @@ __global__ void __launch_bounds__(512) lnms_stair_kernel(LayeredArgs a) {
             for (int u = 0; u < P; ++u) {
                 const uint32_t r = br + u;
                 const bool hi = (dsl >> u) & 1u;
-                const bool cin = u > 0 || br > 0;  // a constant in all but the first copy
+                const bool cin = u > 0 || br > 0;      // a constant in all but the first copy
+                const bool last = u == P - 1 && wrap;  // false in all but the last copy
                 if (hi)
-                    stair_layer<DH>(a, S, r, e0, cin, sst[u], sbo[u], carry);
+                    stair_layer<DH>(a, S, r, e0, cin, last, sst[u], sbo[u], carry);
                 else
-                    stair_layer<DH - 1>(a, S, r, e0, cin, sst[u], sbo[u], carry);
+                    stair_layer<DH - 1>(a, S, r, e0, cin, last, sst[u], sbo[u], carry);
                 e0 += hi ? DH : DH - 1;
                 // loaded whether or not it will be used (past iteration T): in
                 // bounds, one definition per slot; wave-uniform row bases, so
