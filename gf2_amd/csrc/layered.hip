@@ -374,6 +374,10 @@ __device__ __forceinline__ uint32_t stair_unsat(const CodeView &cv, const lds_u6
     uint64_t bad = 0;
     if (z >= 64) {
         for (uint32_t u = threadIdx.x; u < cv.mb * zw; u += nthr) {
+            // once a lane of the wave has found an unsatisfied check the
+            // frame is unsatisfied: the wave's other words are not needed
+            // (the ballot is uniform over the wave, so is the exit)
+            if (__ballot(bad != 0)) break;
             const uint32_t br = u / zw, w = u - br * zw;
             uint64_t acc = 0;
             for (uint32_t e = rp[br]; e < rp[br + 1]; ++e) {
@@ -385,6 +389,7 @@ __device__ __forceinline__ uint32_t stair_unsat(const CodeView &cv, const lds_u6
         }
     } else {
         for (uint32_t c = threadIdx.x; c < cv.m; c += nthr) {
+            if (__ballot(bad != 0)) break;  // as above
             const uint32_t br = c / z, i = c - br * z;
             uint32_t par = 0;
             for (uint32_t e = rp[br]; e < rp[br + 1]; ++e) {
