@@ -58,14 +58,14 @@ LDS_PEAK_GBPS = 256 * 128 * 2.4
 # profiles/n64800_layered_bench.md), scaled by what the counters read for a
 # plain copy of known size (FETCH_SIZE counts half the bytes read, WRITE_SIZE
 # the bytes written: the calibration in profiles/n64800_state_kernels_bench.md).
-MEASURED_TRAFFIC_PER_FI = {"layered_stair": 1035946, "layered_general": 1832398, "check": 1201488,
+MEASURED_TRAFFIC_PER_FI = {"layered_stair": 1036154, "layered_general": 1832398, "check": 1201488,
                            "variable": 1477744}
 # Vector-ALU issue: 256 CUs x 4 SIMDs, one 64-lane FP32/INT32 instruction per
 # SIMD every 4 clocks (16 lanes per clock) at 2.4 GHz; wave-level VALU
 # instructions per frame-iteration measured with rocprofv3 SQ_INSTS_VALU on
 # this workload (profiles/n64800_layered_bench.md).
 VALU_PEAK_GINST = 256 * 4 * 2.4 / 4
-MEASURED_VALU_PER_FI = {"layered_stair": 91960}
+MEASURED_VALU_PER_FI = {"layered_stair": 91482}
 # CPU threads of one GPU's share of the box
 CPU_SHARE = 16
 DEFAULT_EBN0 = 1.62
